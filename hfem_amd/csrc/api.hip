@@ -1,0 +1,453 @@
+// C ABI of the hfem_amd library (declared in include/hfem_amd.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hfem_amd.h"
+#include "hfem_device.h"
+
+namespace hfem {
+hipError_t launch_assemble(const Dims &, const Master &, const Phys &, const double *, const double *, const double *,
+                           const double *, const int *, const int *, int, int, double *, long, int, double *, double *,
+                           double *, double *, double *, double *, double *, hipStream_t);
+hipError_t launch_condense_fp64(int, int, int, int, double *, double *, const double *, double *, double *,
+                                const double *, double *, int *, int, hipStream_t);
+hipError_t launch_condense_mixed(int npv, int ncu, int na, int nb, int e0, int n, double *D, double *F,
+                                 const double *K, double *H, double *Ru, const double *Rh, double *FH, int *info,
+                                 float *work32, long wstride, int grid, int sweeps, hipStream_t s);
+long mixed_work_floats(int na, int nb);
+hipError_t launch_bsr_gather(int, int, int, int, int, const int *, const int *, const int *, const double *, double *,
+                             hipStream_t);
+hipError_t launch_face_gather(int, int, int, int, int, const int *, const int *, const double *, double *,
+                              const double *, double *, hipStream_t);
+hipError_t launch_sumsq(long, const double *, int, double *, int, hipStream_t);
+hipError_t launch_bsr_matvec(int, int, const int *, const int *, const double *, const double *, double *,
+                             hipStream_t);
+}  // namespace hfem
+
+using namespace hfem;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(x)                                                                                     \
+    do {                                                                                              \
+        hipError_t e_ = (x);                                                                          \
+        if (e_ != hipSuccess) return fail(-2, "%s failed: %s", #x, hipGetErrorString(e_));            \
+    } while (0)
+
+struct hfem_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hfem_dims hd{};
+    Dims d{};
+    Phys ph{};
+    bool has_master = false, has_phys = false, has_mesh = false, has_sol = false;
+    // master
+    double *shapvt = nullptr, *shapvg = nullptr, *shapft = nullptr, *shapfg = nullptr;
+    int *perm = nullptr, *invperm = nullptr;
+    // mesh
+    double *dgnodes = nullptr;
+    int *elcon = nullptr, *bf = nullptr, *rowstart = nullptr, *cols = nullptr, *einv = nullptr, *f2e = nullptr;
+    int *blkptr = nullptr, *blklist = nullptr;
+    int nblocks = 0;
+    // state
+    double *UDG = nullptr, *UH = nullptr, *SH = nullptr;
+    // element blocks
+    double *D = nullptr, *F = nullptr, *K = nullptr, *H = nullptr, *Ru = nullptr, *Rh = nullptr, *FH = nullptr,
+           *ru2 = nullptr;
+    int *info = nullptr;
+    // work
+    double *scratch = nullptr;
+    long sstride = 0;
+    int grid_asm = 0, grid_cond = 0;
+    float *work32 = nullptr;
+    long wstride = 0;
+    // trace system
+    double *Hg = nullptr, *Rg = nullptr, *r = nullptr, *partial = nullptr, *xv = nullptr, *yv = nullptr;
+    float ms_asm = 0, ms_cond = 0;
+};
+
+extern "C" const char *hfem_last_error(void) { return g_err.c_str(); }
+
+template <class T>
+static hipError_t dalloc(T **p, size_t n)
+{
+    return hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T));
+}
+
+extern "C" int hfem_create(hfem_ctx **out, const hfem_dims *hd, int device)
+{
+    if (!out || !hd) return fail(-1, "hfem_create: null argument");
+    *out = nullptr;
+    if (hd->nd != 2 && hd->nd != 3) return fail(-1, "hfem_create: nd must be 2 or 3 (got %d)", hd->nd);
+    if (hd->ncu != hd->nd + 2) return fail(-1, "hfem_create: compressible NS needs ncu = nd+2");
+    if (hd->npv <= 0 || hd->ngv <= 0 || hd->npf <= 0 || hd->ngf <= 0 || hd->nfe <= 0 || hd->ne <= 0 || hd->nf <= 0)
+        return fail(-1, "hfem_create: sizes must be positive");
+    if (hd->npv > 64 || hd->npf * hd->nfe > 128) return fail(-1, "hfem_create: element too large (npv<=64)");
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(-1, "hfem_create: device %d out of range (%d devices)", device, ndev);
+    HIPCHK(hipSetDevice(device));
+    hfem_ctx *c = new hfem_ctx();
+    c->device = device;
+    c->hd = *hd;
+    Dims &d = c->d;
+    d.nd = hd->nd; d.ncu = hd->ncu; d.nch = hd->ncu; d.nd1 = hd->nd + 1; d.nc = d.ncu * d.nd1;
+    d.npv = hd->npv; d.ngv = hd->ngv; d.npf = hd->npf; d.ngf = hd->ngf; d.nfe = hd->nfe;
+    d.ndf = d.npf * d.nfe; d.na = d.npv * d.ncu; d.nb = d.ndf * d.nch; d.np = d.ngf * d.nfe;
+    if (d.na > 1024) { delete c; return fail(-1, "hfem_create: na=%d exceeds 1024", d.na); }
+    const long ne = hd->ne;
+    hipError_t e = hipSuccess;
+#define ALLOC(p, n) if (e == hipSuccess) e = dalloc(&c->p, (size_t)(n))
+    ALLOC(D, ne * d.na * d.na);
+    ALLOC(F, ne * d.na * d.nb);
+    ALLOC(K, ne * d.nb * d.na);
+    ALLOC(H, ne * d.nb * d.nb);
+    ALLOC(Ru, ne * d.na);
+    ALLOC(Rh, ne * d.nb);
+    ALLOC(FH, ne * d.nb);
+    ALLOC(ru2, ne);
+    ALLOC(info, ne);
+    ALLOC(partial, 1024);
+    c->sstride = scratch_layout(d).total;
+    c->grid_asm = (int)std::min<long>(ne, 1024);
+    c->grid_cond = (int)std::min<long>(ne, 2048);
+    ALLOC(scratch, (long)c->grid_asm * c->sstride);
+    c->wstride = mixed_work_floats(d.na, d.nb);
+    ALLOC(work32, (long)c->grid_cond * c->wstride);
+#undef ALLOC
+    if (e == hipSuccess) e = hipStreamCreate(&c->stream);
+    for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
+    if (e != hipSuccess) {
+        hfem_destroy(c);
+        return fail(-2, "hfem_create: allocation failed: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return 0;
+}
+
+extern "C" int hfem_destroy(hfem_ctx *c)
+{
+    if (!c) return 0;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    void *ptrs[] = {c->shapvt, c->shapvg, c->shapft, c->shapfg, c->perm, c->invperm, c->dgnodes, c->elcon, c->bf,
+                    c->rowstart, c->cols, c->einv, c->f2e, c->blkptr, c->blklist, c->UDG, c->UH, c->SH, c->D, c->F,
+                    c->K, c->H, c->Ru, c->Rh, c->FH, c->ru2, c->info, c->scratch, c->work32, c->Hg, c->Rg, c->r,
+                    c->partial, c->xv, c->yv};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    for (auto &ev : c->ev)
+        if (ev) hipEventDestroy(ev);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+template <class T>
+static int upload(hfem_ctx *c, T **dst, const T *src, size_t n)
+{
+    if (!*dst) HIPCHK(dalloc(dst, n));
+    HIPCHK(hipMemcpyAsync(*dst, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int hfem_set_master(hfem_ctx *c, const double *shapvt, const double *shapvg, const double *shapft,
+                               const double *shapfg, const int *perm)
+{
+    if (!c || !shapvt || !shapvg || !shapft || !shapfg || !perm) return fail(-1, "hfem_set_master: null argument");
+    HIPCHK(hipSetDevice(c->device));
+    const Dims &d = c->d;
+    std::vector<int> inv((size_t)d.npv * d.nfe, -1);
+    for (int a = 0; a < d.nfe; a++)
+        for (int k = 0; k < d.npf; k++) {
+            int nn = perm[k + d.npf * a];
+            if (nn < 0 || nn >= d.npv) return fail(-1, "hfem_set_master: perm(%d,%d)=%d out of range", k, a, nn);
+            inv[nn + (size_t)d.npv * a] = k;
+        }
+    int rc;
+    if ((rc = upload(c, &c->shapvt, shapvt, (size_t)d.ngv * d.npv * d.nd1))) return rc;
+    if ((rc = upload(c, &c->shapvg, shapvg, (size_t)d.ngv * d.npv * d.nd1))) return rc;
+    if ((rc = upload(c, &c->shapft, shapft, (size_t)d.ngf * d.npf * d.nd))) return rc;
+    if ((rc = upload(c, &c->shapfg, shapfg, (size_t)d.ngf * d.npf * d.nd))) return rc;
+    if ((rc = upload(c, &c->perm, perm, (size_t)d.npf * d.nfe))) return rc;
+    if ((rc = upload(c, &c->invperm, inv.data(), inv.size()))) return rc;
+    c->has_master = true;
+    return 0;
+}
+
+extern "C" int hfem_set_physics(hfem_ctx *c, const double *param, const double *ui, double fc_u, double fc_q, int tdep)
+{
+    if (!c || !param || !ui) return fail(-1, "hfem_set_physics: null argument");
+    if (param[0] <= 1.0 || param[2] <= 0 || param[3] <= 0) return fail(-1, "hfem_set_physics: invalid gamma/Re/Pr");
+    for (int i = 0; i < 6; i++) c->ph.param[i] = param[i];
+    for (int i = 0; i < c->d.nch; i++) c->ph.ui[i] = ui[i];
+    c->ph.fc_u = fc_u;
+    c->ph.fc_q = fc_q;
+    c->ph.tdep = tdep;
+    c->has_phys = true;
+    return 0;
+}
+
+extern "C" int hfem_set_mesh(hfem_ctx *c, const double *dgnodes, const int *elcon, const int *bf, const int *t2f,
+                             const int *rowstart, const int *cols, int nblocks)
+{
+    if (!c || !dgnodes || !elcon || !bf || !t2f || !rowstart || !cols) return fail(-1, "hfem_set_mesh: null argument");
+    HIPCHK(hipSetDevice(c->device));
+    const Dims &d = c->d;
+    const int ne = c->hd.ne, nf = c->hd.nf;
+    if (rowstart[0] != 0 || rowstart[nf] != nblocks) return fail(-1, "hfem_set_mesh: rowstart inconsistent with nblocks");
+    // einv, face-to-element lists and the gather lists of every BSR block
+    std::vector<int> einv((size_t)ne * d.ndf, -1), f2e((size_t)nf * 4, -1);
+    std::vector<int> cnt(nblocks + 1, 0);
+    std::vector<int> kidx((size_t)ne * d.nfe * d.nfe);
+    for (int e = 0; e < ne; e++)
+        for (int a = 0; a < d.nfe; a++) {
+            const int i = t2f[e * d.nfe + a];
+            if (i < 0 || i >= nf) return fail(-1, "hfem_set_mesh: t2f[%d,%d]=%d out of range", e, a, i);
+            for (int m = 0; m < d.npf; m++) {
+                int g = elcon[(long)e * d.ndf + a * d.npf + m] - i * d.npf;
+                if (g < 0 || g >= d.npf) return fail(-1, "hfem_set_mesh: elcon of element %d face %d not on face %d", e, a, i);
+                einv[(long)e * d.ndf + a * d.npf + g] = m;
+            }
+            int side = f2e[4 * i] < 0 ? 0 : 1;
+            if (side == 1 && f2e[4 * i + 2] >= 0) return fail(-1, "hfem_set_mesh: face %d has more than two elements", i);
+            f2e[4 * i + 2 * side] = e;
+            f2e[4 * i + 2 * side + 1] = a;
+            for (int b = 0; b < d.nfe; b++) {
+                const int j = t2f[e * d.nfe + b];
+                int k = rowstart[i];
+                while (k < rowstart[i + 1] && cols[k] != j) k++;
+                if (k == rowstart[i + 1]) return fail(-1, "hfem_set_mesh: block (%d,%d) missing from the trace graph", i, j);
+                kidx[((size_t)e * d.nfe + a) * d.nfe + b] = k;
+                cnt[k + 1]++;
+            }
+        }
+    for (size_t q = 0; q < einv.size(); q++)
+        if (einv[q] < 0) return fail(-1, "hfem_set_mesh: elcon of element %d maps two nodes to one trace node", (int)(q / d.ndf));
+    for (int k = 0; k < nblocks; k++) cnt[k + 1] += cnt[k];
+    std::vector<int> list((size_t)cnt[nblocks] * 3), fillp(cnt.begin(), cnt.end() - 1);
+    for (int e = 0; e < ne; e++)
+        for (int a = 0; a < d.nfe; a++)
+            for (int b = 0; b < d.nfe; b++) {
+                int k = kidx[((size_t)e * d.nfe + a) * d.nfe + b];
+                int q = fillp[k]++;
+                list[3 * q] = e; list[3 * q + 1] = a; list[3 * q + 2] = b;
+            }
+    int rc;
+    if ((rc = upload(c, &c->dgnodes, dgnodes, (size_t)ne * d.npv * d.nd))) return rc;
+    if ((rc = upload(c, &c->elcon, elcon, (size_t)ne * d.ndf))) return rc;
+    if ((rc = upload(c, &c->bf, bf, (size_t)ne * d.nfe))) return rc;
+    if ((rc = upload(c, &c->rowstart, rowstart, (size_t)nf + 1))) return rc;
+    if ((rc = upload(c, &c->cols, cols, (size_t)nblocks))) return rc;
+    if ((rc = upload(c, &c->einv, einv.data(), einv.size()))) return rc;
+    if ((rc = upload(c, &c->f2e, f2e.data(), f2e.size()))) return rc;
+    if ((rc = upload(c, &c->blkptr, cnt.data(), cnt.size()))) return rc;
+    if ((rc = upload(c, &c->blklist, list.data(), list.size()))) return rc;
+    c->nblocks = nblocks;
+    const long bsz = (long)d.nch * d.npf;
+    if (!c->Hg) HIPCHK(dalloc(&c->Hg, (size_t)nblocks * bsz * bsz));
+    if (!c->Rg) HIPCHK(dalloc(&c->Rg, (size_t)nf * bsz));
+    if (!c->r) HIPCHK(dalloc(&c->r, (size_t)nf * bsz));
+    if (!c->xv) HIPCHK(dalloc(&c->xv, (size_t)nf * bsz));
+    if (!c->yv) HIPCHK(dalloc(&c->yv, (size_t)nf * bsz));
+    c->has_mesh = true;
+    return 0;
+}
+
+extern "C" int hfem_set_solution(hfem_ctx *c, const double *UDG, const double *UH, const double *SH)
+{
+    if (!c || !UDG || !UH) return fail(-1, "hfem_set_solution: null argument");
+    HIPCHK(hipSetDevice(c->device));
+    const Dims &d = c->d;
+    const long ne = c->hd.ne, nuh = (long)c->hd.nf * d.npf;
+    int rc;
+    if ((rc = upload(c, &c->UDG, UDG, (size_t)ne * d.npv * d.nc))) return rc;
+    if ((rc = upload(c, &c->UH, UH, (size_t)nuh * d.nch))) return rc;
+    if (SH) {
+        if ((rc = upload(c, &c->SH, SH, (size_t)ne * d.npv * d.nc))) return rc;
+    } else if (!c->SH) {
+        HIPCHK(dalloc(&c->SH, (size_t)ne * d.npv * d.nc));
+        HIPCHK(hipMemsetAsync(c->SH, 0, sizeof(double) * ne * d.npv * d.nc, c->stream));
+    }
+    c->has_sol = true;
+    return 0;
+}
+
+extern "C" int hfem_get_solution(hfem_ctx *c, double *UDG, double *UH)
+{
+    if (!c || !c->has_sol) return fail(-3, "hfem_get_solution: no solution set");
+    HIPCHK(hipSetDevice(c->device));
+    const Dims &d = c->d;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (UDG) HIPCHK(hipMemcpy(UDG, c->UDG, sizeof(double) * c->hd.ne * d.npv * d.nc, hipMemcpyDeviceToHost));
+    if (UH) HIPCHK(hipMemcpy(UH, c->UH, sizeof(double) * (long)c->hd.nf * d.npf * d.nch, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+static int check_range(hfem_ctx *c, int e0, int n, const char *who)
+{
+    if (!c) return fail(-1, "%s: null context", who);
+    if (e0 < 0 || n < 0 || (long)e0 + n > c->hd.ne) return fail(-1, "%s: element range [%d,%d) outside [0,%d)", who, e0, e0 + n, c->hd.ne);
+    return 0;
+}
+
+extern "C" int hfem_assemble(hfem_ctx *c, int e0, int n)
+{
+    int rc = check_range(c, e0, n, "hfem_assemble");
+    if (rc) return rc;
+    if (!c->has_master || !c->has_phys || !c->has_mesh || !c->has_sol)
+        return fail(-3, "hfem_assemble: master, physics, mesh and solution must be set first");
+    if (n == 0) return 0;
+    HIPCHK(hipSetDevice(c->device));
+    Master m{c->shapvt, c->shapvg, c->shapft, c->shapfg, c->perm, c->invperm};
+    const int grid = std::min(n, c->grid_asm);
+    HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    HIPCHK(launch_assemble(c->d, m, c->ph, c->dgnodes, c->UDG, c->UH, c->SH, c->elcon, c->bf, e0, n, c->scratch,
+                           c->sstride, grid, c->Ru, c->D, c->F, c->K, c->H, c->Rh, c->ru2, c->stream));
+    HIPCHK(hipEventRecord(c->ev[1], c->stream));
+    return 0;
+}
+
+extern "C" int hfem_condense(hfem_ctx *c, int e0, int n, int precision)
+{
+    int rc = check_range(c, e0, n, "hfem_condense");
+    if (rc) return rc;
+    if (precision != HFEM_FP64 && precision != HFEM_MIXED) return fail(-1, "hfem_condense: unknown precision %d", precision);
+    if (n == 0) return 0;
+    HIPCHK(hipSetDevice(c->device));
+    const Dims &d = c->d;
+    const int grid = std::min(n, c->grid_cond);
+    HIPCHK(hipEventRecord(c->ev[2], c->stream));
+    if (precision == HFEM_FP64)
+        HIPCHK(launch_condense_fp64(d.na, d.nb, e0, n, c->D, c->F, c->K, c->H, c->Ru, c->Rh, c->FH, c->info, grid,
+                                    c->stream));
+    else
+        HIPCHK(launch_condense_mixed(d.npv, d.ncu, d.na, d.nb, e0, n, c->D, c->F, c->K, c->H, c->Ru, c->Rh, c->FH,
+                                     c->info, c->work32, c->wstride, grid, 2, c->stream));
+    HIPCHK(hipEventRecord(c->ev[3], c->stream));
+    return 0;
+}
+
+extern "C" int hfem_last_kernel_ms(hfem_ctx *c, float *ams, float *cms)
+{
+    if (!c) return fail(-1, "hfem_last_kernel_ms: null context");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (ams) { *ams = 0; if (hipEventElapsedTime(ams, c->ev[0], c->ev[1]) != hipSuccess) *ams = -1; }
+    if (cms) { *cms = 0; if (hipEventElapsedTime(cms, c->ev[2], c->ev[3]) != hipSuccess) *cms = -1; }
+    (void)hipGetLastError();
+    return 0;
+}
+
+extern "C" int hfem_sync(hfem_ctx *c)
+{
+    if (!c) return fail(-1, "hfem_sync: null context");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int hfem_read_elements(hfem_ctx *c, int e0, int n, double *Ru, double *D, double *F, double *K, double *H,
+                                  double *Rh)
+{
+    int rc = check_range(c, e0, n, "hfem_read_elements");
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const Dims &d = c->d;
+    const long na = d.na, nb = d.nb;
+    if (Ru) HIPCHK(hipMemcpy(Ru, c->Ru + e0 * na, sizeof(double) * n * na, hipMemcpyDeviceToHost));
+    if (D) HIPCHK(hipMemcpy(D, c->D + e0 * na * na, sizeof(double) * n * na * na, hipMemcpyDeviceToHost));
+    if (F) HIPCHK(hipMemcpy(F, c->F + e0 * na * nb, sizeof(double) * n * na * nb, hipMemcpyDeviceToHost));
+    if (K) HIPCHK(hipMemcpy(K, c->K + e0 * nb * na, sizeof(double) * n * nb * na, hipMemcpyDeviceToHost));
+    if (H) HIPCHK(hipMemcpy(H, c->H + e0 * nb * nb, sizeof(double) * n * nb * nb, hipMemcpyDeviceToHost));
+    if (Rh) HIPCHK(hipMemcpy(Rh, c->Rh + e0 * nb, sizeof(double) * n * nb, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// FH (condensed trace residual) of [e0, e0+n); not part of the public header's
+// element read so that Rh keeps its uncondensed meaning.
+extern "C" int hfem_read_fh(hfem_ctx *c, int e0, int n, double *FH, int *info)
+{
+    int rc = check_range(c, e0, n, "hfem_read_fh");
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (FH) HIPCHK(hipMemcpy(FH, c->FH + (long)e0 * c->d.nb, sizeof(double) * n * c->d.nb, hipMemcpyDeviceToHost));
+    if (info) HIPCHK(hipMemcpy(info, c->info + e0, sizeof(int) * n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+static int reduce_sum(hfem_ctx *c, long n, const double *x, int square, double *out)
+{
+    const int nblk = 256;
+    HIPCHK(launch_sumsq(n, x, square, c->partial, nblk, c->stream));
+    HIPCHK(launch_sumsq(nblk, c->partial, 0, c->partial + nblk, 1, c->stream));
+    HIPCHK(hipMemcpyAsync(out, c->partial + nblk, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int hfem_assemble_linear_system(hfem_ctx *c, int precision, double *rNorm)
+{
+    if (!c) return fail(-1, "hfem_assemble_linear_system: null context");
+    const int ne = c->hd.ne, nf = c->hd.nf;
+    const Dims &d = c->d;
+    int rc = hfem_assemble(c, 0, ne);
+    if (rc) return rc;
+    const int bsz = d.nch * d.npf;
+    // r = assembled uncondensed Rh (Rhonly)
+    HIPCHK(launch_face_gather(nf, bsz, d.nch, d.npf, d.nb, c->f2e, c->einv, c->Rh, c->r, nullptr, nullptr, c->stream));
+    double s1 = 0, s2 = 0;
+    if ((rc = reduce_sum(c, ne, c->ru2, 0, &s1))) return rc;
+    if ((rc = reduce_sum(c, (long)nf * bsz, c->r, 1, &s2))) return rc;
+    if (rNorm) *rNorm = sqrt(s1 + s2);
+    if ((rc = hfem_condense(c, 0, ne, precision))) return rc;
+    HIPCHK(launch_bsr_gather(c->nblocks, bsz, d.nch, d.npf, d.nb, c->blkptr, c->blklist, c->einv, c->H, c->Hg,
+                             c->stream));
+    HIPCHK(launch_face_gather(nf, bsz, d.nch, d.npf, d.nb, c->f2e, c->einv, c->FH, c->Rg, nullptr, nullptr, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int hfem_read_system(hfem_ctx *c, double *Hg, double *Rg)
+{
+    if (!c || !c->has_mesh) return fail(-3, "hfem_read_system: mesh not set");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const long bsz = (long)c->d.nch * c->d.npf;
+    if (Hg) HIPCHK(hipMemcpy(Hg, c->Hg, sizeof(double) * c->nblocks * bsz * bsz, hipMemcpyDeviceToHost));
+    if (Rg) HIPCHK(hipMemcpy(Rg, c->Rg, sizeof(double) * c->hd.nf * bsz, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int hfem_bsr_matvec(hfem_ctx *c, const double *x, double *y)
+{
+    if (!c || !c->has_mesh || !x || !y) return fail(-1, "hfem_bsr_matvec: bad argument or mesh not set");
+    HIPCHK(hipSetDevice(c->device));
+    const long bsz = (long)c->d.nch * c->d.npf, n = c->hd.nf * bsz;
+    HIPCHK(hipMemcpyAsync(c->xv, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(launch_bsr_matvec(c->hd.nf, (int)bsz, c->rowstart, c->cols, c->Hg, c->xv, c->yv, c->stream));
+    HIPCHK(hipMemcpyAsync(y, c->yv, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}

@@ -1,0 +1,196 @@
+"""Host-side front end of the HIP library (ctypes over the C ABI in include/hfem_amd.h).
+
+``HDGContext`` mirrors the reference's element-path interface for one mesh:
+the method names follow the reference functions they replace
+(assembleElementMatrixVector -> ``assemble_element_matrix_vector``,
+schurElementMatrixVector -> ``schur_element_matrix_vector``,
+assembleLinearSystem -> ``assemble_linear_system`` ...), batched over element
+ranges with the data resident on the GPU.  Errors raise ``HfemError`` with the
+library's message (the reference prints and exits, preprocessing/errormsg.cpp).
+There is no CPU fallback: without the built library or a GPU this raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhfem_amd.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+
+FP64 = 0
+MIXED = 1
+
+
+class HfemError(RuntimeError):
+    pass
+
+
+class hfem_dims(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("nd", "ncu", "npv", "ngv", "npf", "ngf", "nfe", "ne", "nf")]
+
+
+EXPORTS = {
+    "hfem_last_error": (ctypes.c_char_p, []),
+    "hfem_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.POINTER(hfem_dims), ctypes.c_int]),
+    "hfem_destroy": (ctypes.c_int, [_vp]),
+    "hfem_set_master": (ctypes.c_int, [_vp, _dp, _dp, _dp, _dp, _ip]),
+    "hfem_set_physics": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int]),
+    "hfem_set_mesh": (ctypes.c_int, [_vp, _dp, _ip, _ip, _ip, _ip, _ip, ctypes.c_int]),
+    "hfem_set_solution": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+    "hfem_get_solution": (ctypes.c_int, [_vp, _dp, _dp]),
+    "hfem_assemble": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "hfem_condense": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "hfem_read_elements": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "hfem_assemble_linear_system": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "hfem_read_system": (ctypes.c_int, [_vp, _dp, _dp]),
+    "hfem_bsr_matvec": (ctypes.c_int, [_vp, _dp, _dp]),
+    "hfem_last_kernel_ms": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
+    "hfem_sync": (ctypes.c_int, [_vp]),
+}
+EXTRA_EXPORTS = {
+    "hfem_read_fh": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _ip]),
+}
+
+
+def load():
+    """Load the built library (raises if it is missing: there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise HfemError(f"{LIB_PATH} is not built; run python -m hfem_amd.build")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in {**EXPORTS, **EXTRA_EXPORTS}.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise HfemError(load().hfem_last_error().decode())
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class HDGContext:
+    """Device-resident HDG element path for one Problem (hfem_amd.problem)."""
+
+    def __init__(self, prob, device=0):
+        L = load()
+        self.prob = prob
+        m = prob.master
+        d = prob.dims
+        self.d = d
+        dims = hfem_dims(prob.nd, prob.ncu, m.npv, m.ngv, m.npf, m.ngf, m.nfe, prob.ne, prob.mesh.nf)
+        h = _vp()
+        _check(L.hfem_create(ctypes.byref(h), ctypes.byref(dims), device))
+        self.h = h
+        _check(L.hfem_set_master(h, _d(_f64(m.flat("shapvt"))), _d(_f64(m.flat("shapvg"))),
+                                 _d(_f64(m.flat("shapft"))), _d(_f64(m.flat("shapfg"))),
+                                 _i(_i32(m.perm.reshape(-1, order="F")))))
+        self.set_physics(prob.fc_u, prob.fc_q, prob.tdep)
+        msh = prob.mesh
+        _check(L.hfem_set_mesh(h, _d(_f64(prob.dgnodes_ref())), _i(_i32(msh.elcon)), _i(_i32(msh.bf)),
+                               _i(_i32(msh.t2f)), _i(_i32(msh.ent2entStart)), _i(_i32(msh.ent2ent)),
+                               len(msh.ent2ent)))
+        self.set_solution(prob.UDG, prob.UH, prob.SH)
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().hfem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    # -- state ---------------------------------------------------------------
+    def set_physics(self, fc_u, fc_q=1.0, tdep=0):
+        p = self.prob
+        _check(load().hfem_set_physics(self.h, _d(_f64(p.param)), _d(_f64(p.ui)), fc_u, fc_q, tdep))
+
+    def set_solution(self, UDG, UH, SH=None):
+        _check(load().hfem_set_solution(self.h, _d(_f64(UDG)), _d(_f64(UH)),
+                                        _d(_f64(SH)) if SH is not None else None))
+
+    def get_solution(self):
+        p = self.prob
+        UDG = np.zeros_like(p.UDG)
+        UH = np.zeros_like(p.UH)
+        _check(load().hfem_get_solution(self.h, _d(UDG), _d(UH)))
+        return UDG, UH
+
+    # -- element path ----------------------------------------------------------
+    def assemble_element_matrix_vector(self, e0=0, n=None):
+        n = self.prob.ne - e0 if n is None else n
+        _check(load().hfem_assemble(self.h, e0, n))
+
+    def schur_element_matrix_vector(self, e0=0, n=None, precision=FP64):
+        n = self.prob.ne - e0 if n is None else n
+        _check(load().hfem_condense(self.h, e0, n, precision))
+
+    def read_elements(self, e0=0, n=None, which=("Ru", "D", "F", "K", "H", "Rh")):
+        d = self.d
+        n = self.prob.ne - e0 if n is None else n
+        na, nb = d["na"], d["nb"]
+        sizes = dict(Ru=na, D=na * na, F=na * nb, K=nb * na, H=nb * nb, Rh=nb)
+        out = {k: np.zeros((n, sizes[k])) for k in which}
+        args = [_d(out[k]) if k in out else None for k in ("Ru", "D", "F", "K", "H", "Rh")]
+        _check(load().hfem_read_elements(self.h, e0, n, *args))
+        return out
+
+    def read_fh(self, e0=0, n=None):
+        n = self.prob.ne - e0 if n is None else n
+        FH = np.zeros((n, self.d["nb"]))
+        info = np.zeros(n, dtype=np.int32)
+        _check(load().hfem_read_fh(self.h, e0, n, _d(FH), _i(info)))
+        return FH, info
+
+    def kernel_ms(self):
+        a = ctypes.c_float()
+        c = ctypes.c_float()
+        _check(load().hfem_last_kernel_ms(self.h, ctypes.byref(a), ctypes.byref(c)))
+        return a.value, c.value
+
+    def sync(self):
+        _check(load().hfem_sync(self.h))
+
+    # -- global trace system ----------------------------------------------------
+    def assemble_linear_system(self, precision=FP64):
+        r = ctypes.c_double()
+        _check(load().hfem_assemble_linear_system(self.h, precision, ctypes.byref(r)))
+        return r.value
+
+    def read_system(self):
+        msh = self.prob.mesh
+        bsz = self.d["nch"] * self.d["npf"]
+        Hg = np.zeros(len(msh.ent2ent) * bsz * bsz)
+        Rg = np.zeros(msh.nf * bsz)
+        _check(load().hfem_read_system(self.h, _d(Hg), _d(Rg)))
+        return Hg, Rg
+
+    def bsr_matvec(self, x):
+        x = _f64(x)
+        y = np.zeros_like(x)
+        _check(load().hfem_bsr_matvec(self.h, _d(x), _d(y)))
+        return y

@@ -1,0 +1,39 @@
+"""The C-ABI library loads and exports every symbol include/hfem_amd.h declares
+(no GPU calls here)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "hfem_amd.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(hfem_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_header():
+    from hfem_amd import lib
+    if not os.path.exists(lib.LIB_PATH):
+        import subprocess, sys
+        subprocess.check_call([sys.executable, "-m", "hfem_amd.build"], cwd=ROOT)
+    L = ctypes.CDLL(lib.LIB_PATH)
+    syms = declared_symbols()
+    assert len(syms) >= 15
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+    # the Python binding covers the same set
+    assert set(syms) <= set(lib.EXPORTS)
+
+
+def test_error_path_without_device():
+    """Argument validation happens before any device call and reports through hfem_last_error."""
+    from hfem_amd import lib
+    L = lib.load()
+    h = ctypes.c_void_p()
+    bad = lib.hfem_dims(4, 6, 10, 10, 4, 4, 3, 1, 1)   # nd = 4 is rejected
+    rc = L.hfem_create(ctypes.byref(h), ctypes.byref(bad), 0)
+    assert rc < 0 and b"nd must be 2 or 3" in L.hfem_last_error()

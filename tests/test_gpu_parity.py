@@ -1,0 +1,133 @@
+"""HIP element path vs the CPU oracle and the reference's golden vectors (GPU).
+
+Tolerances (relative Frobenius norm, max over elements):
+* FP64 path: element matrices and condensed blocks <= 1e-13; the element
+  residual Ru and inv(D) Ru <= 1e-10 (cancellation between O(1) volume and face
+  terms; the reference's own impl 0 and impl 1 differ by ~1e-12 on it).
+* Mixed path (FP32 factorisation + FP64 refinement): <= 1e-10 on AH, FH,
+  inv(D)F, inv(D)Ru.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from hfem_amd.problem import make_problem
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+RTOL_MAT = 1e-13
+RTOL_RES = 1e-10
+RTOL_MIXED = 1e-10
+
+
+def relerr(a, b):
+    a = np.asarray(a).reshape(len(b), -1)
+    b = np.asarray(b).reshape(len(b), -1)
+    return float((np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)).max())
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+CASES = [("tri", 2, 2), ("tri", 2, 4), ("tet", 1, 2), ("tet", 1, 3), ("hex", 3, 2), ("tet", 2, 3)]
+
+
+@pytest.mark.parametrize("kind,nc,p", CASES)
+def test_assembly_and_condensation_vs_oracle(oracle_mod, kind, nc, p):
+    from hfem_amd.lib import HDGContext, FP64, MIXED
+    O = oracle_mod
+    pr = make_problem(kind, nc, p, seed=3)
+    d = pr.dims
+    say(f"case {kind}{nc} p{p}: ne={pr.ne} na={d['na']} nb={d['nb']}")
+    ref = O.Oracle(pr).assemble()
+    ctx = HDGContext(pr)
+    ctx.assemble_element_matrix_vector()
+    g = ctx.read_elements()
+    for k in ("D", "F", "K", "H", "Rh"):
+        e = relerr(g[k], ref[k])
+        say(f"  assemble {k}: {e:.2e}")
+        assert e < RTOL_MAT, k
+    assert relerr(g["Ru"], ref["Ru"]) < RTOL_RES
+    b = {k: v.copy() for k, v in ref.items()}
+    O.schur(0, d["npv"], d["ncu"], b)
+    ctx.schur_element_matrix_vector(precision=FP64)
+    c = ctx.read_elements(which=("Ru", "F", "H"))
+    FH, info = ctx.read_fh()
+    assert np.all(info == 0)
+    for name, got, want, tol in [("DinvF", c["F"], b["F"], RTOL_MAT), ("AH", c["H"], b["H"], RTOL_MAT),
+                                 ("DinvRu", c["Ru"], b["Ru"], RTOL_RES), ("FH", FH, b["Rh"], RTOL_RES)]:
+        e = relerr(got, want)
+        say(f"  fp64 condense {name}: {e:.2e}")
+        assert e < tol, name
+    # mixed precision on freshly assembled blocks
+    ctx.assemble_element_matrix_vector()
+    ctx.schur_element_matrix_vector(precision=MIXED)
+    c = ctx.read_elements(which=("Ru", "F", "H"))
+    FH, info = ctx.read_fh()
+    for name, got, want in [("DinvF", c["F"], b["F"]), ("AH", c["H"], b["H"]), ("DinvRu", c["Ru"], b["Ru"]),
+                            ("FH", FH, b["Rh"])]:
+        e = relerr(got, want)
+        say(f"  mixed condense {name}: {e:.2e}")
+        assert e < RTOL_MIXED, name
+    ctx.close()
+
+
+@pytest.mark.parametrize("name,kind,nc,p", [("tri2_p2", "tri", 2, 2), ("tet1_p3", "tet", 1, 3),
+                                            ("hex3_p2", "hex", 3, 2)])
+def test_against_reference_golden(name, kind, nc, p):
+    """Directly against the reference element path's outputs (schurImplementation 0)."""
+    from hfem_amd.lib import HDGContext, FP64
+    gold = np.load(os.path.join(GOLD, f"elem_{name}.npz"))
+    pr = make_problem(kind, nc, p, seed=1)
+    say(f"golden {name}")
+    keep = gold["keep"]
+    ctx = HDGContext(pr)
+    ctx.assemble_element_matrix_vector()
+    g = ctx.read_elements()
+    for k in ("D", "F", "K", "H", "Rh"):
+        assert relerr(g[k][keep], gold[k]) < RTOL_MAT, k
+    ctx.schur_element_matrix_vector(precision=FP64)
+    c = ctx.read_elements(which=("Ru", "H"))
+    FH, _ = ctx.read_fh()
+    assert relerr(c["H"][keep], gold["AH_impl0"]) < RTOL_MAT
+    assert relerr(FH[keep], gold["FH_impl0"]) < RTOL_RES
+    assert relerr(c["Ru"][keep], gold["DinvRu_impl0"]) < RTOL_RES
+    ctx.close()
+
+
+@pytest.mark.parametrize("kind,nc,p", [("tri", 3, 2), ("tet", 1, 3)])
+def test_linear_system_and_matvec(oracle_mod, kind, nc, p):
+    from hfem_amd.lib import HDGContext, FP64
+    O = oracle_mod
+    pr = make_problem(kind, nc, p, seed=4)
+    d = pr.dims
+    say(f"system {kind}{nc} p{p}")
+    ref = O.Oracle(pr).assemble()
+    rnorm_ref = np.sqrt((ref["Ru"] ** 2).sum() + (O.scatter(pr, np.zeros((pr.ne, d["nb"] ** 2)), ref["Rh"])[1] ** 2).sum())
+    O.schur(0, d["npv"], d["ncu"], ref)
+    Hg_ref, Rg_ref = O.scatter(pr, ref["H"], ref["Rh"])
+    ctx = HDGContext(pr)
+    rnorm = ctx.assemble_linear_system(FP64)
+    assert abs(rnorm - rnorm_ref) < 1e-12 * rnorm_ref
+    Hg, Rg = ctx.read_system()
+    assert np.linalg.norm(Hg - Hg_ref) < RTOL_MAT * np.linalg.norm(Hg_ref)
+    assert np.linalg.norm(Rg - Rg_ref) < RTOL_RES * np.linalg.norm(Rg_ref)
+    x = np.random.default_rng(0).standard_normal(len(Rg))
+    y = ctx.bsr_matvec(x)
+    yref = O.bsr_matvec(pr, Hg_ref, x)
+    assert np.linalg.norm(y - yref) < 1e-13 * np.linalg.norm(yref)
+    ctx.close()
+
+
+def test_errors_are_reported():
+    from hfem_amd.lib import HDGContext, HfemError
+    pr = make_problem("tri", 1, 1, seed=0)
+    ctx = HDGContext(pr)
+    with pytest.raises(HfemError, match="outside"):
+        ctx.assemble_element_matrix_vector(0, pr.ne + 1)
+    with pytest.raises(HfemError, match="precision"):
+        ctx.schur_element_matrix_vector(0, 1, precision=7)
+    ctx.close()
