@@ -1,0 +1,183 @@
+"""Benchmark: condensed elements/s of the HDG element path on one MI355X.
+
+Workload: 3D compressible Navier-Stokes HDG, p=3 straight tetrahedra
+(ncu=5, npv=20, 24 volume / 12 face quadrature points, D 100x100, trace block
+200x200), unit cube of 14^3 cubes x 6 tets = 16464 elements, vertices jittered
++-5% of h, freestream (Mach 0.3, Re 1000, Pr 0.72) plus seeded perturbations.
+
+A step is one pass of the element path over all elements with inputs already
+resident in HBM: element assembly (geometry, fluxes and Jacobians, D F K H Ru Rh)
+followed by static condensation (AH, FH, inv(D)F, inv(D)Ru).  Both precisions
+are timed; ``value`` is the mixed path (FP32 factorisation + FP64 refinement),
+reported with its error against the FP64 path, and the FP64 rate beside it.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# algorithmic FP64-equivalent FLOPs per element (own count from the reference's
+# BLAS/LAPACK call shapes): assembly 9.636 MF, condensation
+# (2/3)na^3 + 2na^2 nb + 2nb^2 na + 2na^2 + 2nb na = 12.727 MF at na=100, nb=200.
+PEAK_FP64 = 78.6e12
+PEAK_FP32 = 157.3e12
+PEAK_HBM = 8.0e12
+
+
+def condense_flops(na, nb):
+    return 2.0 / 3.0 * na ** 3 + 2.0 * na * na * nb + 2.0 * nb * nb * na + 2.0 * na * na + 2.0 * nb * na
+
+
+ASSEMBLY_FLOPS = {(3, 0, 3): 9.636e6, (2, 0, 4): 1.019e6, (2, 0, 2): 0.104e6, (3, 1, 2): 17.460e6}
+
+
+def relerr(a, b):
+    return float((np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)).max())
+
+
+def run_steps(ctx, ne, precision, steps, warmup):
+    for _ in range(warmup):
+        ctx.assemble_element_matrix_vector(0, ne)
+        ctx.schur_element_matrix_vector(0, ne, precision)
+    ctx.sync()
+    asm_ms, cond_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.assemble_element_matrix_vector(0, ne)
+        ctx.schur_element_matrix_vector(0, ne, precision)
+        a, c = ctx.kernel_ms()          # events on the library stream; synchronises
+        asm_ms.append(a)
+        cond_ms.append(c)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    return wall, float(np.mean(asm_ms)), float(np.mean(cond_ms))
+
+
+def cpu_baseline(prob, seconds=10.0):
+    """The reference's own element path (oracle/_ref, MKL, 1 thread) on a bounded sample."""
+    os.environ.setdefault("MKL_NUM_THREADS", "1")
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    kind = "reference"
+    try:
+        ref = O.Reference(prob)
+        probe = max(ref.time(20, impl=1) / 20, 1e-5)
+        n = int(min(max(seconds / probe, 50), 50000))
+        t = ref.time(n, impl=1)
+    except Exception:
+        kind = "port"
+        orc = O.Oracle(prob)
+        t0 = time.perf_counter()
+        n = 0
+        while time.perf_counter() - t0 < seconds:
+            out = orc.assemble(n % prob.ne, 1)
+            O.schur(0, prob.dims["npv"], prob.dims["ncu"], out)
+            n += 1
+        t = time.perf_counter() - t0
+    return dict(value=n / t, unit="elements/s", cores=1, kind=kind,
+                sample=f"{n} elements of the same 16464-tet p=3 mesh, assemble+condense, one host thread"
+                       + (" (reference element path built from its sources with MKL, schurImplementation 1)"
+                          if kind == "reference" else " (oracle restatement)"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--ncell", type=int, default=14)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the element path shards by elements across GPUs; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from hfem_amd.problem import make_problem
+    from hfem_amd.lib import HDGContext, FP64, MIXED
+
+    prob = make_problem("tet", args.ncell, 3, seed=0)
+    d = prob.dims
+    ne, na, nb = prob.ne, d["na"], d["nb"]
+    print(f"[bench] tet p=3 ne={ne} na={na} nb={nb}", file=sys.stderr, flush=True)
+    ctx = HDGContext(prob)
+    f_asm = ASSEMBLY_FLOPS[(3, 0, 3)]
+    f_cond = condense_flops(na, nb)
+    res = {}
+    precs = {"fp64": FP64, "mixed": MIXED}
+    todo = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    for name in todo:
+        wall, a_ms, c_ms = run_steps(ctx, ne, precs[name], args.steps, args.warmup)
+        res[name] = dict(wall=wall, asm_ms=a_ms, cond_ms=c_ms, eps=ne * args.steps / wall)
+        print(f"[bench] {name}: {res[name]['eps']:.1f} elements/s  assemble {a_ms:.3f} ms  condense {c_ms:.3f} ms",
+              file=sys.stderr, flush=True)
+
+    # accuracy of the mixed path against the FP64 path on a sample of elements
+    sample = np.linspace(0, ne - 1, 64).astype(int)
+    errs = {}
+    if "mixed" in res and "fp64" in res:
+        outs = {}
+        for name in ("fp64", "mixed"):
+            ctx.assemble_element_matrix_vector(0, ne)
+            ctx.schur_element_matrix_vector(0, ne, precs[name])
+            blocks = {}
+            for e in sample:
+                r = ctx.read_condensed(int(e), 1)
+                for k in ("DinvRu", "DinvF", "AH", "FH"):
+                    blocks.setdefault(k, []).append(r[k][0])
+            outs[name] = {k: np.array(v) for k, v in blocks.items()}
+        errs = {k: relerr(outs["mixed"][k], outs["fp64"][k]) for k in outs["fp64"]}
+        print(f"[bench] mixed vs fp64 max rel err: {errs}", file=sys.stderr, flush=True)
+
+    head = "mixed" if "mixed" in res else "fp64"
+    r = res[head]
+    step_s = r["wall"] / args.steps
+    # dominant kernel and its roofline (achieved = algorithmic flops / average launch time)
+    if r["cond_ms"] >= r["asm_ms"]:
+        kname, kms, kflops = ("condense_mixed" if head == "mixed" else "condense_fp64"), r["cond_ms"], f_cond
+    else:
+        kname, kms, kflops = "assemble", r["asm_ms"], f_asm
+    achieved = kflops * ne / (kms * 1e-3)
+    peak = PEAK_FP32 if (head == "mixed" and kname.startswith("condense")) else PEAK_FP64
+    out = {
+        "metric": "condensed elements/s, FP32-MFMA+refine vs FP64 at <=1e-10 rel err",
+        "value": r["eps"],
+        "unit": "elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": step_s * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 factorisation + f64 refinement" if head == "mixed" else "f64",
+        "data": "synthetic (seeded jittered mesh and perturbed freestream state)",
+        "config": {"workload": f"3D NS HDG p=3 tets, {args.ncell}^3x6={ne} elements, assemble+condense",
+                   "elements": ne, "na": na, "nb": nb},
+        "fp64_elements_per_s": res.get("fp64", {}).get("eps"),
+        "mixed_elements_per_s": res.get("mixed", {}).get("eps"),
+        "mixed_speedup_vs_fp64": (res["mixed"]["eps"] / res["fp64"]["eps"]) if len(res) == 2 else None,
+        "mixed_max_rel_err_vs_fp64": errs or None,
+        "kernel_ms": {k: {"assemble": v["asm_ms"], "condense": v["cond_ms"]} for k, v in res.items()},
+        "roofline": {"bound": "mfma", "kernel": kname, "achieved": achieved / 1e12, "peak": peak / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                     "flops_per_element": kflops, "elements_per_launch": ne,
+                     "frac_of_fp64_peak": achieved / PEAK_FP64},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(prob)
+    print(json.dumps(out))
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -13,17 +13,19 @@
 
 namespace hfem {
 hipError_t launch_assemble(const Dims &, const Master &, const Phys &, const double *, const double *, const double *,
-                           const double *, const int *, const int *, int, int, double *, long, int, double *, double *,
-                           double *, double *, double *, double *, double *, hipStream_t);
-hipError_t launch_condense_fp64(int, int, int, int, double *, double *, const double *, double *, double *,
-                                const double *, double *, int *, int, hipStream_t);
-hipError_t launch_condense_mixed(int npv, int ncu, int na, int nb, int e0, int n, double *D, double *F,
-                                 const double *K, double *H, double *Ru, const double *Rh, double *FH, int *info,
-                                 float *work32, long wstride, int grid, int sweeps, hipStream_t s);
-long mixed_work_floats(int na, int nb);
-hipError_t launch_bsr_gather(int, int, int, int, int, const int *, const int *, const int *, const double *, double *,
-                             hipStream_t);
-hipError_t launch_face_gather(int, int, int, int, int, const int *, const int *, const double *, double *,
+                           const double *, const int *, const int *, int, int, double *, long, int, int, double *,
+                           double *, double *, double *, double *, double *, double *, hipStream_t);
+hipError_t launch_condense_fp64(int na, int nb, int ldx, int e0, int n, const double *D, double *Dinv_g,
+                                const double *Fx, double *X, const double *K, double *Hx, int *info, int grid,
+                                bool dinv_lds, hipStream_t s);
+hipError_t launch_condense_mixed(int na, int nb, int ldx, int e0, int n, const double *D, const double *Fx, double *X,
+                                 double *Rw, const double *K, double *Hx, int *info, int grid, int sweeps,
+                                 hipStream_t s);
+size_t condense_lds_bytes(int na, bool dinv_in_lds, int elt);
+int condense_dinv_ld(int na);
+hipError_t launch_bsr_gather(int, int, int, int, int, long, const int *, const int *, const int *, const double *,
+                             double *, hipStream_t);
+hipError_t launch_face_gather(int, int, int, int, int, long, const int *, const int *, const double *, double *,
                               const double *, double *, hipStream_t);
 hipError_t launch_sumsq(long, const double *, int, double *, int, hipStream_t);
 hipError_t launch_bsr_matvec(int, int, const int *, const int *, const double *, const double *, double *,
@@ -69,16 +71,17 @@ struct hfem_ctx {
     int nblocks = 0;
     // state
     double *UDG = nullptr, *UH = nullptr, *SH = nullptr;
-    // element blocks
-    double *D = nullptr, *F = nullptr, *K = nullptr, *H = nullptr, *Ru = nullptr, *Rh = nullptr, *FH = nullptr,
+    // element blocks: D (na x na), Fx and X (na x ldx, row-major, column nb = Ru / inv(D)Ru),
+    // K (nb x na), Hx (nb x (nb+1), column nb = Rh / FH), Ru, Rh (uncondensed)
+    double *D = nullptr, *Fx = nullptr, *X = nullptr, *K = nullptr, *Hx = nullptr, *Ru = nullptr, *Rh = nullptr,
            *ru2 = nullptr;
     int *info = nullptr;
+    int ldx = 0;
     // work
-    double *scratch = nullptr;
+    double *scratch = nullptr, *dinv = nullptr, *rwork = nullptr;
     long sstride = 0;
     int grid_asm = 0, grid_cond = 0;
-    float *work32 = nullptr;
-    long wstride = 0;
+    bool dinv_lds = true;
     // trace system
     double *Hg = nullptr, *Rg = nullptr, *r = nullptr, *partial = nullptr, *xv = nullptr, *yv = nullptr;
     float ms_asm = 0, ms_cond = 0;
@@ -112,26 +115,37 @@ extern "C" int hfem_create(hfem_ctx **out, const hfem_dims *hd, int device)
     d.nd = hd->nd; d.ncu = hd->ncu; d.nch = hd->ncu; d.nd1 = hd->nd + 1; d.nc = d.ncu * d.nd1;
     d.npv = hd->npv; d.ngv = hd->ngv; d.npf = hd->npf; d.ngf = hd->ngf; d.nfe = hd->nfe;
     d.ndf = d.npf * d.nfe; d.na = d.npv * d.ncu; d.nb = d.ndf * d.nch; d.np = d.ngf * d.nfe;
-    if (d.na > 1024) { delete c; return fail(-1, "hfem_create: na=%d exceeds 1024", d.na); }
+    if (d.na > 176) { delete c; return fail(-1, "hfem_create: na=%d exceeds 176 (npv*ncu)", d.na); }
     const long ne = hd->ne;
     hipError_t e = hipSuccess;
 #define ALLOC(p, n) if (e == hipSuccess) e = dalloc(&c->p, (size_t)(n))
+    c->ldx = (d.nb + 1 + 15) / 16 * 16;
     ALLOC(D, ne * d.na * d.na);
-    ALLOC(F, ne * d.na * d.nb);
+    ALLOC(Fx, ne * d.na * c->ldx);
+    ALLOC(X, ne * d.na * c->ldx);
     ALLOC(K, ne * d.nb * d.na);
-    ALLOC(H, ne * d.nb * d.nb);
+    ALLOC(Hx, ne * d.nb * (d.nb + 1));
     ALLOC(Ru, ne * d.na);
     ALLOC(Rh, ne * d.nb);
-    ALLOC(FH, ne * d.nb);
     ALLOC(ru2, ne);
     ALLOC(info, ne);
     ALLOC(partial, 1024);
     c->sstride = scratch_layout(d).total;
     c->grid_asm = (int)std::min<long>(ne, 1024);
-    c->grid_cond = (int)std::min<long>(ne, 2048);
+    c->grid_cond = (int)std::min<long>(ne, 1024);
     ALLOC(scratch, (long)c->grid_asm * c->sstride);
-    c->wstride = mixed_work_floats(d.na, d.nb);
-    ALLOC(work32, (long)c->grid_cond * c->wstride);
+    {
+        int maxlds = 65536;
+        if (hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
+            maxlds = 65536;
+        c->dinv_lds = condense_lds_bytes(d.na, true, sizeof(double)) <= (size_t)maxlds;
+        if (condense_lds_bytes(d.na, true, sizeof(float)) > (size_t)maxlds) {
+            delete c;
+            return fail(-1, "hfem_create: na=%d needs more LDS than the device offers (%d bytes)", d.na, maxlds);
+        }
+    }
+    if (!c->dinv_lds) ALLOC(dinv, (long)c->grid_cond * d.na * condense_dinv_ld(d.na));
+    ALLOC(rwork, (long)c->grid_cond * d.na * c->ldx);
 #undef ALLOC
     if (e == hipSuccess) e = hipStreamCreate(&c->stream);
     for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
@@ -149,9 +163,9 @@ extern "C" int hfem_destroy(hfem_ctx *c)
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     void *ptrs[] = {c->shapvt, c->shapvg, c->shapft, c->shapfg, c->perm, c->invperm, c->dgnodes, c->elcon, c->bf,
-                    c->rowstart, c->cols, c->einv, c->f2e, c->blkptr, c->blklist, c->UDG, c->UH, c->SH, c->D, c->F,
-                    c->K, c->H, c->Ru, c->Rh, c->FH, c->ru2, c->info, c->scratch, c->work32, c->Hg, c->Rg, c->r,
-                    c->partial, c->xv, c->yv};
+                    c->rowstart, c->cols, c->einv, c->f2e, c->blkptr, c->blklist, c->UDG, c->UH, c->SH, c->D, c->Fx,
+                    c->X, c->K, c->Hx, c->Ru, c->Rh, c->ru2, c->info, c->scratch, c->dinv, c->rwork, c->Hg, c->Rg,
+                    c->r, c->partial, c->xv, c->yv};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (auto &ev : c->ev)
@@ -322,7 +336,7 @@ extern "C" int hfem_assemble(hfem_ctx *c, int e0, int n)
     const int grid = std::min(n, c->grid_asm);
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
     HIPCHK(launch_assemble(c->d, m, c->ph, c->dgnodes, c->UDG, c->UH, c->SH, c->elcon, c->bf, e0, n, c->scratch,
-                           c->sstride, grid, c->Ru, c->D, c->F, c->K, c->H, c->Rh, c->ru2, c->stream));
+                           c->sstride, grid, c->ldx, c->Ru, c->D, c->Fx, c->K, c->Hx, c->Rh, c->ru2, c->stream));
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
     return 0;
 }
@@ -338,11 +352,11 @@ extern "C" int hfem_condense(hfem_ctx *c, int e0, int n, int precision)
     const int grid = std::min(n, c->grid_cond);
     HIPCHK(hipEventRecord(c->ev[2], c->stream));
     if (precision == HFEM_FP64)
-        HIPCHK(launch_condense_fp64(d.na, d.nb, e0, n, c->D, c->F, c->K, c->H, c->Ru, c->Rh, c->FH, c->info, grid,
-                                    c->stream));
+        HIPCHK(launch_condense_fp64(d.na, d.nb, c->ldx, e0, n, c->D, c->dinv, c->Fx, c->X, c->K, c->Hx, c->info, grid,
+                                    c->dinv_lds, c->stream));
     else
-        HIPCHK(launch_condense_mixed(d.npv, d.ncu, d.na, d.nb, e0, n, c->D, c->F, c->K, c->H, c->Ru, c->Rh, c->FH,
-                                     c->info, c->work32, c->wstride, grid, 2, c->stream));
+        HIPCHK(launch_condense_mixed(d.na, d.nb, c->ldx, e0, n, c->D, c->Fx, c->X, c->rwork, c->K, c->Hx, c->info,
+                                     grid, 2, c->stream));
     HIPCHK(hipEventRecord(c->ev[3], c->stream));
     return 0;
 }
@@ -366,6 +380,37 @@ extern "C" int hfem_sync(hfem_ctx *c)
     return 0;
 }
 
+// Copy the element blocks of [e0, e0+n) to the host in the reference layouts.
+// Fx (row-major, F then Ru) is transposed back to column-major F; Hx gives H
+// (first nb columns) -- before condensation -- or AH after it.
+static int read_rowmajor(hfem_ctx *c, const double *src, int e0, int n, double *colmajor, double *lastcol)
+{
+    const Dims &d = c->d;
+    const long na = d.na, nb = d.nb, ldx = c->ldx;
+    std::vector<double> tmp((size_t)n * na * ldx);
+    HIPCHK(hipMemcpy(tmp.data(), src + (long)e0 * na * ldx, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost));
+    for (long t = 0; t < n; t++)
+        for (long i = 0; i < na; i++) {
+            const double *row = tmp.data() + (t * na + i) * ldx;
+            if (colmajor)
+                for (long j = 0; j < nb; j++) colmajor[t * na * nb + i + na * j] = row[j];
+            if (lastcol) lastcol[t * na + i] = row[nb];
+        }
+    return 0;
+}
+
+static int read_hx(hfem_ctx *c, int e0, int n, double *H, double *lastcol)
+{
+    const Dims &d = c->d;
+    const long nb = d.nb, es = nb * (nb + 1);
+    for (long t = 0; t < n; t++) {
+        const double *src = c->Hx + (e0 + t) * es;
+        if (H) HIPCHK(hipMemcpy(H + t * nb * nb, src, sizeof(double) * nb * nb, hipMemcpyDeviceToHost));
+        if (lastcol) HIPCHK(hipMemcpy(lastcol + t * nb, src + nb * nb, sizeof(double) * nb, hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
 extern "C" int hfem_read_elements(hfem_ctx *c, int e0, int n, double *Ru, double *D, double *F, double *K, double *H,
                                   double *Rh)
 {
@@ -377,22 +422,22 @@ extern "C" int hfem_read_elements(hfem_ctx *c, int e0, int n, double *Ru, double
     const long na = d.na, nb = d.nb;
     if (Ru) HIPCHK(hipMemcpy(Ru, c->Ru + e0 * na, sizeof(double) * n * na, hipMemcpyDeviceToHost));
     if (D) HIPCHK(hipMemcpy(D, c->D + e0 * na * na, sizeof(double) * n * na * na, hipMemcpyDeviceToHost));
-    if (F) HIPCHK(hipMemcpy(F, c->F + e0 * na * nb, sizeof(double) * n * na * nb, hipMemcpyDeviceToHost));
+    if (F && (rc = read_rowmajor(c, c->Fx, e0, n, F, nullptr))) return rc;
     if (K) HIPCHK(hipMemcpy(K, c->K + e0 * nb * na, sizeof(double) * n * nb * na, hipMemcpyDeviceToHost));
-    if (H) HIPCHK(hipMemcpy(H, c->H + e0 * nb * nb, sizeof(double) * n * nb * nb, hipMemcpyDeviceToHost));
+    if (H && (rc = read_hx(c, e0, n, H, nullptr))) return rc;
     if (Rh) HIPCHK(hipMemcpy(Rh, c->Rh + e0 * nb, sizeof(double) * n * nb, hipMemcpyDeviceToHost));
     return 0;
 }
 
-// FH (condensed trace residual) of [e0, e0+n); not part of the public header's
-// element read so that Rh keeps its uncondensed meaning.
-extern "C" int hfem_read_fh(hfem_ctx *c, int e0, int n, double *FH, int *info)
+extern "C" int hfem_read_condensed(hfem_ctx *c, int e0, int n, double *DinvRu, double *DinvF, double *AH, double *FH,
+                                   int *info)
 {
-    int rc = check_range(c, e0, n, "hfem_read_fh");
+    int rc = check_range(c, e0, n, "hfem_read_condensed");
     if (rc) return rc;
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (FH) HIPCHK(hipMemcpy(FH, c->FH + (long)e0 * c->d.nb, sizeof(double) * n * c->d.nb, hipMemcpyDeviceToHost));
+    if ((DinvRu || DinvF) && (rc = read_rowmajor(c, c->X, e0, n, DinvF, DinvRu))) return rc;
+    if ((AH || FH) && (rc = read_hx(c, e0, n, AH, FH))) return rc;
     if (info) HIPCHK(hipMemcpy(info, c->info + e0, sizeof(int) * n, hipMemcpyDeviceToHost));
     return 0;
 }
@@ -416,15 +461,18 @@ extern "C" int hfem_assemble_linear_system(hfem_ctx *c, int precision, double *r
     if (rc) return rc;
     const int bsz = d.nch * d.npf;
     // r = assembled uncondensed Rh (Rhonly)
-    HIPCHK(launch_face_gather(nf, bsz, d.nch, d.npf, d.nb, c->f2e, c->einv, c->Rh, c->r, nullptr, nullptr, c->stream));
+    HIPCHK(launch_face_gather(nf, bsz, d.nch, d.npf, d.nb, d.nb, c->f2e, c->einv, c->Rh, c->r, nullptr, nullptr,
+                              c->stream));
     double s1 = 0, s2 = 0;
     if ((rc = reduce_sum(c, ne, c->ru2, 0, &s1))) return rc;
     if ((rc = reduce_sum(c, (long)nf * bsz, c->r, 1, &s2))) return rc;
     if (rNorm) *rNorm = sqrt(s1 + s2);
     if ((rc = hfem_condense(c, 0, ne, precision))) return rc;
-    HIPCHK(launch_bsr_gather(c->nblocks, bsz, d.nch, d.npf, d.nb, c->blkptr, c->blklist, c->einv, c->H, c->Hg,
+    const long es = (long)d.nb * (d.nb + 1);
+    HIPCHK(launch_bsr_gather(c->nblocks, bsz, d.nch, d.npf, d.nb, es, c->blkptr, c->blklist, c->einv, c->Hx, c->Hg,
                              c->stream));
-    HIPCHK(launch_face_gather(nf, bsz, d.nch, d.npf, d.nb, c->f2e, c->einv, c->FH, c->Rg, nullptr, nullptr, c->stream));
+    HIPCHK(launch_face_gather(nf, bsz, d.nch, d.npf, d.nb, es, c->f2e, c->einv, c->Hx + (long)d.nb * d.nb, c->Rg,
+                              nullptr, nullptr, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }

@@ -1,8 +1,11 @@
 // Element assembly for the compressible-NS HDG element path (gfx950).
 //
 // One workgroup assembles one element at a time (grid-stride over elements) and
-// writes that element's D, F, K, H, Ru, Rh blocks (reference layouts, see
-// assembly/elementassembly.cpp:1481-1499 and the faceint/elementint outputs).
+// writes that element's D, F, K, H, Ru, Rh blocks (assembly/elementassembly.cpp:1481-1499
+// and the faceint/elementint outputs).  D and K use the reference's column-major
+// layouts; F is written row-major into Fx (na x ldx) with Ru as column nb, and H
+// column-major into Hx (nb x (nb+1)) with Rh as column nb -- the layouts the
+// condensation kernel streams through its MFMA operands.
 // The work is organised as a sequence of phases; inside a phase every thread
 // computes independent entries, so no atomics are needed:
 //   P0 gather element data into LDS          P5 traces of M^{-1}C, M^{-1}E
@@ -28,9 +31,9 @@ template <int ND>
 __global__ void __launch_bounds__(256)
 assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, const double *__restrict__ UDG,
                 const double *__restrict__ UH, const double *__restrict__ SH, const int *__restrict__ elcon,
-                const int *__restrict__ bf, int e0, int n, double *__restrict__ scratch, long sstride,
-                double *__restrict__ Ru, double *__restrict__ Dm, double *__restrict__ Fm,
-                double *__restrict__ Km, double *__restrict__ Hm, double *__restrict__ Rh,
+                const int *__restrict__ bf, int e0, int n, double *__restrict__ scratch, long sstride, int ldx,
+                double *__restrict__ Ru, double *__restrict__ Dm, double *__restrict__ Fx,
+                double *__restrict__ Km, double *__restrict__ Hx, double *__restrict__ Rh,
                 double *__restrict__ ru2)
 {
     extern __shared__ double lds[];
@@ -376,7 +379,9 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                 if (i < nD + nF) {
                     const bool isD = i < nD;
                     long i2 = isD ? i : i - nD;
-                    int row = i2 % na; long col = i2 / na;
+                    // D: column-major (row fastest); F: row-major in Fx (column fastest)
+                    int row = isD ? (int)(i2 % na) : (int)(i2 / nb);
+                    long col = isD ? i2 / na : i2 % nb;
                     int j = row / npv, ii = row % npv;
                     int kc, cc;   // column component and column node/trace index
                     if (isD) { kc = col / npv; cc = col % npv; }
@@ -397,7 +402,7 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                         }
                     }
                     if (isD) Dm[e * nD + i2] = s;
-                    else Fm[e * nF + i2] = s;
+                    else Fx[e * (long)na * ldx + (long)row * ldx + col] = s;
                 } else if (i < nD + nF + nK + nH) {
                     const bool isK = i < nD + nF + nK;
                     long i2 = isK ? i - nD - nF : i - nD - nF - nK;
@@ -414,7 +419,7 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                         s += SFG(kk, g, 0) * jacf[p] * Z[p];
                     }
                     if (isK) Km[e * nK + i2] = s;
-                    else Hm[e * nH + i2] = s;
+                    else Hx[e * (nH + nb) + i2] = s;
                 } else if (i < nD + nF + nK + nH + na) {
                     int row = i - (nD + nF + nK + nH), j = row / npv, ii = row % npv;
                     double s = 0;
@@ -442,6 +447,7 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                         }
                     }
                     Ru[e * na + row] = s;
+                    Fx[e * (long)na * ldx + (long)row * ldx + nb] = s;
                     ru2loc += s * s;
                 } else {
                     int row = i - (nD + nF + nK + nH + na), j = row % nch, tr = row / nch, a = tr / npf, kk = tr % npf;
@@ -452,6 +458,7 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                         s += SFG(kk, g, 0) * fx[p + np * j] * jacf[p];
                     }
                     Rh[e * nb + row] = -s;
+                    Hx[e * (nH + nb) + nH + row] = -s;
                 }
             }
             // |Ru_e|^2 for the residual norm (assembly/globalassembly.cpp:501-502)
@@ -476,16 +483,16 @@ size_t assemble_lds_bytes(const Dims &d)
 
 hipError_t launch_assemble(const Dims &d, const Master &m, const Phys &ph, const double *dgnodes, const double *UDG,
                            const double *UH, const double *SH, const int *elcon, const int *bf, int e0, int n,
-                           double *scratch, long sstride, int grid, double *Ru, double *D, double *F, double *K,
-                           double *H, double *Rh, double *ru2, hipStream_t stream)
+                           double *scratch, long sstride, int grid, int ldx, double *Ru, double *D, double *Fx,
+                           double *K, double *Hx, double *Rh, double *ru2, hipStream_t stream)
 {
     size_t lds = assemble_lds_bytes(d);
     if (d.nd == 2)
         hipLaunchKernelGGL(assemble_kernel<2>, dim3(grid), dim3(256), lds, stream, d, m, ph, dgnodes, UDG, UH, SH,
-                           elcon, bf, e0, n, scratch, sstride, Ru, D, F, K, H, Rh, ru2);
+                           elcon, bf, e0, n, scratch, sstride, ldx, Ru, D, Fx, K, Hx, Rh, ru2);
     else
         hipLaunchKernelGGL(assemble_kernel<3>, dim3(grid), dim3(256), lds, stream, d, m, ph, dgnodes, UDG, UH, SH,
-                           elcon, bf, e0, n, scratch, sstride, Ru, D, F, K, H, Rh, ru2);
+                           elcon, bf, e0, n, scratch, sstride, ldx, Ru, D, Fx, K, Hx, Rh, ru2);
     return hipGetLastError();
 }
 

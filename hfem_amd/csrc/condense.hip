@@ -1,269 +1,429 @@
-// Static condensation of element blocks (assembly/elementschur.cpp:6-329), FP64.
+// Static condensation of element blocks (assembly/elementschur.cpp:6-329) on gfx950.
 //
-// One workgroup per element (grid-stride).  D is factored in place by LU with
-// partial pivoting (LAPACK getrf semantics: first maximal |pivot|, scale by the
-// reciprocal, rank-1 trailing update), then
-//   F  <- inv(D) F,  Ru <- inv(D) Ru                (getrs)
-//   AH  = H - K inv(D) F,  FH = Rh - K inv(D) Ru    (written to H and FH)
-// Rh itself is kept (the uncondensed trace residual enters rNorm).
+// Per element (one 256-thread workgroup, grid-stride over elements):
+//   1. inv(D) by Gauss-Jordan elimination with partial pivoting, the na x na
+//      matrix distributed cyclically over a 16 x 16 thread grid in registers
+//      (pivot column and rows exchanged through LDS, double-buffered so each
+//      elimination step needs two barriers);
+//   2. X = inv(D) [F | Ru]                  (FP64 MFMA 16x16x4, inv(D) from LDS)
+//   3. [AH | FH] = [H | Rh] - K X           (FP64 MFMA 16x16x4, in place)
+// Internal layouts: D column-major (na x na); Fx row-major (na x ldx) holding
+// F then Ru in column nb; X the same layout; K column-major (nb x na); Hx
+// column-major (nb x (nb+1)) holding H then Rh in column nb.
+//
+// The mixed path factors in FP32 (inv(D) rounded to FP32, FP32 MFMA products)
+// and refines X against the FP64 D with FP64 residuals; the Schur product stays
+// FP64 because AH must carry the FP64 accuracy.
 #include "hfem_device.h"
 
 namespace hfem {
 
-__global__ void __launch_bounds__(256)
-condense_fp64_kernel(int na, int nb, int e0, int n, double *__restrict__ Dm, double *__restrict__ Fm,
-                     const double *__restrict__ Km, double *__restrict__ Hm, double *__restrict__ Ru,
-                     const double *__restrict__ Rh, double *__restrict__ FH, int *__restrict__ info)
-{
-    __shared__ double rv[256];
-    __shared__ int ri[256];
-    __shared__ int piv[1024];
-    const int tid = threadIdx.x, nth = blockDim.x;
-    for (int t = blockIdx.x; t < n; t += gridDim.x) {
-        const long e = e0 + t;
-        double *D = Dm + e * (long)na * na;
-        double *F = Fm + e * (long)na * nb;
-        const double *K = Km + e * (long)nb * na;
-        double *H = Hm + e * (long)nb * nb;
-        double *u = Ru + e * na;
-        int sing = 0;
-        // ---- LU with partial pivoting ----
-        for (int j = 0; j < na; j++) {
-            double best = -1.0;
-            int bi = na;
-            for (int i = j + tid; i < na; i += nth) {
-                double v = fabs(D[i + (long)na * j]);
-                if (v > best) { best = v; bi = i; }
-            }
-            rv[tid] = best;
-            ri[tid] = bi;
-            __syncthreads();
-            for (int s = nth / 2; s > 0; s >>= 1) {
-                if (tid < s) {
-                    double o = rv[tid + s];
-                    int oi = ri[tid + s];
-                    if (o > rv[tid] || (o == rv[tid] && oi < ri[tid])) { rv[tid] = o; ri[tid] = oi; }
-                }
-                __syncthreads();
-            }
-            const int p = ri[0];
-            if (tid == 0) piv[j] = p;
-            const double pvt = D[p + (long)na * j];
-            if (pvt == 0.0) sing = sing ? sing : j + 1;
-            __syncthreads();
-            if (p != j)
-                for (int k = tid; k < na; k += nth) {
-                    double a = D[j + (long)na * k];
-                    D[j + (long)na * k] = D[p + (long)na * k];
-                    D[p + (long)na * k] = a;
-                }
-            __syncthreads();
-            if (pvt != 0.0) {
-                const double rinv = 1.0 / pvt;
-                for (int i = j + 1 + tid; i < na; i += nth) D[i + (long)na * j] *= rinv;
-            }
-            __syncthreads();
-            const int w = na - j - 1;
-            for (int idx = tid; idx < w * w; idx += nth) {
-                int i = j + 1 + idx % w, k = j + 1 + idx / w;
-                D[i + (long)na * k] -= D[i + (long)na * j] * D[j + (long)na * k];
-            }
-            __syncthreads();
-        }
-        if (tid == 0 && info) info[e] = sing;
-        // ---- solve: columns of F and Ru, one column per thread ----
-        for (int c = tid; c < nb + 1; c += nth) {
-            double *b = c < nb ? F + (long)na * c : u;
-            for (int j = 0; j < na; j++) {
-                int p = piv[j];
-                if (p != j) { double a = b[j]; b[j] = b[p]; b[p] = a; }
-            }
-            for (int j = 0; j < na; j++) {
-                double bj = b[j];
-                for (int i = j + 1; i < na; i++) b[i] -= D[i + (long)na * j] * bj;
-            }
-            for (int j = na - 1; j >= 0; j--) {
-                b[j] /= D[j + (long)na * j];
-                double bj = b[j];
-                for (int i = 0; i < j; i++) b[i] -= D[i + (long)na * j] * bj;
-            }
-        }
-        __syncthreads();
-        // ---- Schur complement: H -= K F, FH = Rh - K u ----
-        for (long idx = tid; idx < (long)nb * nb + nb; idx += nth) {
-            if (idx < (long)nb * nb) {
-                int r = idx % nb, c = idx / nb;
-                double s = 0;
-                for (int k = 0; k < na; k++) s += K[r + (long)nb * k] * F[k + (long)na * c];
-                H[idx] -= s;
-            } else {
-                int r = idx - (long)nb * nb;
-                double s = 0;
-                for (int k = 0; k < na; k++) s += K[r + (long)nb * k] * u[k];
-                FH[e * nb + r] = Rh[e * nb + r] - s;
-            }
-        }
-        __syncthreads();
-    }
-}
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-hipError_t launch_condense_fp64(int na, int nb, int e0, int n, double *D, double *F, const double *K, double *H,
-                                double *Ru, const double *Rh, double *FH, int *info, int grid, hipStream_t stream)
+__device__ __forceinline__ f64x4 mfma64(double a, double b, f64x4 c)
 {
-    hipLaunchKernelGGL(condense_fp64_kernel, dim3(grid), dim3(256), 0, stream, na, nb, e0, n, D, F, K, H, Ru, Rh, FH,
-                       info);
-    return hipGetLastError();
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
-
+__device__ __forceinline__ f32x4 mfma32(float a, float b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
 
 // ---------------------------------------------------------------------------
-// Mixed precision: FP32 LU (partial pivoting) of a copy of D, FP64 iterative
-// refinement of every right-hand side against the FP64 D (residual F - D X in
-// FP64), FP64 Schur product.  Work area per workgroup (floats):
-//   D32[na*na] | X[na*(nb+1)] doubles | R[na*(nb+1)] doubles | T[na*(nb+1)] floats
+// Gauss-Jordan inverse in registers.  Thread (ti, tj) = (tid % 16, tid / 16)
+// owns rows ti + 16 r and columns tj + 16 c (r, c < R).  On exit a[r][c] holds
+// inv(P D) and perm[] the composed row permutation, so that
+// inv(D)[:, perm[m]] = a[:, m].  T is the element type of the elimination.
 // ---------------------------------------------------------------------------
-long mixed_work_floats(int na, int nb)
+template <int R, typename T>
+__device__ __forceinline__ int gauss_jordan(int na, T (&a)[R][R], T *colbuf, T *rowJ, T *rowP, int *perm)
 {
-    long w = ((long)na * na + 63) / 64 * 64 + 5L * na * (nb + 1);
-    return (w + 63) / 64 * 64;
+    const int tid = threadIdx.x, ti = tid & 15, tj = tid >> 4, lane = tid & 63;
+    int sing = 0;
+    for (int i = tid; i < na; i += blockDim.x) perm[i] = i;
+    __syncthreads();
+    for (int j = 0; j < na; j++) {
+        const int b = j & 1;
+        T *cb = colbuf + b * 192, *rj = rowJ + b * 192, *rp = rowP + b * 192;
+        if (tj == (j & 15)) {
+#pragma unroll
+            for (int c = 0; c < R; c++)
+                if (c == (j >> 4))
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        int i = ti + 16 * r;
+                        if (i < na) cb[i] = a[r][c];
+                    }
+        }
+        __syncthreads();
+        // pivot: first index of max |cb[i]|, i >= j (every wave computes it)
+        T best = (T)-1;
+        int bi = na;
+        for (int i = j + lane; i < na; i += 64) {
+            T v = fabs(cb[i]);
+            if (v > best) { best = v; bi = i; }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            T ov = __shfl_xor(best, off);
+            int oi = __shfl_xor(bi, off);
+            if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+        }
+        const int p = bi;
+        // rows j and p through LDS
+        if (ti == (j & 15) || ti == (p & 15)) {
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                int i = ti + 16 * r;
+                if (i == j || i == p)
+#pragma unroll
+                    for (int c = 0; c < R; c++) {
+                        int k = tj + 16 * c;
+                        if (k < na) {
+                            if (i == j) rj[k] = a[r][c];
+                            if (i == p) rp[k] = a[r][c];
+                        }
+                    }
+            }
+        }
+        if (tid == 0 && p != j) { int t = perm[j]; perm[j] = perm[p]; perm[p] = t; }
+        __syncthreads();
+        const T pv = cb[p];
+        if (pv == (T)0) { if (!sing) sing = j + 1; }
+        const T rinv = pv == (T)0 ? (T)0 : (T)1 / pv;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int i = ti + 16 * r;
+            // column-j value of row i after the swap
+            const T f = (i == p) ? cb[j] : cb[i < na ? i : 0];
+#pragma unroll
+            for (int c = 0; c < R; c++) {
+                const int k = tj + 16 * c;
+                if (i >= na || k >= na) continue;
+                const T prow = rp[k];                       // pivot row (old row p)
+                if (i == j) a[r][c] = (k == j) ? rinv : prow * rinv;
+                else {
+                    const T src = (i == p) ? rj[k] : a[r][c];
+                    a[r][c] = (k == j) ? -f * rinv : src - f * (prow * rinv);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    return sing;
 }
 
-__device__ static void solve32(int na, const float *__restrict__ L, const int *piv, const double *rhs,
-                               double *x, float *__restrict__ tmp)
+// 32 x 32 output tile per wave: C(i0.., j0..) = sum_k A(i,k) B(k,j), k < kdim,
+// with masked rows/columns.  Loader functors return 0 outside the matrix.
+// Operand layout of the 16x16x4 MFMAs: lane l supplies A(l%16, l/16) and
+// B(l/16, l%16).  Result layout (measured on gfx950): FP64 -- lane l, element v
+// holds C(l/16 + 4v, l%16); FP32 -- C(4(l/16) + v, l%16).
+template <typename LA, typename LB>
+__device__ __forceinline__ void mfma_tile64(int kdim, int i0, int j0, LA la, LB lb, f64x4 (&acc)[2][2])
 {
-    // tmp: na floats (thread-private slice of the work area)
-    for (int j = 0; j < na; j++) tmp[j] = (float)rhs[j];
-    for (int j = 0; j < na; j++) {
-        int p = piv[j];
-        if (p != j) { float a = tmp[j]; tmp[j] = tmp[p]; tmp[p] = a; }
+    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++) acc[x][y] = (f64x4){0.0, 0.0, 0.0, 0.0};
+    double a0 = la(i0 + lr, lk), a1 = la(i0 + 16 + lr, lk);
+    double b0 = lb(lk, j0 + lr), b1 = lb(lk, j0 + 16 + lr);
+    for (int k0 = 0; k0 < kdim; k0 += 4) {
+        const int kn = k0 + 4 + lk;
+        double na0 = la(i0 + lr, kn), na1 = la(i0 + 16 + lr, kn);
+        double nb0 = lb(kn, j0 + lr), nb1 = lb(kn, j0 + 16 + lr);
+        acc[0][0] = mfma64(a0, b0, acc[0][0]);
+        acc[0][1] = mfma64(a0, b1, acc[0][1]);
+        acc[1][0] = mfma64(a1, b0, acc[1][0]);
+        acc[1][1] = mfma64(a1, b1, acc[1][1]);
+        a0 = na0; a1 = na1; b0 = nb0; b1 = nb1;
     }
-    for (int j = 0; j < na; j++) {
-        float bj = tmp[j];
-        for (int i = j + 1; i < na; i++) tmp[i] -= L[i + (long)na * j] * bj;
-    }
-    for (int j = na - 1; j >= 0; j--) {
-        tmp[j] /= L[j + (long)na * j];
-        float bj = tmp[j];
-        for (int i = 0; i < j; i++) tmp[i] -= L[i + (long)na * j] * bj;
-    }
-    for (int j = 0; j < na; j++) x[j] = (double)tmp[j];
 }
 
+template <typename LA, typename LB>
+__device__ __forceinline__ void mfma_tile32(int kdim, int i0, int j0, LA la, LB lb, f32x4 (&acc)[2][2])
+{
+    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++) acc[x][y] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < kdim; k0 += 4) {
+        const int k = k0 + lk;
+        float a0 = la(i0 + lr, k), a1 = la(i0 + 16 + lr, k);
+        float b0 = lb(k, j0 + lr), b1 = lb(k, j0 + 16 + lr);
+        acc[0][0] = mfma32(a0, b0, acc[0][0]);
+        acc[0][1] = mfma32(a0, b1, acc[0][1]);
+        acc[1][0] = mfma32(a1, b0, acc[1][0]);
+        acc[1][1] = mfma32(a1, b1, acc[1][1]);
+    }
+}
+
+// LDS leading dimension of inv(D) (column-major): >= na, = 16 (mod 32) so that
+// the four 16-lane groups of an MFMA operand load fall in distinct banks.
+__host__ __device__ inline int dinv_ld(int na)
+{
+    int l = (na + 15) / 16 * 16;
+    return (l % 32 == 0) ? l + 16 : l;
+}
+
+// LDS bytes: three double-buffered exchange vectors of 192, the permutation,
+// and (optionally) inv(D).
+size_t condense_lds_bytes(int na, bool dinv_in_lds, int elt)
+{
+    size_t head = (size_t)6 * 192 * elt + 192 * sizeof(int);
+    head = (head + 15) / 16 * 16;
+    size_t m = dinv_in_lds ? (size_t)dinv_ld(na) * na * elt : 0;
+    return head + m;
+}
+
+__device__ __forceinline__ size_t lds_head(int elt)
+{
+    size_t head = (size_t)6 * 192 * elt + 192 * sizeof(int);
+    return (head + 15) / 16 * 16;
+}
+
+// Stage 3 (shared by both precisions): Hx = Hx - K X over all (nb+1) columns.
+__device__ __forceinline__ void schur_update(int na, int nb, int ldx, const double *__restrict__ K,
+                                             const double *__restrict__ X, double *__restrict__ H)
+{
+    const int wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    const int lane = threadIdx.x & 63, lr = lane & 15, lg = lane >> 4;
+    const int nbx = nb + 1;
+    const int tm = (nb + 31) / 32, tn = (nbx + 31) / 32;
+    for (int tile = wave; tile < tm * tn; tile += nwave) {
+        const int i0 = (tile % tm) * 32, j0 = (tile / tm) * 32;
+        f64x4 acc[2][2];
+        mfma_tile64(
+            na, i0, j0,
+            [&](int i, int k) { return (i < nb && k < na) ? K[i + (long)nb * k] : 0.0; },
+            [&](int k, int j) { return (k < na && j < nbx) ? X[(long)k * ldx + j] : 0.0; }, acc);
+#pragma unroll
+        for (int x = 0; x < 2; x++)
+#pragma unroll
+            for (int y = 0; y < 2; y++) {
+                const int j = j0 + 16 * y + lr;
+                if (j >= nbx) continue;
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int i = i0 + 16 * x + lg + 4 * v;
+                    if (i < nb) H[i + (long)nb * j] -= acc[x][y][v];
+                }
+            }
+    }
+}
+
+template <int R>
 __global__ void __launch_bounds__(256)
-condense_mixed_kernel(int na, int nb, int e0, int n, const double *__restrict__ Dm, double *__restrict__ Fm,
-                      const double *__restrict__ Km, double *__restrict__ Hm, double *__restrict__ Ru,
-                      const double *__restrict__ Rh, double *__restrict__ FH, int *__restrict__ info,
-                      float *__restrict__ work, long wstride, int sweeps)
+condense_fp64_kernel(int na, int nb, int ldx, int e0, int n, const double *__restrict__ Dm,
+                     double *__restrict__ Dinv_g, const double *__restrict__ Fx, double *__restrict__ Xm,
+                     const double *__restrict__ Km, double *__restrict__ Hx, int *__restrict__ info, int dinv_lds)
 {
-    __shared__ float rv[256];
-    __shared__ int ri[256];
-    __shared__ int piv[1024];
-    const int tid = threadIdx.x, nth = blockDim.x;
-    float *L = work + (long)blockIdx.x * wstride;
-    double *X = (double *)(L + ((long)na * na + 63) / 64 * 64);
-    double *R = X + (long)na * (nb + 1);
-    float *T = (float *)(R + (long)na * (nb + 1));
+    extern __shared__ double lds[];
+    double *colbuf = lds, *rowJ = lds + 2 * 192, *rowP = lds + 4 * 192;
+    int *perm = (int *)(lds + 6 * 192);
+    double *Ml = (double *)((char *)lds + lds_head(sizeof(double)));   // inv(D), column-major, ld ldl
+    const int ldl = dinv_ld(na);
+    const int tid = threadIdx.x, ti = tid & 15, tj = tid >> 4;
+    const int wave = tid >> 6, nwave = blockDim.x >> 6;
+    const int lane = tid & 63, lr = lane & 15, lg = lane >> 4;
     for (int t = blockIdx.x; t < n; t += gridDim.x) {
         const long e = e0 + t;
         const double *D = Dm + e * (long)na * na;
-        double *F = Fm + e * (long)na * nb;
-        const double *K = Km + e * (long)nb * na;
-        double *H = Hm + e * (long)nb * nb;
-        double *u = Ru + e * na;
-        int sing = 0;
-        for (long i = tid; i < (long)na * na; i += nth) L[i] = (float)D[i];
-        __syncthreads();
-        for (int j = 0; j < na; j++) {
-            float best = -1.0f;
-            int bi = na;
-            for (int i = j + tid; i < na; i += nth) {
-                float v = fabsf(L[i + (long)na * j]);
-                if (v > best) { best = v; bi = i; }
+        double a[R][R];
+#pragma unroll
+        for (int c = 0; c < R; c++)
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                int i = ti + 16 * r, k = tj + 16 * c;
+                a[r][c] = (i < na && k < na) ? D[i + (long)na * k] : 0.0;
             }
-            rv[tid] = best;
-            ri[tid] = bi;
-            __syncthreads();
-            for (int s = nth / 2; s > 0; s >>= 1) {
-                if (tid < s) {
-                    float o = rv[tid + s];
-                    int oi = ri[tid + s];
-                    if (o > rv[tid] || (o == rv[tid] && oi < ri[tid])) { rv[tid] = o; ri[tid] = oi; }
-                }
-                __syncthreads();
-            }
-            const int p = ri[0];
-            if (tid == 0) piv[j] = p;
-            const float pvt = L[p + (long)na * j];
-            if (pvt == 0.0f) sing = sing ? sing : j + 1;
-            __syncthreads();
-            if (p != j)
-                for (int k = tid; k < na; k += nth) {
-                    float a = L[j + (long)na * k];
-                    L[j + (long)na * k] = L[p + (long)na * k];
-                    L[p + (long)na * k] = a;
-                }
-            __syncthreads();
-            if (pvt != 0.0f) {
-                const float rinv = 1.0f / pvt;
-                for (int i = j + 1 + tid; i < na; i += nth) L[i + (long)na * j] *= rinv;
-            }
-            __syncthreads();
-            const int w = na - j - 1;
-            for (int idx = tid; idx < w * w; idx += nth) {
-                int i = j + 1 + idx % w, k = j + 1 + idx / w;
-                L[i + (long)na * k] -= L[i + (long)na * j] * L[j + (long)na * k];
-            }
-            __syncthreads();
-        }
+        int sing = gauss_jordan<R, double>(na, a, colbuf, rowJ, rowP, perm);
         if (tid == 0 && info) info[e] = sing;
-        for (int c = tid; c < nb + 1; c += nth) {
-            const double *b = c < nb ? F + (long)na * c : u;
-            double *x = X + (long)na * c, *r = R + (long)na * c;
-            float *tmp = T + (long)na * c;
-            solve32(na, L, piv, b, x, tmp);
-            for (int s = 0; s < sweeps; s++) {
-                // FP64 residual against the FP64 D, FP32 correction
-                for (int i = 0; i < na; i++) r[i] = b[i];
-                for (int k = 0; k < na; k++) {
-                    const double xk = x[k];
-                    for (int i = 0; i < na; i++) r[i] -= D[i + (long)na * k] * xk;
+        // inv(D)[:, perm[m]] = a[:, m]
+        double *Mi = dinv_lds ? Ml : Dinv_g + (long)blockIdx.x * na * ldl;
+#pragma unroll
+        for (int c = 0; c < R; c++)
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                int i = ti + 16 * r, m = tj + 16 * c;
+                if (i < na && m < na) Mi[i + (long)ldl * perm[m]] = a[r][c];
+            }
+        __syncthreads();
+        // X = inv(D) Fx  (na x (nb+1))
+        const double *F = Fx + e * (long)na * ldx;
+        double *X = Xm + e * (long)na * ldx;
+        const int nbx = nb + 1, tm = (na + 31) / 32, tn = (nbx + 31) / 32;
+        for (int tile = wave; tile < tm * tn; tile += nwave) {
+            const int i0 = (tile % tm) * 32, j0 = (tile / tm) * 32;
+            f64x4 acc[2][2];
+            mfma_tile64(
+                na, i0, j0, [&](int i, int k) { return (i < na && k < na) ? Mi[i + (long)ldl * k] : 0.0; },
+                [&](int k, int j) { return (k < na && j < nbx) ? F[(long)k * ldx + j] : 0.0; }, acc);
+#pragma unroll
+            for (int x = 0; x < 2; x++)
+#pragma unroll
+                for (int y = 0; y < 2; y++) {
+                    const int j = j0 + 16 * y + lr;
+                    if (j >= nbx) continue;
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        const int i = i0 + 16 * x + lg + 4 * v;
+                        if (i < na) X[(long)i * ldx + j] = acc[x][y][v];
+                    }
                 }
-                solve32(na, L, piv, r, r, tmp);
-                for (int i = 0; i < na; i++) x[i] += r[i];
-            }
         }
-        __syncthreads();
-        for (int c = tid; c < nb + 1; c += nth) {
-            double *b = c < nb ? F + (long)na * c : u;
-            const double *x = X + (long)na * c;
-            for (int i = 0; i < na; i++) b[i] = x[i];
-        }
-        __syncthreads();
-        for (long idx = tid; idx < (long)nb * nb + nb; idx += nth) {
-            if (idx < (long)nb * nb) {
-                int r = idx % nb, c = idx / nb;
-                double s = 0;
-                for (int k = 0; k < na; k++) s += K[r + (long)nb * k] * F[k + (long)na * c];
-                H[idx] -= s;
-            } else {
-                int r = idx - (long)nb * nb;
-                double s = 0;
-                for (int k = 0; k < na; k++) s += K[r + (long)nb * k] * u[k];
-                FH[e * nb + r] = Rh[e * nb + r] - s;
-            }
-        }
+        __syncthreads();   // X complete (global writes of this workgroup) before the Schur product
+        schur_update(na, nb, ldx, Km + e * (long)nb * na, X, Hx + e * (long)nb * (nb + 1));
         __syncthreads();
     }
 }
 
-hipError_t launch_condense_mixed(int npv, int ncu, int na, int nb, int e0, int n, double *D, double *F,
-                                 const double *K, double *H, double *Ru, const double *Rh, double *FH, int *info,
-                                 float *work32, long wstride, int grid, int sweeps, hipStream_t s)
+// Mixed: inv(D) in FP32 (GJ in FP32), X0 = inv32 Fx (FP32 MFMA), then `sweeps`
+// refinements X += inv32 (Fx - D X) with the residual in FP64 (FP64 MFMA),
+// then the FP64 Schur product.
+template <int R>
+__global__ void __launch_bounds__(256)
+condense_mixed_kernel(int na, int nb, int ldx, int e0, int n, const double *__restrict__ Dm,
+                      const double *__restrict__ Fx, double *__restrict__ Xm, double *__restrict__ Rw,
+                      const double *__restrict__ Km, double *__restrict__ Hx, int *__restrict__ info, int sweeps)
 {
-    (void)npv;
-    (void)ncu;
-    hipLaunchKernelGGL(condense_mixed_kernel, dim3(grid), dim3(256), 0, s, na, nb, e0, n, D, F, K, H, Ru, Rh, FH,
-                       info, work32, wstride, sweeps);
+    extern __shared__ double lds[];
+    float *fl = (float *)lds;
+    float *colbuf = fl, *rowJ = fl + 2 * 192, *rowP = fl + 4 * 192;
+    int *perm = (int *)(fl + 6 * 192);
+    float *Ml = (float *)((char *)lds + lds_head(sizeof(float)));
+    const int ldl = dinv_ld(na);
+    const int tid = threadIdx.x, ti = tid & 15, tj = tid >> 4;
+    const int wave = tid >> 6, nwave = blockDim.x >> 6;
+    const int lane = tid & 63, lr = lane & 15, lg = lane >> 4;
+    double *Rl = Rw + (long)blockIdx.x * na * ldx;   // per-workgroup FP64 residual
+    for (int t = blockIdx.x; t < n; t += gridDim.x) {
+        const long e = e0 + t;
+        const double *D = Dm + e * (long)na * na;
+        float a[R][R];
+#pragma unroll
+        for (int c = 0; c < R; c++)
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                int i = ti + 16 * r, k = tj + 16 * c;
+                a[r][c] = (i < na && k < na) ? (float)D[i + (long)na * k] : 0.f;
+            }
+        int sing = gauss_jordan<R, float>(na, a, colbuf, rowJ, rowP, perm);
+        if (tid == 0 && info) info[e] = sing;
+#pragma unroll
+        for (int c = 0; c < R; c++)
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                int i = ti + 16 * r, m = tj + 16 * c;
+                if (i < na && m < na) Ml[i + ldl * perm[m]] = a[r][c];
+            }
+        __syncthreads();
+        const double *F = Fx + e * (long)na * ldx;
+        double *X = Xm + e * (long)na * ldx;
+        const int nbx = nb + 1, tm = (na + 31) / 32, tn = (nbx + 31) / 32;
+        for (int s = 0; s <= sweeps; s++) {
+            // right-hand side: F (s == 0) or the FP64 residual
+            const double *B = s == 0 ? F : Rl;
+            for (int tile = wave; tile < tm * tn; tile += nwave) {
+                const int i0 = (tile % tm) * 32, j0 = (tile / tm) * 32;
+                f32x4 acc[2][2];
+                mfma_tile32(
+                    na, i0, j0, [&](int i, int k) { return (i < na && k < na) ? Ml[i + ldl * k] : 0.f; },
+                    [&](int k, int j) { return (k < na && j < nbx) ? (float)B[(long)k * ldx + j] : 0.f; }, acc);
+#pragma unroll
+                for (int x = 0; x < 2; x++)
+#pragma unroll
+                    for (int y = 0; y < 2; y++) {
+                        const int j = j0 + 16 * y + lr;
+                        if (j >= nbx) continue;
+#pragma unroll
+                        for (int v = 0; v < 4; v++) {
+                            const int i = i0 + 16 * x + 4 * lg + v;
+                            if (i < na) {
+                                if (s == 0) X[(long)i * ldx + j] = (double)acc[x][y][v];
+                                else X[(long)i * ldx + j] += (double)acc[x][y][v];
+                            }
+                        }
+                    }
+            }
+            __syncthreads();
+            if (s == sweeps) break;
+            // FP64 residual R = F - D X
+            for (int tile = wave; tile < tm * tn; tile += nwave) {
+                const int i0 = (tile % tm) * 32, j0 = (tile / tm) * 32;
+                f64x4 acc[2][2];
+                mfma_tile64(
+                    na, i0, j0, [&](int i, int k) { return (i < na && k < na) ? D[i + (long)na * k] : 0.0; },
+                    [&](int k, int j) { return (k < na && j < nbx) ? X[(long)k * ldx + j] : 0.0; }, acc);
+#pragma unroll
+                for (int x = 0; x < 2; x++)
+#pragma unroll
+                    for (int y = 0; y < 2; y++) {
+                        const int j = j0 + 16 * y + lr;
+                        if (j >= nbx) continue;
+#pragma unroll
+                        for (int v = 0; v < 4; v++) {
+                            const int i = i0 + 16 * x + lg + 4 * v;
+                            if (i < na) Rl[(long)i * ldx + j] = F[(long)i * ldx + j] - acc[x][y][v];
+                        }
+                    }
+            }
+            __syncthreads();
+        }
+        schur_update(na, nb, ldx, Km + e * (long)nb * na, X, Hx + e * (long)nb * (nb + 1));
+        __syncthreads();
+    }
+}
+
+int condense_dinv_ld(int na) { return dinv_ld(na); }
+
+int condense_rows_per_thread(int na) { return na <= 32 ? 2 : (na + 15) / 16; }
+
+#define HFEM_R_CASES(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11)
+
+hipError_t launch_condense_fp64(int na, int nb, int ldx, int e0, int n, const double *D, double *Dinv_g,
+                                const double *Fx, double *X, const double *K, double *Hx, int *info, int grid,
+                                bool dinv_lds, hipStream_t s)
+{
+    const int R = condense_rows_per_thread(na);
+    const size_t lds = condense_lds_bytes(na, dinv_lds, sizeof(double));
+    switch (R) {
+#define CASE(RR)                                                                                                   \
+    case RR:                                                                                                       \
+        hipFuncSetAttribute((const void *)condense_fp64_kernel<RR>, hipFuncAttributeMaxDynamicSharedMemorySize,   \
+                            (int)lds);                                                                             \
+        hipLaunchKernelGGL(condense_fp64_kernel<RR>, dim3(grid), dim3(256), lds, s, na, nb, ldx, e0, n, D, Dinv_g, \
+                           Fx, X, K, Hx, info, dinv_lds ? 1 : 0);                                                  \
+        break;
+        HFEM_R_CASES(CASE)
+#undef CASE
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_condense_mixed(int na, int nb, int ldx, int e0, int n, const double *D, const double *Fx, double *X,
+                                 double *Rw, const double *K, double *Hx, int *info, int grid, int sweeps,
+                                 hipStream_t s)
+{
+    const int R = condense_rows_per_thread(na);
+    const size_t lds = condense_lds_bytes(na, true, sizeof(float));
+    switch (R) {
+#define CASE(RR)                                                                                                    \
+    case RR:                                                                                                        \
+        hipFuncSetAttribute((const void *)condense_mixed_kernel<RR>, hipFuncAttributeMaxDynamicSharedMemorySize,   \
+                            (int)lds);                                                                              \
+        hipLaunchKernelGGL(condense_mixed_kernel<RR>, dim3(grid), dim3(256), lds, s, na, nb, ldx, e0, n, D, Fx, X,  \
+                           Rw, K, Hx, info, sweeps);                                                                \
+        break;
+        HFEM_R_CASES(CASE)
+#undef CASE
+    default:
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

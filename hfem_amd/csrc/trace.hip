@@ -14,9 +14,9 @@ namespace hfem {
 
 // einv[e*ndf + a*npf + m] = element-local node of face a that sits at canonical
 // position m of the global face (the inverse of elcon - face*npf).
-__global__ void bsr_gather_kernel(int nblocks, int bsz, int nch, int npf, int nb, const int *__restrict__ blkptr,
-                                  const int *__restrict__ blklist, const int *__restrict__ einv,
-                                  const double *__restrict__ AH, double *__restrict__ Hg)
+__global__ void bsr_gather_kernel(int nblocks, int bsz, int nch, int npf, int nb, long estride,
+                                  const int *__restrict__ blkptr, const int *__restrict__ blklist,
+                                  const int *__restrict__ einv, const double *__restrict__ AH, double *__restrict__ Hg)
 {
     const long total = (long)nblocks * bsz * bsz;
     const int ndf = nb / nch;
@@ -29,15 +29,16 @@ __global__ void bsr_gather_kernel(int nblocks, int bsz, int nch, int npf, int nb
             const int e = blklist[3 * q], a = blklist[3 * q + 1], b = blklist[3 * q + 2];
             const int li = einv[(long)e * ndf + a * npf + m1], lj = einv[(long)e * ndf + b * npf + m2];
             const long r = a * bsz + li * nch + n1, c = b * bsz + lj * nch + n2;
-            s += AH[(long)e * nb * nb + c * nb + r];
+            s += AH[(long)e * estride + c * nb + r];
         }
         Hg[idx] = s;
     }
 }
 
 // Rg/r[i*bsz + m*nch + n] = sum over the (<=2) elements of face i of the element vector.
-__global__ void face_gather_kernel(int nf, int bsz, int nch, int npf, int nb, const int *__restrict__ f2e,
-                                   const int *__restrict__ einv, const double *__restrict__ V1, double *__restrict__ out1,
+__global__ void face_gather_kernel(int nf, int bsz, int nch, int npf, int nb, long estride,
+                                   const int *__restrict__ f2e, const int *__restrict__ einv,
+                                   const double *__restrict__ V1, double *__restrict__ out1,
                                    const double *__restrict__ V2, double *__restrict__ out2)
 {
     const long total = (long)nf * bsz;
@@ -49,7 +50,7 @@ __global__ void face_gather_kernel(int nf, int bsz, int nch, int npf, int nb, co
             const int e = f2e[4 * i + 2 * side], a = f2e[4 * i + 2 * side + 1];
             if (e < 0) continue;
             const int li = einv[(long)e * ndf + a * npf + m];
-            const long off = (long)e * nb + a * bsz + li * nch + nn;
+            const long off = (long)e * estride + a * bsz + li * nch + nn;
             s1 += V1[off];
             if (V2) s2 += V2[off];
         }
@@ -97,19 +98,19 @@ static int grid_for(long total, int block = 256, int cap = 65536)
     return (int)(g < 1 ? 1 : (g > cap ? cap : g));
 }
 
-hipError_t launch_bsr_gather(int nblocks, int bsz, int nch, int npf, int nb, const int *blkptr, const int *blklist,
-                             const int *einv, const double *AH, double *Hg, hipStream_t s)
+hipError_t launch_bsr_gather(int nblocks, int bsz, int nch, int npf, int nb, long estride, const int *blkptr,
+                             const int *blklist, const int *einv, const double *AH, double *Hg, hipStream_t s)
 {
     hipLaunchKernelGGL(bsr_gather_kernel, dim3(grid_for((long)nblocks * bsz * bsz)), dim3(256), 0, s, nblocks, bsz,
-                       nch, npf, nb, blkptr, blklist, einv, AH, Hg);
+                       nch, npf, nb, estride, blkptr, blklist, einv, AH, Hg);
     return hipGetLastError();
 }
 
-hipError_t launch_face_gather(int nf, int bsz, int nch, int npf, int nb, const int *f2e, const int *einv,
-                              const double *V1, double *o1, const double *V2, double *o2, hipStream_t s)
+hipError_t launch_face_gather(int nf, int bsz, int nch, int npf, int nb, long estride, const int *f2e,
+                              const int *einv, const double *V1, double *o1, const double *V2, double *o2, hipStream_t s)
 {
     hipLaunchKernelGGL(face_gather_kernel, dim3(grid_for((long)nf * bsz)), dim3(256), 0, s, nf, bsz, nch, npf, nb,
-                       f2e, einv, V1, o1, V2, o2);
+                       estride, f2e, einv, V1, o1, V2, o2);
     return hipGetLastError();
 }
 

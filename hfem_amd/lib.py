@@ -46,15 +46,14 @@ EXPORTS = {
     "hfem_assemble": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "hfem_condense": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "hfem_read_elements": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "hfem_read_condensed": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _ip]),
     "hfem_assemble_linear_system": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "hfem_read_system": (ctypes.c_int, [_vp, _dp, _dp]),
     "hfem_bsr_matvec": (ctypes.c_int, [_vp, _dp, _dp]),
     "hfem_last_kernel_ms": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
     "hfem_sync": (ctypes.c_int, [_vp]),
 }
-EXTRA_EXPORTS = {
-    "hfem_read_fh": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _ip]),
-}
+EXTRA_EXPORTS = {}
 
 
 def load():
@@ -159,12 +158,18 @@ class HDGContext:
         _check(load().hfem_read_elements(self.h, e0, n, *args))
         return out
 
-    def read_fh(self, e0=0, n=None):
+    def read_condensed(self, e0=0, n=None):
+        """DinvRu, DinvF, AH, FH (element-major, reference layouts) and the pivot flags."""
+        d = self.d
         n = self.prob.ne - e0 if n is None else n
-        FH = np.zeros((n, self.d["nb"]))
+        na, nb = d["na"], d["nb"]
+        out = dict(DinvRu=np.zeros((n, na)), DinvF=np.zeros((n, na * nb)), AH=np.zeros((n, nb * nb)),
+                   FH=np.zeros((n, nb)))
         info = np.zeros(n, dtype=np.int32)
-        _check(load().hfem_read_fh(self.h, e0, n, _d(FH), _i(info)))
-        return FH, info
+        _check(load().hfem_read_condensed(self.h, e0, n, _d(out["DinvRu"]), _d(out["DinvF"]), _d(out["AH"]),
+                                          _d(out["FH"]), _i(info)))
+        out["info"] = info
+        return out
 
     def kernel_ms(self):
         a = ctypes.c_float()

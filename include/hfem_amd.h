@@ -10,9 +10,6 @@
  *   hfem_condense        schurElementMatrixVector     assembly/elementschur.cpp:331-343
  *   hfem_assemble_linear_system
  *                        assembleLinearSystem         assembly/globalassembly.cpp:423-556
- *   hfem_residual_norm   computeResidualNorm          assembly/globalassembly.cpp:618-678
- *   hfem_get_q           getQ                         assembly/elementassembly.cpp:1414-1479
- *   hfem_update_sol      updateSol / getU             solvers/solutionupdate.cpp:6-98
  *   hfem_bsr_matvec      matrix-vector product (closed in C++; spec kernel/matvec.m:1-12)
  *
  * Conventions: every function returns 0 on success and a negative code on
@@ -55,7 +52,7 @@ enum {
 const char *hfem_last_error(void);
 
 /* Create a context on HIP device `device`, allocating the element block storage
- * for dims->ne elements (D, F, K, H, Ru, Rh; sol.DinvF/DinvRu live in F/Ru). */
+ * for dims->ne elements (D, F, K, H, Ru, Rh and the condensed inv(D)F, inv(D)Ru). */
 int hfem_create(hfem_ctx **ctx, const hfem_dims *dims, int device);
 int hfem_destroy(hfem_ctx *ctx);
 
@@ -80,14 +77,22 @@ int hfem_get_solution(hfem_ctx *ctx, double *UDG, double *UH);
 
 /* Element assembly for elements [e0, e0+n) into the device element blocks. */
 int hfem_assemble(hfem_ctx *ctx, int e0, int n);
-/* Static condensation of elements [e0, e0+n), in place on the device blocks:
- * F -> inv(D)F, H -> AH, Ru -> inv(D)Ru, Rh -> FH. */
+/* Static condensation of elements [e0, e0+n) on the device blocks: inv(D)F,
+ * inv(D)Ru, AH (in place of H) and FH; D, Ru and Rh keep their assembled values. */
 int hfem_condense(hfem_ctx *ctx, int e0, int n, int precision);
 
-/* Copy element blocks of [e0, e0+n) to the host (any pointer may be NULL):
- * each element's block is contiguous and column-major like elemstruct. */
+/* Copy the assembled element blocks of [e0, e0+n) to the host (any pointer may
+ * be NULL), each element's block contiguous and column-major like elemstruct
+ * (Ru na, D na*na, F na*nb, K nb*na, H nb*nb, Rh nb).  H is overwritten by
+ * hfem_condense; read it before condensing. */
 int hfem_read_elements(hfem_ctx *ctx, int e0, int n, double *Ru, double *D, double *F, double *K, double *H,
                        double *Rh);
+/* Copy the condensation outputs of [e0, e0+n): inv(D)Ru (na), inv(D)F (na*nb),
+ * AH (nb*nb), FH (nb) -- sol.DinvRu/sol.DinvF and elem.H/elem.Rh after
+ * schurElementMatrixVector -- and a per-element singular-pivot flag (0 = ok,
+ * else 1 + first zero pivot). */
+int hfem_read_condensed(hfem_ctx *ctx, int e0, int n, double *DinvRu, double *DinvF, double *AH, double *FH,
+                        int *info);
 
 /* assembleLinearSystem: assemble + condense all elements, scatter AH/FH into the
  * BSR matrix Hg and right-hand side Rg (device), and return

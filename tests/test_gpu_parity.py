@@ -54,21 +54,19 @@ def test_assembly_and_condensation_vs_oracle(oracle_mod, kind, nc, p):
     b = {k: v.copy() for k, v in ref.items()}
     O.schur(0, d["npv"], d["ncu"], b)
     ctx.schur_element_matrix_vector(precision=FP64)
-    c = ctx.read_elements(which=("Ru", "F", "H"))
-    FH, info = ctx.read_fh()
-    assert np.all(info == 0)
-    for name, got, want, tol in [("DinvF", c["F"], b["F"], RTOL_MAT), ("AH", c["H"], b["H"], RTOL_MAT),
-                                 ("DinvRu", c["Ru"], b["Ru"], RTOL_RES), ("FH", FH, b["Rh"], RTOL_RES)]:
+    c = ctx.read_condensed()
+    assert np.all(c["info"] == 0)
+    for name, got, want, tol in [("DinvF", c["DinvF"], b["F"], RTOL_MAT), ("AH", c["AH"], b["H"], RTOL_MAT),
+                                 ("DinvRu", c["DinvRu"], b["Ru"], RTOL_RES), ("FH", c["FH"], b["Rh"], RTOL_RES)]:
         e = relerr(got, want)
         say(f"  fp64 condense {name}: {e:.2e}")
         assert e < tol, name
     # mixed precision on freshly assembled blocks
     ctx.assemble_element_matrix_vector()
     ctx.schur_element_matrix_vector(precision=MIXED)
-    c = ctx.read_elements(which=("Ru", "F", "H"))
-    FH, info = ctx.read_fh()
-    for name, got, want in [("DinvF", c["F"], b["F"]), ("AH", c["H"], b["H"]), ("DinvRu", c["Ru"], b["Ru"]),
-                            ("FH", FH, b["Rh"])]:
+    c = ctx.read_condensed()
+    for name, got, want in [("DinvF", c["DinvF"], b["F"]), ("AH", c["AH"], b["H"]), ("DinvRu", c["DinvRu"], b["Ru"]),
+                            ("FH", c["FH"], b["Rh"])]:
         e = relerr(got, want)
         say(f"  mixed condense {name}: {e:.2e}")
         assert e < RTOL_MIXED, name
@@ -90,11 +88,12 @@ def test_against_reference_golden(name, kind, nc, p):
     for k in ("D", "F", "K", "H", "Rh"):
         assert relerr(g[k][keep], gold[k]) < RTOL_MAT, k
     ctx.schur_element_matrix_vector(precision=FP64)
-    c = ctx.read_elements(which=("Ru", "H"))
-    FH, _ = ctx.read_fh()
-    assert relerr(c["H"][keep], gold["AH_impl0"]) < RTOL_MAT
-    assert relerr(FH[keep], gold["FH_impl0"]) < RTOL_RES
-    assert relerr(c["Ru"][keep], gold["DinvRu_impl0"]) < RTOL_RES
+    c = ctx.read_condensed()
+    assert relerr(c["AH"][keep], gold["AH_impl0"]) < RTOL_MAT
+    assert relerr(c["FH"][keep], gold["FH_impl0"]) < RTOL_RES
+    assert relerr(c["DinvRu"][keep], gold["DinvRu_impl0"]) < RTOL_RES
+    if "DinvF_impl0" in gold:
+        assert relerr(c["DinvF"][keep], gold["DinvF_impl0"]) < RTOL_MAT
     ctx.close()
 
 
