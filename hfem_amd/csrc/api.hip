@@ -22,6 +22,8 @@ hipError_t launch_condense_mixed(int na, int nb, int ldx, int e0, int n, const d
                                  double *Rw, const double *K, double *Hx, int *info, int grid, int sweeps,
                                  hipStream_t s);
 size_t condense_lds_bytes(int na, bool dinv_in_lds, int elt);
+size_t assemble_lds_bytes(const Dims &d);
+bool assemble_supported(int nd, int npv, int npf);
 int condense_dinv_ld(int na);
 hipError_t launch_bsr_gather(int, int, int, int, int, long, const int *, const int *, const int *, const double *,
                              double *, hipStream_t);
@@ -82,6 +84,7 @@ struct hfem_ctx {
     long sstride = 0;
     int grid_asm = 0, grid_cond = 0;
     bool dinv_lds = true;
+    int maxlds = 0;
     // trace system
     double *Hg = nullptr, *Rg = nullptr, *r = nullptr, *partial = nullptr, *xv = nullptr, *yv = nullptr;
     float ms_asm = 0, ms_cond = 0;
@@ -131,14 +134,24 @@ extern "C" int hfem_create(hfem_ctx **out, const hfem_dims *hd, int device)
     ALLOC(info, ne);
     ALLOC(partial, 1024);
     c->sstride = scratch_layout(d).total;
-    c->grid_asm = (int)std::min<long>(ne, 1024);
+    c->grid_asm = (int)std::min<long>(ne, 512);
     c->grid_cond = (int)std::min<long>(ne, 1024);
     ALLOC(scratch, (long)c->grid_asm * c->sstride);
     {
         int maxlds = 65536;
         if (hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
             maxlds = 65536;
+        c->maxlds = maxlds;
         c->dinv_lds = condense_lds_bytes(d.na, true, sizeof(double)) <= (size_t)maxlds;
+        if (assemble_lds_bytes(d) > (size_t)maxlds) {
+            delete c;
+            return fail(-1, "hfem_create: element family needs %zu bytes of LDS, device offers %d",
+                        assemble_lds_bytes(d), maxlds);
+        }
+        if (!assemble_supported(d.nd, d.npv, d.npf)) {
+            delete c;
+            return fail(-1, "hfem_create: no assembly kernel compiled for nd=%d npv=%d npf=%d", d.nd, d.npv, d.npf);
+        }
         if (condense_lds_bytes(d.na, true, sizeof(float)) > (size_t)maxlds) {
             delete c;
             return fail(-1, "hfem_create: na=%d needs more LDS than the device offers (%d bytes)", d.na, maxlds);

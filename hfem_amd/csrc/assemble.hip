@@ -6,28 +6,32 @@
 // layouts; F is written row-major into Fx (na x ldx) with Ru as column nb, and H
 // column-major into Hx (nb x (nb+1)) with Rh as column nb -- the layouts the
 // condensation kernel streams through its MFMA operands.
-// The work is organised as a sequence of phases; inside a phase every thread
-// computes independent entries, so no atomics are needed:
-//   P0 gather element data into LDS          P5 traces of M^{-1}C, M^{-1}E
-//   P1 volume/face geometry                   P6 flux Jacobians by forward-mode AD,
-//   P2 q operators M, C, E                       one Jacobian column per thread
-//   P3 Cholesky of M (LDS)                    P7 wrl = -[s_u jac; f_u . Xx]
-//   P4 M^{-1}C, M^{-1}E (column per thread)   P8 q-folded integrands (Wd, Wf, Zd, Zf)
-//                                             P9 contractions with the test functions
-// Element rows sum the faces that contain the row's node ("gather" form of the
-// reference's per-face scatter, assembly/elementassembly.cpp:862-924,1154-1318).
+//
+// Phases (barriers between them; inside a phase threads own disjoint outputs):
+//   P0 element data and the master tables in LDS  P5 traces of M^{-1}C, M^{-1}E;
+//   P1 volume/face geometry                          solution at quadrature points
+//   P2 q operators M, C, E (qint)                  P6 flux Jacobians by forward-mode AD,
+//   P3 Cholesky of M                                  one Jacobian column per thread
+//   P4 M^{-1}C, M^{-1}E (column per thread)        P7 wrl = -[s_u jac ; f_u . Xx]
+//   P8 element rows: one thread per column of D or F, all NPV rows in registers;
+//      the q-folded integrand (elementint :510-549, faceint :962-1072) is formed
+//      on the fly and contracted with the test functions, volume then faces
+//      (faces containing the row node: the gather form of faceint's scatter)
+//   P9 trace rows: one thread per (face, column) of K or H, npf rows in registers;
+//      Ru and Rh entries.
 #include "hfem_device.h"
 
 namespace hfem {
 
-#define SVT(g, n, dd) m.shapvt[(g) + d.ngv * ((n) + d.npv * (dd))]
-#define SVG(n, g, dd) m.shapvg[(n) + d.npv * ((g) + d.ngv * (dd))]
-#define SFT(g, k, dd) m.shapft[(g) + d.ngf * ((k) + d.npf * (dd))]
-#define SFG(k, g, dd) m.shapfg[(k) + d.npf * ((g) + d.ngf * (dd))]
-#define PERM(k, a) m.perm[(k) + d.npf * (a)]
-#define INVP(n, a) m.invperm[(n) + d.npv * (a)]
+// LDS copies of the master tables, transposed so that the thread-varying index is
+// fastest: svt[n + npv*(g + ngv*dd)], svg[i + npv*(g + ngv*dd)] (as shapvg),
+// sft[k + npf*(g + ngf*dd)], sfg[k + npf*(g + ngf*dd)] (as shapfg).
+struct LdsTables {
+    double *svt, *svg, *sft, *sfg;
+    int *perm, *invperm;
+};
 
-template <int ND>
+template <int ND, int NPV, int NPF>
 __global__ void __launch_bounds__(256)
 assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, const double *__restrict__ UDG,
                 const double *__restrict__ UH, const double *__restrict__ SH, const int *__restrict__ elcon,
@@ -38,30 +42,62 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
 {
     extern __shared__ double lds[];
     const int tid = threadIdx.x, nth = blockDim.x;
-    const int npv = d.npv, ngv = d.ngv, npf = d.npf, ngf = d.ngf, nfe = d.nfe, ndf = d.ndf;
-    const int ncu = d.ncu, nch = d.nch, nc = d.nc, na = d.na, nb = d.nb, nd1 = d.nd1, np = d.np;
-    constexpr int NCU = ND + 2, NC = NCU * (ND + 1);
+    constexpr int NCU = ND + 2, NC = NCU * (ND + 1), ND1 = ND + 1;
+    const int npv = NPV, ngv = d.ngv, npf = NPF, ngf = d.ngf, nfe = d.nfe, ndf = d.ndf;
+    const int ncu = NCU, nch = NCU, nc = NC, na = d.na, nb = d.nb, nd1 = ND1, np = d.np;
 
-    double *pn = lds;                         // npv*nd
-    double *udg = pn + npv * ND;              // npv*nc
-    double *uh = udg + npv * nc;              // ndf*nch  (uh[k*ndf + j])
-    double *sh = uh + ndf * nch;              // npv*ncu
-    double *jac = sh + npv * ncu;             // ngv
-    double *Xx = jac + ngv;                   // ngv*nd*nd   Xx(g,x,r) = jac d xi_r / d x_x
-    double *jacf = Xx + ngv * ND * ND;        // np
-    double *nl = jacf + np;                   // np*nd
-    double *M = nl + np * ND;                 // npv*npv
-    double *red = M + npv * npv;              // 256 (reduction)
-    int *bnd = (int *)(red + 256);            // nfe
+    // ---- LDS layout ----
+    double *p = lds;
+    double *svt = p; p += npv * ngv * nd1;
+    double *svg = p; p += npv * ngv * nd1;
+    double *sft = p; p += npf * ngf * ND;
+    double *sfg = p; p += npf * ngf * ND;
+    double *pn = p; p += npv * ND;
+    double *udg = p; p += npv * nc;
+    double *uh = p; p += ndf * nch;            // uh[k*ndf + j]
+    double *sh = p; p += npv * ncu;
+    double *jac = p; p += ngv;
+    double *Xx = p; p += ngv * ND * ND;        // Xx(g,x,r) = jac d xi_r / d x_x
+    double *jacf = p; p += np;
+    double *nl = p; p += np * ND;
+    double *M = p; p += npv * npv;
+    double *shapMiC = p; p += npv * ngv * ND;  // [n + npv*(g + ngv*x)]
+    double *shapMiE = p; p += ndf * ngv * ND;  // [l + ndf*(g + ngv*x)], includes the minus sign
+    double *udgg = p; p += ngv * nc;           // [g + ngv*c]
+    double *ufg = p; p += np * nc;             // u, q traced on the faces [p + np*c]
+    double *uhgl = p; p += np * nch;           // uh at face points [p + np*c]
+    double *red = p; p += 256;
+    int *perm = (int *)p;
+    int *invperm = perm + npf * nfe;
+    int *bnd = invperm + npv * nfe;
 
-    const Scratch L = scratch_layout(d);
+    // ---- per-workgroup scratch (global) ----
     double *S = scratch + (long)blockIdx.x * sstride;
-    double *C = S + L.C, *E = S + L.E, *shapMiC = S + L.shapMiC, *shapMiE = S + L.shapMiE;
-    double *shapMiCf = S + L.shapMiCf, *shapMiEf = S + L.shapMiEf, *f = S + L.f, *fu = S + L.fu;
-    double *fh = S + L.fh, *fhu = S + L.fhu, *fhuh = S + L.fhuh, *fb = S + L.fb, *fbu = S + L.fbu;
-    double *fbuh = S + L.fbuh, *wrl = S + L.wrl, *Wd = S + L.Wd, *Wf = S + L.Wf;
-    double *Zd = S + L.Zd, *ZdT = S + L.ZdT, *Zf = S + L.Zf, *ZfT = S + L.ZfT;
+    const Scratch L = scratch_layout(d);
+    double *C = S + L.C, *E = S + L.E, *shapMiCf = S + L.shapMiCf, *shapMiEf = S + L.shapMiEf;
+    double *f = S + L.f, *fu = S + L.fu, *fh = S + L.fh, *fhu = S + L.fhu, *fhuh = S + L.fhuh;
+    double *fb = S + L.fb, *fbu = S + L.fbu, *fbuh = S + L.fbuh, *wrl = S + L.wrl;
     const double tau = ph.param[5];
+
+    // master tables once per workgroup
+    for (int i = tid; i < npv * ngv * nd1; i += nth) {
+        const int nn = i % npv, r = i / npv, g = r % ngv, dd = r / ngv;
+        svt[i] = m.shapvt[g + ngv * (nn + npv * dd)];
+        svg[i] = m.shapvg[i];
+    }
+    for (int i = tid; i < npf * ngf * ND; i += nth) {
+        const int k = i % npf, r = i / npf, g = r % ngf, dd = r / ngf;
+        sft[i] = m.shapft[g + ngf * (k + npf * dd)];
+        sfg[i] = m.shapfg[i];
+    }
+    for (int i = tid; i < npf * nfe; i += nth) perm[i] = m.perm[i];
+    for (int i = tid; i < npv * nfe; i += nth) invperm[i] = m.invperm[i];
+#define SVT(g, nn, dd) svt[(nn) + npv * ((g) + ngv * (dd))]
+#define SVG(i, g, dd) svg[(i) + npv * ((g) + ngv * (dd))]
+#define SFT(g, k, dd) sft[(k) + npf * ((g) + ngf * (dd))]
+#define SFG(k, g, dd) sfg[(k) + npf * ((g) + ngf * (dd))]
+#define PERM(k, a) perm[(k) + npf * (a)]
+#define INVP(nn, a) invperm[(nn) + npv * (a)]
 
     for (int t = blockIdx.x; t < n; t += gridDim.x) {
         const long e = e0 + t;
@@ -104,7 +140,7 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                 for (int x = 0; x < ND; x++)
                     for (int r = 0; r < ND; r++) Xx[g + ngv * (x + ND * r)] = adj[r][x];
             } else {
-                const int p = i - ngv, g = p % ngf, a = p / ngf;
+                const int pp = i - ngv, g = pp % ngf, a = pp / ngf;
                 double tg[2][3];
                 for (int r = 0; r < ND - 1; r++)
                     for (int x = 0; x < ND; x++) {
@@ -122,8 +158,8 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                 double len = 0;
                 for (int x = 0; x < ND; x++) len += nv[x] * nv[x];
                 len = sqrt(len);
-                jacf[p] = len;
-                for (int x = 0; x < ND; x++) nl[p + np * x] = nv[x] / len;
+                jacf[pp] = len;
+                for (int x = 0; x < ND; x++) nl[pp + np * x] = nv[x] / len;
             }
         }
         __syncthreads();
@@ -149,8 +185,8 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                     double s = 0;
                     if (k >= 0)
                         for (int g = 0; g < ngf; g++) {
-                            int p = g + ngf * a;
-                            s += (SFG(k, g, 0) * SFT(g, j, 0)) * nl[p + np * x] * jacf[p];
+                            int pp = g + ngf * a;
+                            s += (SFG(k, g, 0) * SFT(g, j, 0)) * nl[pp + np * x] * jacf[pp];
                         }
                     E[r] = s;
                 }
@@ -176,48 +212,75 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
         // ---------------- P4: M^{-1} C, M^{-1} E (a column per thread) ----------------
         for (int col = tid; col < ND * (npv + ndf); col += nth) {
             double *b = col < ND * npv ? C + (long)col * npv : E + (long)(col - ND * npv) * npv;
-            for (int i = 0; i < npv; i++) {
-                double s = b[i];
-                for (int k = 0; k < i; k++) s -= M[k + npv * i] * b[k];
-                b[i] = s / M[i + npv * i];
+            double x[NPV];
+#pragma unroll
+            for (int i = 0; i < NPV; i++) x[i] = b[i];
+#pragma unroll
+            for (int i = 0; i < NPV; i++) {
+                double s = x[i];
+#pragma unroll
+                for (int k = 0; k < i; k++) s -= M[k + npv * i] * x[k];
+                x[i] = s / M[i + npv * i];
             }
-            for (int i = npv - 1; i >= 0; i--) {
-                double s = b[i];
-                for (int k = i + 1; k < npv; k++) s -= M[i + npv * k] * b[k];
-                b[i] = s / M[i + npv * i];
+#pragma unroll
+            for (int i = NPV - 1; i >= 0; i--) {
+                double s = x[i];
+#pragma unroll
+                for (int k = i + 1; k < NPV; k++) s -= M[i + npv * k] * x[k];
+                x[i] = s / M[i + npv * i];
             }
+#pragma unroll
+            for (int i = 0; i < NPV; i++) b[i] = x[i];
         }
         __syncthreads();
 
-        // ---------------- P5: traces of M^{-1}C, M^{-1}E ----------------
+        // ---------------- P5: traces of M^{-1}C, M^{-1}E; solution at points ----------------
         {
             const int n1 = ngv * npv * ND, n2 = ngv * ndf * ND, n3 = np * npv * ND, n4 = np * ndf * ND;
-            for (int i = tid; i < n1 + n2 + n3 + n4; i += nth) {
+            const int n5 = ngv * nc, n6 = np * nc, n7 = np * nch;
+            const int tot = n1 + n2 + n3 + n4 + n5 + n6 + n7;
+            for (int i = tid; i < tot; i += nth) {
                 if (i < n1) {
-                    int g = i % ngv, r = i / ngv, nn = r % npv, x = r / npv;
+                    int nn = i % npv, r = i / npv, g = r % ngv, x = r / ngv;
                     double s = 0;
                     for (int l = 0; l < npv; l++) s += SVT(g, l, 0) * C[x * npv * npv + nn * npv + l];
                     shapMiC[i] = s;
                 } else if (i < n1 + n2) {
-                    int r0 = i - n1, g = r0 % ngv, r = r0 / ngv, l = r % ndf, x = r / ndf;
+                    int r0 = i - n1, l = r0 % ndf, r = r0 / ndf, g = r % ngv, x = r / ngv;
                     double s = 0;
                     for (int k = 0; k < npv; k++) s += SVT(g, k, 0) * E[(long)x * npv * ndf + l * npv + k];
                     shapMiE[r0] = -s;
                 } else if (i < n1 + n2 + n3) {
-                    int r0 = i - n1 - n2, p = r0 % np, r = r0 / np, nn = r % npv, x = r / npv;
-                    int g = p % ngf, a = p / ngf;
+                    int r0 = i - n1 - n2, nn = r0 % npv, r = r0 / npv, pp = r % np, x = r / np;
+                    int g = pp % ngf, a = pp / ngf;
                     double s = 0;
                     for (int k = 0; k < npf; k++) s += SFT(g, k, 0) * C[x * npv * npv + nn * npv + PERM(k, a)];
                     shapMiCf[r0] = s;
-                } else {
-                    int r0 = i - n1 - n2 - n3, p = r0 % np, r = r0 / np, l = r % ndf, x = r / ndf;
-                    int g = p % ngf, a = p / ngf;
+                } else if (i < n1 + n2 + n3 + n4) {
+                    int r0 = i - n1 - n2 - n3, l = r0 % ndf, r = r0 / ndf, pp = r % np, x = r / np;
+                    int g = pp % ngf, a = pp / ngf;
                     double s = 0;
                     for (int k = 0; k < npf; k++) s += SFT(g, k, 0) * E[(long)x * npv * ndf + l * npv + PERM(k, a)];
                     shapMiEf[r0] = s;
+                } else if (i < n1 + n2 + n3 + n4 + n5) {
+                    int r0 = i - n1 - n2 - n3 - n4, g = r0 % ngv, c = r0 / ngv;
+                    double s = 0;
+                    for (int q = 0; q < npv; q++) s += SVT(g, q, 0) * udg[c * npv + q];
+                    udgg[r0] = s;
+                } else if (i < n1 + n2 + n3 + n4 + n5 + n6) {
+                    int r0 = i - n1 - n2 - n3 - n4 - n5, pp = r0 % np, c = r0 / np, g = pp % ngf, a = pp / ngf;
+                    double s = 0;
+                    for (int q = 0; q < npf; q++) s += SFT(g, q, 0) * udg[c * npv + PERM(q, a)];
+                    ufg[r0] = s;
+                } else {
+                    int r0 = i - n1 - n2 - n3 - n4 - n5 - n6, pp = r0 % np, c = r0 / np, g = pp % ngf, a = pp / ngf;
+                    double s = 0;
+                    for (int q = 0; q < npf; q++) s += SFT(g, q, 0) * uh[c * ndf + a * npf + q];
+                    uhgl[r0] = s;
                 }
             }
         }
+        __syncthreads();
 
         // ---------------- P6: fluxes and Jacobian columns ----------------
         {
@@ -226,72 +289,53 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                 if (i < nvol) {
                     const int g = i % ngv, k = i / ngv;
                     dual w[NC], fl[NCU * ND];
-                    for (int c = 0; c < NC; c++) {
-                        double s = 0;
-                        for (int q = 0; q < npv; q++) s += SVT(g, q, 0) * udg[c * npv + q];
-                        w[c] = mk(s, c == k ? 1.0 : 0.0);
-                    }
+                    for (int c = 0; c < NC; c++) w[c] = mk(udgg[g + ngv * c], c == k ? 1.0 : 0.0);
                     ns_flux<ND>(ph.param, w, fl);
                     for (int jx = 0; jx < NCU * ND; jx++) {
                         fu[g + ngv * (jx + NCU * ND * k)] = fl[jx].d;
                         if (k == 0) f[g + ngv * jx] = fl[jx].v;
                     }
                 } else if (i < nvol + nfac) {
-                    const int r = i - nvol, p = r % np, k = r / np, g = p % ngf, a = p / ngf;
+                    const int r = i - nvol, pp = r % np, k = r / np;
                     dual w[NC], fl[NCU * ND];
-                    double ug[NCU];
-                    for (int c = 0; c < NC; c++) {
-                        double s = 0;
-                        if (c < NCU) {
-                            double su = 0;
-                            for (int q = 0; q < npf; q++) {
-                                s += SFT(g, q, 0) * uh[c * ndf + a * npf + q];
-                                su += SFT(g, q, 0) * udg[c * npv + PERM(q, a)];
-                            }
-                            ug[c] = su;
-                        } else {
-                            for (int q = 0; q < npf; q++) s += SFT(g, q, 0) * udg[c * npv + PERM(q, a)];
-                        }
-                        w[c] = mk(s, c == k ? 1.0 : 0.0);
-                    }
+                    for (int c = 0; c < NC; c++)
+                        w[c] = mk(c < NCU ? uhgl[pp + np * c] : ufg[pp + np * c], c == k ? 1.0 : 0.0);
                     ns_flux<ND>(ph.param, w, fl);
                     for (int j = 0; j < NCU; j++) {
                         dual fn = mk(0.0);
-                        for (int x = 0; x < ND; x++) fn = fn + nl[p + np * x] * fl[j + NCU * x];
-                        fhu[p + np * (j + NCU * k)] = (k < NCU) ? (k == j ? tau : 0.0) : fn.d;
-                        if (k < NCU) fhuh[p + np * (j + NCU * k)] = fn.d - (k == j ? tau : 0.0);
-                        if (k == 0) fh[p + np * j] = fn.v + tau * (ug[j] - w[j].v);
+                        for (int x = 0; x < ND; x++) fn = fn + nl[pp + np * x] * fl[j + NCU * x];
+                        fhu[pp + np * (j + NCU * k)] = (k < NCU) ? (k == j ? tau : 0.0) : fn.d;
+                        if (k < NCU) fhuh[pp + np * (j + NCU * k)] = fn.d - (k == j ? tau : 0.0);
+                        if (k == 0) fh[pp + np * j] = fn.v + tau * (ufg[pp + np * j] - w[j].v);
                     }
                 } else {
-                    const int r = i - nvol - nfac, p = r % np, k = r / np, g = p % ngf, a = p / ngf;
+                    const int r = i - nvol - nfac, pp = r % np, k = r / np, a = pp / ngf;
                     if (!bnd[a]) continue;
-                    dual uhd[NCU], A0[NCU * NCU], A1[NCU * NCU];
-                    double ug[NCU], nrm[ND];
-                    for (int c = 0; c < NCU; c++) {
-                        double s = 0, su = 0;
-                        for (int q = 0; q < npf; q++) {
-                            s += SFT(g, q, 0) * uh[c * ndf + a * npf + q];
-                            su += SFT(g, q, 0) * udg[c * npv + PERM(q, a)];
-                        }
-                        uhd[c] = mk(s, c == k ? 1.0 : 0.0);
-                        ug[c] = su;
+                    // fb = (A+|A|)(u-uh) - (A-|A|)(ui-uh) = A (u-ui) + |A| (u+ui-2uh): one
+                    // Jacobian matrix live at a time
+                    dual uhd[NCU], Am[NCU * NCU], acc[NCU];
+                    double nrm[ND];
+                    for (int c = 0; c < NCU; c++) uhd[c] = mk(uhgl[pp + np * c], c == k ? 1.0 : 0.0);
+                    for (int x = 0; x < ND; x++) nrm[x] = nl[pp + np * x];
+                    for (int j = 0; j < NCU; j++) acc[j] = mk(0.0);
+                    for (int absflag = 0; absflag < 2; absflag++) {
+                        ns_normal_jac<ND>(ph.param[0], ph.param[1], uhd, nrm, absflag, Am);
+                        for (int j = 0; j < NCU; j++)
+                            for (int q = 0; q < NCU; q++) {
+                                const dual uq = mk(ufg[pp + np * q]);
+                                const dual w = absflag ? uq + mk(ph.ui[q]) - 2.0 * uhd[q] : uq - mk(ph.ui[q]);
+                                acc[j] = acc[j] + Am[q * NCU + j] * w;
+                                if (k == 0) {
+                                    double *o = &fbu[pp + np * (j + NCU * q)];
+                                    *o = absflag ? *o + Am[q * NCU + j].v : Am[q * NCU + j].v;
+                                }
+                            }
                     }
-                    for (int x = 0; x < ND; x++) nrm[x] = nl[p + np * x];
-                    ns_normal_jac<ND>(ph.param[0], ph.param[1], uhd, nrm, 0, A0);
-                    ns_normal_jac<ND>(ph.param[0], ph.param[1], uhd, nrm, 1, A1);
                     for (int j = 0; j < NCU; j++) {
-                        dual acc = mk(0.0);
-                        for (int q = 0; q < NCU; q++) {
-                            dual ap = A0[q * NCU + j] + A1[q * NCU + j];
-                            dual am = A0[q * NCU + j] - A1[q * NCU + j];
-                            acc = acc + ap * (mk(ug[q]) - uhd[q]);
-                            acc = acc - am * (mk(ph.ui[q]) - uhd[q]);
-                        }
-                        fbuh[p + np * (j + NCU * k)] = acc.d;
+                        fbuh[pp + np * (j + NCU * k)] = acc[j].d;
                         if (k == 0) {
-                            fb[p + np * j] = acc.v;
-                            for (int q = 0; q < NC; q++)
-                                fbu[p + np * (j + NCU * q)] = q < NCU ? A0[q * NCU + j].v + A1[q * NCU + j].v : 0.0;
+                            fb[pp + np * j] = acc[j].v;
+                            for (int q = NCU; q < NC; q++) fbu[pp + np * (j + NCU * q)] = 0.0;
                         }
                     }
                 }
@@ -315,122 +359,123 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
         }
         __syncthreads();
 
-        // ---------------- P8: q-folded integrands ----------------
+        // ---------------- P8: element rows of D and F ----------------
+        // thread column: D column (kc, cc) for row block j  -> cc = element node
+        //                F column (cc, kc) for row block j  -> cc = trace node
         {
-            const long nWd = (long)ngv * nd1 * ncu * ncu * npv, nWf = (long)ngv * nd1 * ncu * ncu * ndf;
-            const long nZd = (long)np * nch * ncu * npv, nZf = (long)np * nch * nch * ndf;
-            const long tot = nWd + nWf + 2 * nZd + 2 * nZf;
-            for (long i = tid; i < tot; i += nth) {
-                if (i < nWd) {
-                    int g = i % ngv; long r = i / ngv;
-                    int mq = r % nd1; r /= nd1; int j = r % ncu; r /= ncu; int kc = r % ncu; int nn = r / ncu;
-                    const double *wr = wrl + g + ngv * (mq + nd1 * j);
-                    const long cs = (long)ngv * nd1 * ncu;
-                    double v = wr[cs * kc] * SVT(g, nn, 0);
-                    for (int x = 0; x < ND; x++) v += wr[cs * (ncu * (x + 1) + kc)] * shapMiC[g + ngv * (nn + npv * x)];
-                    Wd[i] = v;
-                } else if (i < nWd + nWf) {
-                    long i2 = i - nWd;
-                    int g = i2 % ngv; long r = i2 / ngv;
-                    int mq = r % nd1; r /= nd1; int j = r % ncu; r /= ncu; int kc = r % ncu; int l = r / ncu;
-                    const double *wr = wrl + g + ngv * (mq + nd1 * j);
-                    const long cs = (long)ngv * nd1 * ncu;
-                    double v = 0;
-                    for (int x = 0; x < ND; x++) v += wr[cs * (ncu * (x + 1) + kc)] * shapMiE[g + ngv * (l + ndf * x)];
-                    Wf[i2] = v;
-                } else if (i < nWd + nWf + 2 * nZd) {
-                    long i2 = i - nWd - nWf;
-                    const int trace = i2 >= nZd;
-                    if (trace) i2 -= nZd;
-                    int p = i2 % np; long r = i2 / np;
-                    int j = r % nch; r /= nch; int kc = r % ncu; int nn = r / ncu;
-                    int a = p / ngf, g = p % ngf;
-                    if (trace && !bnd[a]) continue;
-                    const double *xu = trace ? fbu : fhu;
-                    int k = INVP(nn, a);
-                    double v = k >= 0 ? xu[p + np * (j + nch * kc)] * SFT(g, k, 0) : 0.0;
-                    for (int x = 0; x < ND; x++)
-                        v += xu[p + np * (j + nch * (ncu * (x + 1) + kc))] * shapMiCf[p + np * (nn + npv * x)];
-                    (trace ? ZdT : Zd)[i2] = v;
+            const int nDc = ncu * ncu * npv, nFc = ncu * nch * ndf;
+            const long cs = (long)ngv * nd1 * ncu;          // stride of wrl's column index
+            for (int col = tid; col < nDc + nFc; col += nth) {
+                const bool isD = col < nDc;
+                int cc, kc, j;
+                if (isD) { cc = col % npv; int r = col / npv; kc = r % ncu; j = r / ncu; }
+                else { int c2 = col - nDc; cc = c2 % ndf; int r = c2 / ndf; kc = r % nch; j = r / nch; }
+                double acc[NPV];
+#pragma unroll
+                for (int i = 0; i < NPV; i++) acc[i] = 0.0;
+                // volume: sum_{g,m'} shapvg(i,g,m') W(g,m')
+                const double *wr = wrl + ngv * nd1 * (long)j;
+                for (int mq = 0; mq < nd1; mq++)
+                    for (int g = 0; g < ngv; g++) {
+                        const double *wg = wr + g + ngv * mq;
+                        double w = 0.0;
+                        if (isD) w = wg[cs * kc] * SVT(g, cc, 0);
+#pragma unroll
+                        for (int x = 0; x < ND; x++)
+                            w += wg[cs * (ncu * (x + 1) + kc)] *
+                                 (isD ? shapMiC[cc + npv * (g + ngv * x)] : shapMiE[cc + ndf * (g + ngv * x)]);
+#pragma unroll
+                        for (int i = 0; i < NPV; i++) acc[i] += SVG(i, g, mq) * w;
+                    }
+                // faces: rows on face a get sum_g shapfg(kk,g) jacf z(p)
+                for (int a = 0; a < nfe; a++) {
+                    for (int g = 0; g < ngf; g++) {
+                        const int pp = g + ngf * a;
+                        double z;
+                        if (isD) {
+                            int k = INVP(cc, a);
+                            z = k >= 0 ? fhu[pp + np * (j + nch * kc)] * SFT(g, k, 0) : 0.0;
+#pragma unroll
+                            for (int x = 0; x < ND; x++)
+                                z += fhu[pp + np * (j + nch * (ncu * (x + 1) + kc))] * shapMiCf[cc + npv * (pp + np * x)];
+                        } else {
+                            z = (cc / npf == a) ? fhuh[pp + np * (j + nch * kc)] * SFT(g, cc % npf, 0) : 0.0;
+#pragma unroll
+                            for (int x = 0; x < ND; x++)
+                                z -= fhu[pp + np * (j + nch * (ncu * (x + 1) + kc))] * shapMiEf[cc + ndf * (pp + np * x)];
+                        }
+                        const double jz = jacf[pp] * z;
+#pragma unroll
+                        for (int i = 0; i < NPV; i++) {
+                            const int kk = INVP(i, a);
+                            if (kk >= 0) acc[i] += SFG(kk, g, 0) * jz;
+                        }
+                    }
+                }
+                if (isD) {
+                    double *Dcol = Dm + e * (long)na * na + (long)na * (kc * npv + cc) + j * npv;
+#pragma unroll
+                    for (int i = 0; i < NPV; i++) Dcol[i] = acc[i];
                 } else {
-                    long i2 = i - nWd - nWf - 2 * nZd;
-                    const int trace = i2 >= nZf;
-                    if (trace) i2 -= nZf;
-                    int p = i2 % np; long r = i2 / np;
-                    int j = r % nch; r /= nch; int kc = r % nch; int l = r / nch;
-                    int a = p / ngf, g = p % ngf;
-                    if (trace && !bnd[a]) continue;
-                    const double *xu = trace ? fbu : fhu, *xuh = trace ? fbuh : fhuh;
-                    double v = (l / npf == a) ? xuh[p + np * (j + nch * kc)] * SFT(g, l % npf, 0) : 0.0;
-                    for (int x = 0; x < ND; x++)
-                        v -= xu[p + np * (j + nch * (ncu * (x + 1) + kc))] * shapMiEf[p + np * (l + ndf * x)];
-                    (trace ? ZfT : Zf)[i2] = v;
+                    double *Fcol = Fx + e * (long)na * ldx + (long)(j * npv) * ldx + cc * nch + kc;
+#pragma unroll
+                    for (int i = 0; i < NPV; i++) Fcol[(long)i * ldx] = acc[i];
                 }
             }
         }
-        __syncthreads();
 
-        // ---------------- P9: contractions -> D, F, K, H, Ru, Rh ----------------
+        // ---------------- P9: trace rows of K and H; Ru, Rh ----------------
         {
-            const long nD = (long)na * na, nF = (long)na * nb, nK = (long)nb * na, nH = (long)nb * nb;
-            const long tot = nD + nF + nK + nH + na + nb;
+            const int nKc = nfe * nch * ncu * npv, nHc = nfe * nch * nch * ndf;
+            const long nH = (long)nb * nb;
             double ru2loc = 0.0;
-            for (long i = tid; i < tot; i += nth) {
-                if (i < nD + nF) {
-                    const bool isD = i < nD;
-                    long i2 = isD ? i : i - nD;
-                    // D: column-major (row fastest); F: row-major in Fx (column fastest)
-                    int row = isD ? (int)(i2 % na) : (int)(i2 / nb);
-                    long col = isD ? i2 / na : i2 % nb;
-                    int j = row / npv, ii = row % npv;
-                    int kc, cc;   // column component and column node/trace index
-                    if (isD) { kc = col / npv; cc = col % npv; }
-                    else { kc = col % nch; cc = col / nch; }
-                    const double *W = isD ? Wd + (long)ngv * nd1 * (j + ncu * (kc + ncu * (long)cc))
-                                          : Wf + (long)ngv * nd1 * (j + ncu * (kc + ncu * (long)cc));
-                    double s = 0;
-                    for (int mq = 0; mq < nd1; mq++)
-                        for (int g = 0; g < ngv; g++) s += SVG(ii, g, mq) * W[g + ngv * mq];
-                    for (int a = 0; a < nfe; a++) {
-                        int kk = INVP(ii, a);
-                        if (kk < 0) continue;
-                        const double *Z = isD ? Zd + np * (j + nch * (kc + ncu * (long)cc))
-                                              : Zf + np * (j + nch * (kc + nch * (long)cc));
-                        for (int g = 0; g < ngf; g++) {
-                            int p = g + ngf * a;
-                            s += SFG(kk, g, 0) * jacf[p] * Z[p];
-                        }
-                    }
-                    if (isD) Dm[e * nD + i2] = s;
-                    else Fx[e * (long)na * ldx + (long)row * ldx + col] = s;
-                } else if (i < nD + nF + nK + nH) {
-                    const bool isK = i < nD + nF + nK;
-                    long i2 = isK ? i - nD - nF : i - nD - nF - nK;
-                    int row = i2 % nb; long col = i2 / nb;
-                    int j = row % nch, tr = row / nch, a = tr / npf, kk = tr % npf;
-                    int kc, cc;
-                    if (isK) { kc = col / npv; cc = col % npv; }
-                    else { kc = col % nch; cc = col / nch; }
-                    const double *Z = bnd[a] ? (isK ? ZdT : ZfT) : (isK ? Zd : Zf);
-                    Z += isK ? np * (j + nch * (kc + ncu * (long)cc)) : np * (j + nch * (kc + nch * (long)cc));
-                    double s = 0;
+            for (int col = tid; col < nKc + nHc + na + nb; col += nth) {
+                if (col < nKc + nHc) {
+                    const bool isK = col < nKc;
+                    int a, j, kc, cc;
+                    if (isK) { cc = col % npv; int r = col / npv; kc = r % ncu; r /= ncu; j = r % nch; a = r / nch; }
+                    else { int c2 = col - nKc; cc = c2 % ndf; int r = c2 / ndf; kc = r % nch; r /= nch; j = r % nch; a = r / nch; }
+                    const bool bd = bnd[a];
+                    const double *xu = bd ? fbu : fhu, *xuh = bd ? fbuh : fhuh;
+                    double acc[NPF];
+#pragma unroll
+                    for (int k = 0; k < NPF; k++) acc[k] = 0.0;
                     for (int g = 0; g < ngf; g++) {
-                        int p = g + ngf * a;
-                        s += SFG(kk, g, 0) * jacf[p] * Z[p];
+                        const int pp = g + ngf * a;
+                        double z;
+                        if (isK) {
+                            int k = INVP(cc, a);
+                            z = k >= 0 ? xu[pp + np * (j + nch * kc)] * SFT(g, k, 0) : 0.0;
+#pragma unroll
+                            for (int x = 0; x < ND; x++)
+                                z += xu[pp + np * (j + nch * (ncu * (x + 1) + kc))] * shapMiCf[cc + npv * (pp + np * x)];
+                        } else {
+                            z = (cc / npf == a) ? xuh[pp + np * (j + nch * kc)] * SFT(g, cc % npf, 0) : 0.0;
+#pragma unroll
+                            for (int x = 0; x < ND; x++)
+                                z -= xu[pp + np * (j + nch * (ncu * (x + 1) + kc))] * shapMiEf[cc + ndf * (pp + np * x)];
+                        }
+                        const double jz = jacf[pp] * z;
+#pragma unroll
+                        for (int k = 0; k < NPF; k++) acc[k] += SFG(k, g, 0) * jz;
                     }
-                    if (isK) Km[e * nK + i2] = s;
-                    else Hx[e * (nH + nb) + i2] = s;
-                } else if (i < nD + nF + nK + nH + na) {
-                    int row = i - (nD + nF + nK + nH), j = row / npv, ii = row % npv;
+                    if (isK) {
+                        double *Kc = Km + e * (long)nb * na + (long)nb * (kc * npv + cc) + (a * npf) * nch + j;
+#pragma unroll
+                        for (int k = 0; k < NPF; k++) Kc[k * nch] = acc[k];
+                    } else {
+                        double *Hc = Hx + e * (nH + nb) + (long)nb * (cc * nch + kc) + (a * npf) * nch + j;
+#pragma unroll
+                        for (int k = 0; k < NPF; k++) Hc[k * nch] = acc[k];
+                    }
+                } else if (col < nKc + nHc + na) {
+                    int row = col - nKc - nHc, j = row / npv, ii = row % npv;
                     double s = 0;
                     for (int g = 0; g < ngv; g++) {
                         if (ph.tdep) {
-                            double sv = 0, ug = 0;
-                            for (int q = 0; q < npv; q++) {
-                                sv += SVT(g, q, 0) * sh[j * npv + q];
-                                ug += SVT(g, q, 0) * udg[j * npv + q];
-                            }
-                            s += SVG(ii, g, 0) * (sv - ug * ph.fc_u) * jac[g];
+                            double sv = 0;
+                            for (int q = 0; q < npv; q++) sv += SVT(g, q, 0) * sh[j * npv + q];
+                            s += SVG(ii, g, 0) * (sv - udgg[g + ngv * j] * ph.fc_u) * jac[g];
                         }
                         for (int mq = 0; mq < ND; mq++) {
                             double tx = 0;
@@ -442,20 +487,20 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                         int kk = INVP(ii, a);
                         if (kk < 0) continue;
                         for (int g = 0; g < ngf; g++) {
-                            int p = g + ngf * a;
-                            s -= SFG(kk, g, 0) * fh[p + np * j] * jacf[p];
+                            int pp = g + ngf * a;
+                            s -= SFG(kk, g, 0) * fh[pp + np * j] * jacf[pp];
                         }
                     }
                     Ru[e * na + row] = s;
                     Fx[e * (long)na * ldx + (long)row * ldx + nb] = s;
                     ru2loc += s * s;
                 } else {
-                    int row = i - (nD + nF + nK + nH + na), j = row % nch, tr = row / nch, a = tr / npf, kk = tr % npf;
+                    int row = col - nKc - nHc - na, j = row % nch, tr = row / nch, a = tr / npf, kk = tr % npf;
                     const double *fx = bnd[a] ? fb : fh;
                     double s = 0;
                     for (int g = 0; g < ngf; g++) {
-                        int p = g + ngf * a;
-                        s += SFG(kk, g, 0) * fx[p + np * j] * jacf[p];
+                        int pp = g + ngf * a;
+                        s += SFG(kk, g, 0) * fx[pp + np * j] * jacf[pp];
                     }
                     Rh[e * nb + row] = -s;
                     Hx[e * (nH + nb) + nH + row] = -s;
@@ -472,13 +517,36 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
         }
         __syncthreads();
     }
+#undef SVT
+#undef SVG
+#undef SFT
+#undef SFG
+#undef PERM
+#undef INVP
 }
 
 size_t assemble_lds_bytes(const Dims &d)
 {
-    size_t n = (size_t)d.npv * d.nd + (size_t)d.npv * d.nc + (size_t)d.ndf * d.nch + (size_t)d.npv * d.ncu + d.ngv +
-               (size_t)d.ngv * d.nd * d.nd + d.np + (size_t)d.np * d.nd + (size_t)d.npv * d.npv + 256;
-    return n * sizeof(double) + 64 * sizeof(int);
+    size_t n = 2 * (size_t)d.npv * d.ngv * d.nd1 + 2 * (size_t)d.npf * d.ngf * d.nd + (size_t)d.npv * d.nd +
+               (size_t)d.npv * d.nc + (size_t)d.ndf * d.nch + (size_t)d.npv * d.ncu + d.ngv +
+               (size_t)d.ngv * d.nd * d.nd + d.np + (size_t)d.np * d.nd + (size_t)d.npv * d.npv +
+               (size_t)d.npv * d.ngv * d.nd + (size_t)d.ndf * d.ngv * d.nd + (size_t)d.ngv * d.nc +
+               (size_t)d.np * d.nc + (size_t)d.np * d.nch + 256;
+    size_t ints = (size_t)d.npf * d.nfe + (size_t)d.npv * d.nfe + d.nfe + 8;
+    return n * sizeof(double) + ints * sizeof(int);
+}
+
+// element families with compiled kernels: (nd, npv, npf)
+#define HFEM_FAMILIES(X)                                                            \
+    X(2, 3, 2) X(2, 6, 3) X(2, 10, 4) X(2, 15, 5) X(2, 21, 6)                       \
+    X(3, 4, 3) X(3, 10, 6) X(3, 20, 10) X(3, 35, 15) X(3, 8, 4) X(3, 27, 9)
+
+bool assemble_supported(int nd, int npv, int npf)
+{
+#define CHECK(a, b, c) if (nd == a && npv == b && npf == c) return true;
+    HFEM_FAMILIES(CHECK)
+#undef CHECK
+    return false;
 }
 
 hipError_t launch_assemble(const Dims &d, const Master &m, const Phys &ph, const double *dgnodes, const double *UDG,
@@ -487,13 +555,17 @@ hipError_t launch_assemble(const Dims &d, const Master &m, const Phys &ph, const
                            double *K, double *Hx, double *Rh, double *ru2, hipStream_t stream)
 {
     size_t lds = assemble_lds_bytes(d);
-    if (d.nd == 2)
-        hipLaunchKernelGGL(assemble_kernel<2>, dim3(grid), dim3(256), lds, stream, d, m, ph, dgnodes, UDG, UH, SH,
-                           elcon, bf, e0, n, scratch, sstride, ldx, Ru, D, Fx, K, Hx, Rh, ru2);
-    else
-        hipLaunchKernelGGL(assemble_kernel<3>, dim3(grid), dim3(256), lds, stream, d, m, ph, dgnodes, UDG, UH, SH,
-                           elcon, bf, e0, n, scratch, sstride, ldx, Ru, D, Fx, K, Hx, Rh, ru2);
-    return hipGetLastError();
+#define LAUNCH(a, b, c)                                                                                            \
+    if (d.nd == a && d.npv == b && d.npf == c) {                                                                   \
+        hipFuncSetAttribute((const void *)assemble_kernel<a, b, c>, hipFuncAttributeMaxDynamicSharedMemorySize,   \
+                            (int)lds);                                                                             \
+        hipLaunchKernelGGL((assemble_kernel<a, b, c>), dim3(grid), dim3(256), lds, stream, d, m, ph, dgnodes, UDG, \
+                           UH, SH, elcon, bf, e0, n, scratch, sstride, ldx, Ru, D, Fx, K, Hx, Rh, ru2);            \
+        return hipGetLastError();                                                                                  \
+    }
+    HFEM_FAMILIES(LAUNCH)
+#undef LAUNCH
+    return hipErrorInvalidValue;
 }
 
 }  // namespace hfem

@@ -146,10 +146,9 @@ __device__ __forceinline__ void ns_normal_jac(double gam, double eps, const dual
     }
 }
 
-// Per-element scratch offsets (doubles) used by the assembly kernel.
+// Per-workgroup scratch offsets (doubles) used by the assembly kernel.
 struct Scratch {
-    long C, E, shapMiC, shapMiE, shapMiCf, shapMiEf, f, fu, fh, fhu, fhuh, fb, fbu, fbuh, wrl, Wd, Wf,
-        Zd, ZdT, Zf, ZfT, total;
+    long C, E, shapMiCf, shapMiEf, f, fu, fh, fhu, fhuh, fb, fbu, fbuh, wrl, total;
 };
 
 __host__ __device__ inline Scratch scratch_layout(const Dims &d)
@@ -159,8 +158,6 @@ __host__ __device__ inline Scratch scratch_layout(const Dims &d)
     auto take = [&](long n) { long r = o; o += (n + 31) / 32 * 32; return r; };
     s.C = take((long)d.nd * d.npv * d.npv);
     s.E = take((long)d.nd * d.npv * d.ndf);
-    s.shapMiC = take((long)d.ngv * d.npv * d.nd);
-    s.shapMiE = take((long)d.ngv * d.ndf * d.nd);
     s.shapMiCf = take((long)d.np * d.npv * d.nd);
     s.shapMiEf = take((long)d.np * d.ndf * d.nd);
     s.f = take((long)d.ngv * d.ncu * d.nd);
@@ -172,12 +169,6 @@ __host__ __device__ inline Scratch scratch_layout(const Dims &d)
     s.fbu = take((long)d.np * d.nch * d.nc);
     s.fbuh = take((long)d.np * d.nch * d.nch);
     s.wrl = take((long)d.ngv * d.nd1 * d.ncu * d.nc);
-    s.Wd = take((long)d.ngv * d.nd1 * d.ncu * d.ncu * d.npv);
-    s.Wf = take((long)d.ngv * d.nd1 * d.ncu * d.ncu * d.ndf);
-    s.Zd = take((long)d.np * d.nch * d.ncu * d.npv);
-    s.ZdT = take((long)d.np * d.nch * d.ncu * d.npv);
-    s.Zf = take((long)d.np * d.nch * d.nch * d.ndf);
-    s.ZfT = take((long)d.np * d.nch * d.nch * d.ndf);
     s.total = o;
     return s;
 }
