@@ -27,11 +27,13 @@ from .master import make_master
 __all__ = ["Problem", "make_problem", "CONFIGS", "freestream_state"]
 
 
-def freestream_state(nd, gamma, minf):
+def freestream_state(nd, gamma, minf, angle=0.0):
+    """rho = 1, |v| = 1 at `angle` (radians) from x in the x-y plane, p = 1/(gamma Minf^2)."""
     pinf = 1.0 / (gamma * minf ** 2)
     u = np.zeros(nd + 2)
     u[0] = 1.0
-    u[1] = 1.0
+    u[1] = np.cos(angle)
+    u[2] += np.sin(angle)
     u[nd + 1] = 0.5 + pinf / (gamma - 1.0)
     return u
 
@@ -91,8 +93,12 @@ def _uhat_average(mesh, UDG, ncu):
 
 
 def make_problem(kind, ncell, porder, seed=0, tau=2.0, re=1000.0, pr=0.72, minf=0.3, gamma=1.4,
-                 amp=1e-2, qamp=1e-2, jitter=0.05, name=None):
-    """Build a seeded case.  kind: 'tri' (2D), 'tet' (3D), 'hex' (3D periodic)."""
+                 amp=1e-2, qamp=1e-2, jitter=0.05, angle=0.0, name=None):
+    """Build a seeded case.  kind: 'tri' (2D), 'tet' (3D), 'hex' (3D periodic).
+
+    ``angle`` turns the freestream in the x-y plane; at angle 0 the flow is
+    tangential to the y-faces of the box, where the normal velocity crosses zero
+    and |A_n| of the far-field flux is not differentiable."""
     if kind == "tri":
         msh = meshmod.square_tri_mesh(ncell, porder, seed=seed, jitter=jitter)
     elif kind == "tet":
@@ -106,7 +112,7 @@ def make_problem(kind, ncell, porder, seed=0, tau=2.0, re=1000.0, pr=0.72, minf=
     nc = ncu * (nd + 1)
     m = make_master(nd, msh.elemtype, porder)
     rng = np.random.default_rng(seed + 7919)
-    u0 = freestream_state(nd, gamma, minf)
+    u0 = freestream_state(nd, gamma, minf, angle)
     x = msh.dgnodes                                   # (ne, npv, nd)
     ne = msh.ne
     UDG = np.zeros((ne, nc, m.npv))

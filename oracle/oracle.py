@@ -46,6 +46,14 @@ def lib():
         L.hfo_fbou.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
         L.hfo_scatter.argtypes = [ctypes.c_int] * 4 + [_ip, _ip, _ip, _ip, _dp, _dp, _dp, _dp]
         L.hfo_bsr_matvec.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _ip, _dp, _dp, _dp]
+        L.hfo_dense_inverse.argtypes = [ctypes.c_int, _dp]
+        L.hfo_mdf_order.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _ip, _dp, ctypes.c_int, _ip, _ip]
+        L.hfo_bilu0_factor.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _ip, _ip, _ip, _dp]
+        L.hfo_bilu0_apply.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _ip, _ip, _ip, _dp, _dp, _dp]
+        L.hfo_block_jacobi_factor.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _dp, _dp]
+        L.hfo_gmres.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _ip, _dp, ctypes.c_int, _dp, _ip, _ip, _dp, _dp,
+                                ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _ip, _dp]
+        L.hfo_update_u.argtypes = [ctypes.c_int] * 6 + [_ip, _dp, _dp, _dp, ctypes.c_double, _dp]
         _lib = L
     return _lib
 
@@ -230,3 +238,115 @@ class Reference(Oracle):
         p = self.prob
         return ref_lib().ref_time(*self._common(), p.ne, p.UH.shape[0], _d(self.dgnodes), _d(_c64(p.UDG)),
                                   _d(_c64(p.UH)), _d(_c64(p.SH)), _i(self.elcon), _i(self.bf), nelem, impl)
+
+
+# --------------------------------------------------------------------------------------------
+# trace-system solver (restated from the reference's MATLAB specifications; see solver_oracle.c)
+
+class BSR:
+    """A BSR matrix in the reference layout: rows = faces, the diagonal block first."""
+
+    def __init__(self, rowpts, colind, blocks, bsz):
+        self.rowpts = _c32(rowpts)
+        self.colind = _c32(colind)
+        self.bsz = int(bsz)
+        self.nrows = len(self.rowpts) - 1
+        self.blocks = _c64(blocks).reshape(-1)
+
+    @classmethod
+    def from_problem(cls, prob, Hg):
+        msh = prob.mesh
+        return cls(msh.ent2entStart, msh.ent2ent, Hg, prob.nch * prob.master.npf)
+
+    def dense(self):
+        n, b = self.nrows, self.bsz
+        A = np.zeros((n * b, n * b))
+        blk = self.blocks.reshape(-1, b, b).transpose(0, 2, 1)
+        for i in range(n):
+            for k in range(self.rowpts[i], self.rowpts[i + 1]):
+                j = self.colind[k]
+                A[i * b:(i + 1) * b, j * b:(j + 1) * b] = blk[k]
+        return A
+
+    def matvec(self, x):
+        y = np.zeros(self.nrows * self.bsz)
+        lib().hfo_bsr_matvec(self.nrows, self.bsz, _i(self.rowpts), _i(self.colind), _d(self.blocks), _d(_c64(x)),
+                             _d(y))
+        return y
+
+
+def dense_inverse(A):
+    A = np.asfortranarray(A, dtype=np.float64).copy(order="F")
+    info = lib().hfo_dense_inverse(A.shape[0], A.ctypes.data_as(_dp))
+    return A, info
+
+
+def mdf_order(M, exact=False):
+    o2u = np.zeros(M.nrows, dtype=np.int32)
+    u2o = np.zeros(M.nrows, dtype=np.int32)
+    lib().hfo_mdf_order(M.nrows, M.bsz, _i(M.rowpts), _i(M.colind), _d(M.blocks), int(exact), _i(o2u), _i(u2o))
+    return o2u, u2o
+
+
+def bilu0_factor(M, o2u, u2o):
+    Mg = M.blocks.copy()
+    info = lib().hfo_bilu0_factor(M.nrows, M.bsz, _i(M.rowpts), _i(M.colind), _i(_c32(o2u)), _i(_c32(u2o)), _d(Mg))
+    return Mg, info
+
+
+def bilu0_apply(M, o2u, u2o, Mg, x):
+    v = np.zeros(M.nrows * M.bsz)
+    lib().hfo_bilu0_apply(M.nrows, M.bsz, _i(M.rowpts), _i(M.colind), _i(_c32(o2u)), _i(_c32(u2o)), _d(_c64(Mg)),
+                          _d(_c64(x)), _d(v))
+    return v
+
+
+def block_jacobi_factor(M):
+    Dinv = np.zeros(M.nrows * M.bsz * M.bsz)
+    info = lib().hfo_block_jacobi_factor(M.nrows, M.bsz, _i(M.rowpts), _d(M.blocks), _d(Dinv))
+    return Dinv, info
+
+
+PREC_NONE, PREC_BILU0, PREC_BJ = -1, 0, 2
+
+
+def gmres(M, b, prec=PREC_BILU0, Mg=None, o2u=None, u2o=None, x0=None, restart=30, tol=1e-10, maxit=1000,
+          orthog=1):
+    """Left-preconditioned restarted GMRES (kernel/gmres_bilu0.m). Returns x, flag, iters, rev."""
+    n = M.nrows
+    x = np.zeros(n * M.bsz) if x0 is None else _c64(x0).copy()
+    o2u = np.arange(n, dtype=np.int32) if o2u is None else _c32(o2u)
+    u2o = np.arange(n, dtype=np.int32) if u2o is None else _c32(u2o)
+    Mg = np.zeros(1) if Mg is None else _c64(Mg)
+    iters = ctypes.c_int()
+    rev = np.zeros(maxit)
+    flag = lib().hfo_gmres(n, M.bsz, _i(M.rowpts), _i(M.colind), _d(M.blocks), prec, _d(Mg), _i(o2u), _i(u2o),
+                           _d(_c64(b)), _d(x), restart, tol, maxit, orthog, ctypes.byref(iters), _d(rev))
+    return x, flag, iters.value, rev[:iters.value]
+
+
+def update_u(prob, DinvF, DinvRu, duh, alpha, UDG):
+    """getU (solvers/solutionupdate.cpp:6-67) on all elements; returns the new UDG (u part updated)."""
+    d = prob.dims
+    UDG = _c64(UDG).copy()
+    lib().hfo_update_u(prob.ne, d["npv"], d["nc"], d["ncu"], d["nch"], d["ndf"], _i(_c32(prob.mesh.elcon)),
+                       _d(_c64(DinvF)), _d(_c64(DinvRu)), _d(_c64(duh)), alpha, _d(UDG))
+    return UDG
+
+
+def trace_system(prob, UDG=None, UH=None, SH=None):
+    """assembleLinearSystem on the CPU: element blocks, condensation (LU with partial
+    pivoting), scatter.  Returns (BSR Hg, Rg, rNorm, condensed blocks)."""
+    orc = Oracle(prob)
+    blocks = orc.assemble(0, prob.ne, UDG=UDG, UH=UH, SH=SH)
+    Ru0, Rh0 = blocks["Ru"].copy(), blocks["Rh"].copy()
+    msh = prob.mesh
+    d = prob.dims
+    bsz = prob.nch * prob.master.npf
+    # rNorm = sqrt(|Ru|^2 + |r|^2), r the assembled uncondensed Rh (assembly/globalassembly.cpp:538-541)
+    zeros = np.zeros((prob.ne, d["nb"] * d["nb"]))
+    _, r = scatter(prob, zeros, Rh0)
+    rnorm = float(np.sqrt((Ru0 ** 2).sum() + (r ** 2).sum()))
+    schur(0, d["npv"], d["ncu"], blocks)
+    Hg, Rg = scatter(prob, blocks["H"], blocks["Rh"])
+    return BSR(msh.ent2entStart, msh.ent2ent, Hg, bsz), Rg, rnorm, blocks
