@@ -61,6 +61,25 @@ def run_steps(ctx, ne, precision, steps, warmup):
     return wall, float(np.mean(asm_ms)), float(np.mean(cond_ms))
 
 
+def trace_solver(ctx, restart=30, tol=1e-10, maxit=300):
+    """One linear solve of the trace system (computeOrdering, computePreconditioner,
+    gmres of solvers/dgsolvers.cpp:79-105) after a full assembleLinearSystem."""
+    from hfem_amd.lib import FP64, PREC_BILU0
+    t0 = time.perf_counter()
+    rnorm = ctx.assemble_linear_system(FP64)
+    t1 = time.perf_counter()
+    ctx.compute_ordering(1)
+    ctx.compute_preconditioner(PREC_BILU0)
+    flag, it, rev = ctx.gmres(restart=restart, tol=tol, maxit=maxit)
+    st = ctx.solver_stats()
+    _, nlev = ctx.get_ordering()
+    return dict(assemble_linear_system_ms=(t1 - t0) * 1e3, rNorm=rnorm, mdf_levels=nlev,
+                ordering_ms=st["order_ms"], bilu0_factor_ms=st["factor_ms"], gmres_ms=st["gmres_ms"],
+                gmres_iterations=it, gmres_flag=flag, final_relative_residual=float(rev[-1]) if len(rev) else None,
+                matvec_ms=st["matvec_ms"] / max(st["matvecs"], 1),
+                bilu0_apply_ms=st["precond_ms"] / max(st["preconds"], 1))
+
+
 def cpu_baseline(prob, seconds=10.0):
     """The reference's own element path (oracle/_ref, MKL, 1 thread) on a bounded sample."""
     os.environ.setdefault("MKL_NUM_THREADS", "1")
@@ -96,6 +115,7 @@ def main():
     ap.add_argument("--ncell", type=int, default=14)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
+    ap.add_argument("--no-solver", action="store_true", help="skip the trace-system solve")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -173,6 +193,10 @@ def main():
                      "flops_per_element": kflops, "elements_per_launch": ne,
                      "frac_of_fp64_peak": achieved / PEAK_FP64},
     }
+    if not args.no_solver:
+        sol = trace_solver(ctx)
+        print(f"[bench] trace solve: {sol}", file=sys.stderr, flush=True)
+        out["trace_solver"] = sol
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(prob)
     print(json.dumps(out))

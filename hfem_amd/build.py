@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libhfem_amd.so")
-SOURCES = ["api.hip", "assemble.hip", "condense.hip", "trace.hip"]
+SOURCES = ["api.hip", "assemble.hip", "condense.hip", "trace.hip", "solver.hip", "solve.hip"]
 
 HIPCC_FLAGS = [
     "-O3", "--offload-arch=gfx950", "-fPIC", "-shared", "-std=c++17",

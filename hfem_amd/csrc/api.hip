@@ -8,37 +8,13 @@
 #include <string>
 #include <vector>
 
-#include "../../include/hfem_amd.h"
-#include "hfem_device.h"
+#include "ctx.h"
 
-namespace hfem {
-hipError_t launch_assemble(const Dims &, const Master &, const Phys &, const double *, const double *, const double *,
-                           const double *, const int *, const int *, int, int, double *, long, int, int, double *,
-                           double *, double *, double *, double *, double *, double *, hipStream_t);
-hipError_t launch_condense_fp64(int na, int nb, int ldx, int e0, int n, const double *D, double *Dinv_g,
-                                const double *Fx, double *X, const double *K, double *Hx, int *info, int grid,
-                                bool dinv_lds, hipStream_t s);
-hipError_t launch_condense_mixed(int na, int nb, int ldx, int e0, int n, const double *D, const double *Fx, double *X,
-                                 double *Rw, const double *K, double *Hx, int *info, int grid, int sweeps,
-                                 hipStream_t s);
-size_t condense_lds_bytes(int na, bool dinv_in_lds, int elt);
-size_t assemble_lds_bytes(const Dims &d);
-bool assemble_supported(int nd, int npv, int npf);
-int condense_dinv_ld(int na);
-hipError_t launch_bsr_gather(int, int, int, int, int, long, const int *, const int *, const int *, const double *,
-                             double *, hipStream_t);
-hipError_t launch_face_gather(int, int, int, int, int, long, const int *, const int *, const double *, double *,
-                              const double *, double *, hipStream_t);
-hipError_t launch_sumsq(long, const double *, int, double *, int, hipStream_t);
-hipError_t launch_bsr_matvec(int, int, const int *, const int *, const double *, const double *, double *,
-                             hipStream_t);
-}  // namespace hfem
 
-using namespace hfem;
 
 static thread_local std::string g_err;
 
-static int fail(int code, const char *fmt, ...)
+int hfem_fail(int code, const char *fmt, ...)
 {
     char buf[512];
     va_list ap;
@@ -49,46 +25,6 @@ static int fail(int code, const char *fmt, ...)
     return code;
 }
 
-#define HIPCHK(x)                                                                                     \
-    do {                                                                                              \
-        hipError_t e_ = (x);                                                                          \
-        if (e_ != hipSuccess) return fail(-2, "%s failed: %s", #x, hipGetErrorString(e_));            \
-    } while (0)
-
-struct hfem_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    hfem_dims hd{};
-    Dims d{};
-    Phys ph{};
-    bool has_master = false, has_phys = false, has_mesh = false, has_sol = false;
-    // master
-    double *shapvt = nullptr, *shapvg = nullptr, *shapft = nullptr, *shapfg = nullptr;
-    int *perm = nullptr, *invperm = nullptr;
-    // mesh
-    double *dgnodes = nullptr;
-    int *elcon = nullptr, *bf = nullptr, *rowstart = nullptr, *cols = nullptr, *einv = nullptr, *f2e = nullptr;
-    int *blkptr = nullptr, *blklist = nullptr;
-    int nblocks = 0;
-    // state
-    double *UDG = nullptr, *UH = nullptr, *SH = nullptr;
-    // element blocks: D (na x na), Fx and X (na x ldx, row-major, column nb = Ru / inv(D)Ru),
-    // K (nb x na), Hx (nb x (nb+1), column nb = Rh / FH), Ru, Rh (uncondensed)
-    double *D = nullptr, *Fx = nullptr, *X = nullptr, *K = nullptr, *Hx = nullptr, *Ru = nullptr, *Rh = nullptr,
-           *ru2 = nullptr;
-    int *info = nullptr;
-    int ldx = 0;
-    // work
-    double *scratch = nullptr, *dinv = nullptr, *rwork = nullptr;
-    long sstride = 0;
-    int grid_asm = 0, grid_cond = 0;
-    bool dinv_lds = true;
-    int maxlds = 0;
-    // trace system
-    double *Hg = nullptr, *Rg = nullptr, *r = nullptr, *partial = nullptr, *xv = nullptr, *yv = nullptr;
-    float ms_asm = 0, ms_cond = 0;
-};
 
 extern "C" const char *hfem_last_error(void) { return g_err.c_str(); }
 
@@ -100,16 +36,16 @@ static hipError_t dalloc(T **p, size_t n)
 
 extern "C" int hfem_create(hfem_ctx **out, const hfem_dims *hd, int device)
 {
-    if (!out || !hd) return fail(-1, "hfem_create: null argument");
+    if (!out || !hd) return hfem_fail(-1, "hfem_create: null argument");
     *out = nullptr;
-    if (hd->nd != 2 && hd->nd != 3) return fail(-1, "hfem_create: nd must be 2 or 3 (got %d)", hd->nd);
-    if (hd->ncu != hd->nd + 2) return fail(-1, "hfem_create: compressible NS needs ncu = nd+2");
+    if (hd->nd != 2 && hd->nd != 3) return hfem_fail(-1, "hfem_create: nd must be 2 or 3 (got %d)", hd->nd);
+    if (hd->ncu != hd->nd + 2) return hfem_fail(-1, "hfem_create: compressible NS needs ncu = nd+2");
     if (hd->npv <= 0 || hd->ngv <= 0 || hd->npf <= 0 || hd->ngf <= 0 || hd->nfe <= 0 || hd->ne <= 0 || hd->nf <= 0)
-        return fail(-1, "hfem_create: sizes must be positive");
-    if (hd->npv > 64 || hd->npf * hd->nfe > 128) return fail(-1, "hfem_create: element too large (npv<=64)");
+        return hfem_fail(-1, "hfem_create: sizes must be positive");
+    if (hd->npv > 64 || hd->npf * hd->nfe > 128) return hfem_fail(-1, "hfem_create: element too large (npv<=64)");
     int ndev = 0;
     HIPCHK(hipGetDeviceCount(&ndev));
-    if (device < 0 || device >= ndev) return fail(-1, "hfem_create: device %d out of range (%d devices)", device, ndev);
+    if (device < 0 || device >= ndev) return hfem_fail(-1, "hfem_create: device %d out of range (%d devices)", device, ndev);
     HIPCHK(hipSetDevice(device));
     hfem_ctx *c = new hfem_ctx();
     c->device = device;
@@ -118,7 +54,7 @@ extern "C" int hfem_create(hfem_ctx **out, const hfem_dims *hd, int device)
     d.nd = hd->nd; d.ncu = hd->ncu; d.nch = hd->ncu; d.nd1 = hd->nd + 1; d.nc = d.ncu * d.nd1;
     d.npv = hd->npv; d.ngv = hd->ngv; d.npf = hd->npf; d.ngf = hd->ngf; d.nfe = hd->nfe;
     d.ndf = d.npf * d.nfe; d.na = d.npv * d.ncu; d.nb = d.ndf * d.nch; d.np = d.ngf * d.nfe;
-    if (d.na > 176) { delete c; return fail(-1, "hfem_create: na=%d exceeds 176 (npv*ncu)", d.na); }
+    if (d.na > 176) { delete c; return hfem_fail(-1, "hfem_create: na=%d exceeds 176 (npv*ncu)", d.na); }
     const long ne = hd->ne;
     hipError_t e = hipSuccess;
 #define ALLOC(p, n) if (e == hipSuccess) e = dalloc(&c->p, (size_t)(n))
@@ -137,6 +73,9 @@ extern "C" int hfem_create(hfem_ctx **out, const hfem_dims *hd, int device)
     c->grid_asm = (int)std::min<long>(ne, 512);
     c->grid_cond = (int)std::min<long>(ne, 1024);
     ALLOC(scratch, (long)c->grid_asm * c->sstride);
+    c->fstride = flux_layout(d).total;
+    c->chunk = (int)std::min<long>(ne, 4096);
+    ALLOC(fluxbuf, (long)c->chunk * c->fstride);
     {
         int maxlds = 65536;
         if (hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
@@ -145,16 +84,16 @@ extern "C" int hfem_create(hfem_ctx **out, const hfem_dims *hd, int device)
         c->dinv_lds = condense_lds_bytes(d.na, true, sizeof(double)) <= (size_t)maxlds;
         if (assemble_lds_bytes(d) > (size_t)maxlds) {
             delete c;
-            return fail(-1, "hfem_create: element family needs %zu bytes of LDS, device offers %d",
+            return hfem_fail(-1, "hfem_create: element family needs %zu bytes of LDS, device offers %d",
                         assemble_lds_bytes(d), maxlds);
         }
         if (!assemble_supported(d.nd, d.npv, d.npf)) {
             delete c;
-            return fail(-1, "hfem_create: no assembly kernel compiled for nd=%d npv=%d npf=%d", d.nd, d.npv, d.npf);
+            return hfem_fail(-1, "hfem_create: no assembly kernel compiled for nd=%d npv=%d npf=%d", d.nd, d.npv, d.npf);
         }
         if (condense_lds_bytes(d.na, true, sizeof(float)) > (size_t)maxlds) {
             delete c;
-            return fail(-1, "hfem_create: na=%d needs more LDS than the device offers (%d bytes)", d.na, maxlds);
+            return hfem_fail(-1, "hfem_create: na=%d needs more LDS than the device offers (%d bytes)", d.na, maxlds);
         }
     }
     if (!c->dinv_lds) ALLOC(dinv, (long)c->grid_cond * d.na * condense_dinv_ld(d.na));
@@ -164,7 +103,7 @@ extern "C" int hfem_create(hfem_ctx **out, const hfem_dims *hd, int device)
     for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
     if (e != hipSuccess) {
         hfem_destroy(c);
-        return fail(-2, "hfem_create: allocation failed: %s", hipGetErrorString(e));
+        return hfem_fail(-2, "hfem_create: allocation failed: %s", hipGetErrorString(e));
     }
     *out = c;
     return 0;
@@ -177,11 +116,15 @@ extern "C" int hfem_destroy(hfem_ctx *c)
     if (c->stream) hipStreamSynchronize(c->stream);
     void *ptrs[] = {c->shapvt, c->shapvg, c->shapft, c->shapfg, c->perm, c->invperm, c->dgnodes, c->elcon, c->bf,
                     c->rowstart, c->cols, c->einv, c->f2e, c->blkptr, c->blklist, c->UDG, c->UH, c->SH, c->D, c->Fx,
-                    c->X, c->K, c->Hx, c->Ru, c->Rh, c->ru2, c->info, c->scratch, c->dinv, c->rwork, c->Hg, c->Rg,
-                    c->r, c->partial, c->xv, c->yv};
+                    c->X, c->K, c->Hx, c->Ru, c->Rh, c->ru2, c->info, c->scratch, c->dinv, c->rwork, c->fluxbuf, c->Hg, c->Rg,
+                    c->r, c->partial, c->xv, c->yv, c->sv.d_o2u, c->sv.d_u2o, c->sv.d_lrows, c->sv.d_urows,
+                    c->sv.Mg, c->sv.Dinv, c->sv.Cw, c->sv.d_info, c->sv.x, c->sv.V, c->sv.b, c->sv.t, c->sv.dots,
+                    c->sv.coef, c->sv.UDGi};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (auto &ev : c->ev)
+        if (ev) hipEventDestroy(ev);
+    for (auto &ev : c->sv.ev)
         if (ev) hipEventDestroy(ev);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -200,14 +143,14 @@ static int upload(hfem_ctx *c, T **dst, const T *src, size_t n)
 extern "C" int hfem_set_master(hfem_ctx *c, const double *shapvt, const double *shapvg, const double *shapft,
                                const double *shapfg, const int *perm)
 {
-    if (!c || !shapvt || !shapvg || !shapft || !shapfg || !perm) return fail(-1, "hfem_set_master: null argument");
+    if (!c || !shapvt || !shapvg || !shapft || !shapfg || !perm) return hfem_fail(-1, "hfem_set_master: null argument");
     HIPCHK(hipSetDevice(c->device));
     const Dims &d = c->d;
     std::vector<int> inv((size_t)d.npv * d.nfe, -1);
     for (int a = 0; a < d.nfe; a++)
         for (int k = 0; k < d.npf; k++) {
             int nn = perm[k + d.npf * a];
-            if (nn < 0 || nn >= d.npv) return fail(-1, "hfem_set_master: perm(%d,%d)=%d out of range", k, a, nn);
+            if (nn < 0 || nn >= d.npv) return hfem_fail(-1, "hfem_set_master: perm(%d,%d)=%d out of range", k, a, nn);
             inv[nn + (size_t)d.npv * a] = k;
         }
     int rc;
@@ -223,8 +166,8 @@ extern "C" int hfem_set_master(hfem_ctx *c, const double *shapvt, const double *
 
 extern "C" int hfem_set_physics(hfem_ctx *c, const double *param, const double *ui, double fc_u, double fc_q, int tdep)
 {
-    if (!c || !param || !ui) return fail(-1, "hfem_set_physics: null argument");
-    if (param[0] <= 1.0 || param[2] <= 0 || param[3] <= 0) return fail(-1, "hfem_set_physics: invalid gamma/Re/Pr");
+    if (!c || !param || !ui) return hfem_fail(-1, "hfem_set_physics: null argument");
+    if (param[0] <= 1.0 || param[2] <= 0 || param[3] <= 0) return hfem_fail(-1, "hfem_set_physics: invalid gamma/Re/Pr");
     for (int i = 0; i < 6; i++) c->ph.param[i] = param[i];
     for (int i = 0; i < c->d.nch; i++) c->ph.ui[i] = ui[i];
     c->ph.fc_u = fc_u;
@@ -237,11 +180,11 @@ extern "C" int hfem_set_physics(hfem_ctx *c, const double *param, const double *
 extern "C" int hfem_set_mesh(hfem_ctx *c, const double *dgnodes, const int *elcon, const int *bf, const int *t2f,
                              const int *rowstart, const int *cols, int nblocks)
 {
-    if (!c || !dgnodes || !elcon || !bf || !t2f || !rowstart || !cols) return fail(-1, "hfem_set_mesh: null argument");
+    if (!c || !dgnodes || !elcon || !bf || !t2f || !rowstart || !cols) return hfem_fail(-1, "hfem_set_mesh: null argument");
     HIPCHK(hipSetDevice(c->device));
     const Dims &d = c->d;
     const int ne = c->hd.ne, nf = c->hd.nf;
-    if (rowstart[0] != 0 || rowstart[nf] != nblocks) return fail(-1, "hfem_set_mesh: rowstart inconsistent with nblocks");
+    if (rowstart[0] != 0 || rowstart[nf] != nblocks) return hfem_fail(-1, "hfem_set_mesh: rowstart inconsistent with nblocks");
     // einv, face-to-element lists and the gather lists of every BSR block
     std::vector<int> einv((size_t)ne * d.ndf, -1), f2e((size_t)nf * 4, -1);
     std::vector<int> cnt(nblocks + 1, 0);
@@ -249,27 +192,27 @@ extern "C" int hfem_set_mesh(hfem_ctx *c, const double *dgnodes, const int *elco
     for (int e = 0; e < ne; e++)
         for (int a = 0; a < d.nfe; a++) {
             const int i = t2f[e * d.nfe + a];
-            if (i < 0 || i >= nf) return fail(-1, "hfem_set_mesh: t2f[%d,%d]=%d out of range", e, a, i);
+            if (i < 0 || i >= nf) return hfem_fail(-1, "hfem_set_mesh: t2f[%d,%d]=%d out of range", e, a, i);
             for (int m = 0; m < d.npf; m++) {
                 int g = elcon[(long)e * d.ndf + a * d.npf + m] - i * d.npf;
-                if (g < 0 || g >= d.npf) return fail(-1, "hfem_set_mesh: elcon of element %d face %d not on face %d", e, a, i);
+                if (g < 0 || g >= d.npf) return hfem_fail(-1, "hfem_set_mesh: elcon of element %d face %d not on face %d", e, a, i);
                 einv[(long)e * d.ndf + a * d.npf + g] = m;
             }
             int side = f2e[4 * i] < 0 ? 0 : 1;
-            if (side == 1 && f2e[4 * i + 2] >= 0) return fail(-1, "hfem_set_mesh: face %d has more than two elements", i);
+            if (side == 1 && f2e[4 * i + 2] >= 0) return hfem_fail(-1, "hfem_set_mesh: face %d has more than two elements", i);
             f2e[4 * i + 2 * side] = e;
             f2e[4 * i + 2 * side + 1] = a;
             for (int b = 0; b < d.nfe; b++) {
                 const int j = t2f[e * d.nfe + b];
                 int k = rowstart[i];
                 while (k < rowstart[i + 1] && cols[k] != j) k++;
-                if (k == rowstart[i + 1]) return fail(-1, "hfem_set_mesh: block (%d,%d) missing from the trace graph", i, j);
+                if (k == rowstart[i + 1]) return hfem_fail(-1, "hfem_set_mesh: block (%d,%d) missing from the trace graph", i, j);
                 kidx[((size_t)e * d.nfe + a) * d.nfe + b] = k;
                 cnt[k + 1]++;
             }
         }
     for (size_t q = 0; q < einv.size(); q++)
-        if (einv[q] < 0) return fail(-1, "hfem_set_mesh: elcon of element %d maps two nodes to one trace node", (int)(q / d.ndf));
+        if (einv[q] < 0) return hfem_fail(-1, "hfem_set_mesh: elcon of element %d maps two nodes to one trace node", (int)(q / d.ndf));
     for (int k = 0; k < nblocks; k++) cnt[k + 1] += cnt[k];
     std::vector<int> list((size_t)cnt[nblocks] * 3), fillp(cnt.begin(), cnt.end() - 1);
     for (int e = 0; e < ne; e++)
@@ -290,6 +233,11 @@ extern "C" int hfem_set_mesh(hfem_ctx *c, const double *dgnodes, const int *elco
     if ((rc = upload(c, &c->blkptr, cnt.data(), cnt.size()))) return rc;
     if ((rc = upload(c, &c->blklist, list.data(), list.size()))) return rc;
     c->nblocks = nblocks;
+    c->sv.rowpts.assign(rowstart, rowstart + nf + 1);
+    c->sv.colind.assign(cols, cols + nblocks);
+    for (int i = 0; i < nf; i++)
+        if (rowstart[i + 1] <= rowstart[i] || cols[rowstart[i]] != i)
+            return hfem_fail(-1, "hfem_set_mesh: row %d of the trace graph must start with its diagonal block", i);
     const long bsz = (long)d.nch * d.npf;
     if (!c->Hg) HIPCHK(dalloc(&c->Hg, (size_t)nblocks * bsz * bsz));
     if (!c->Rg) HIPCHK(dalloc(&c->Rg, (size_t)nf * bsz));
@@ -302,7 +250,7 @@ extern "C" int hfem_set_mesh(hfem_ctx *c, const double *dgnodes, const int *elco
 
 extern "C" int hfem_set_solution(hfem_ctx *c, const double *UDG, const double *UH, const double *SH)
 {
-    if (!c || !UDG || !UH) return fail(-1, "hfem_set_solution: null argument");
+    if (!c || !UDG || !UH) return hfem_fail(-1, "hfem_set_solution: null argument");
     HIPCHK(hipSetDevice(c->device));
     const Dims &d = c->d;
     const long ne = c->hd.ne, nuh = (long)c->hd.nf * d.npf;
@@ -321,7 +269,7 @@ extern "C" int hfem_set_solution(hfem_ctx *c, const double *UDG, const double *U
 
 extern "C" int hfem_get_solution(hfem_ctx *c, double *UDG, double *UH)
 {
-    if (!c || !c->has_sol) return fail(-3, "hfem_get_solution: no solution set");
+    if (!c || !c->has_sol) return hfem_fail(-3, "hfem_get_solution: no solution set");
     HIPCHK(hipSetDevice(c->device));
     const Dims &d = c->d;
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -332,8 +280,8 @@ extern "C" int hfem_get_solution(hfem_ctx *c, double *UDG, double *UH)
 
 static int check_range(hfem_ctx *c, int e0, int n, const char *who)
 {
-    if (!c) return fail(-1, "%s: null context", who);
-    if (e0 < 0 || n < 0 || (long)e0 + n > c->hd.ne) return fail(-1, "%s: element range [%d,%d) outside [0,%d)", who, e0, e0 + n, c->hd.ne);
+    if (!c) return hfem_fail(-1, "%s: null context", who);
+    if (e0 < 0 || n < 0 || (long)e0 + n > c->hd.ne) return hfem_fail(-1, "%s: element range [%d,%d) outside [0,%d)", who, e0, e0 + n, c->hd.ne);
     return 0;
 }
 
@@ -342,14 +290,15 @@ extern "C" int hfem_assemble(hfem_ctx *c, int e0, int n)
     int rc = check_range(c, e0, n, "hfem_assemble");
     if (rc) return rc;
     if (!c->has_master || !c->has_phys || !c->has_mesh || !c->has_sol)
-        return fail(-3, "hfem_assemble: master, physics, mesh and solution must be set first");
+        return hfem_fail(-3, "hfem_assemble: master, physics, mesh and solution must be set first");
     if (n == 0) return 0;
     HIPCHK(hipSetDevice(c->device));
     Master m{c->shapvt, c->shapvg, c->shapft, c->shapfg, c->perm, c->invperm};
     const int grid = std::min(n, c->grid_asm);
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
     HIPCHK(launch_assemble(c->d, m, c->ph, c->dgnodes, c->UDG, c->UH, c->SH, c->elcon, c->bf, e0, n, c->scratch,
-                           c->sstride, grid, c->ldx, c->Ru, c->D, c->Fx, c->K, c->Hx, c->Rh, c->ru2, c->stream));
+                           c->sstride, grid, c->ldx, c->fluxbuf, c->fstride, c->chunk, c->Ru, c->D, c->Fx, c->K, c->Hx,
+                           c->Rh, c->ru2, c->stream));
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
     return 0;
 }
@@ -358,7 +307,7 @@ extern "C" int hfem_condense(hfem_ctx *c, int e0, int n, int precision)
 {
     int rc = check_range(c, e0, n, "hfem_condense");
     if (rc) return rc;
-    if (precision != HFEM_FP64 && precision != HFEM_MIXED) return fail(-1, "hfem_condense: unknown precision %d", precision);
+    if (precision != HFEM_FP64 && precision != HFEM_MIXED) return hfem_fail(-1, "hfem_condense: unknown precision %d", precision);
     if (n == 0) return 0;
     HIPCHK(hipSetDevice(c->device));
     const Dims &d = c->d;
@@ -376,7 +325,7 @@ extern "C" int hfem_condense(hfem_ctx *c, int e0, int n, int precision)
 
 extern "C" int hfem_last_kernel_ms(hfem_ctx *c, float *ams, float *cms)
 {
-    if (!c) return fail(-1, "hfem_last_kernel_ms: null context");
+    if (!c) return hfem_fail(-1, "hfem_last_kernel_ms: null context");
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (ams) { *ams = 0; if (hipEventElapsedTime(ams, c->ev[0], c->ev[1]) != hipSuccess) *ams = -1; }
@@ -387,7 +336,7 @@ extern "C" int hfem_last_kernel_ms(hfem_ctx *c, float *ams, float *cms)
 
 extern "C" int hfem_sync(hfem_ctx *c)
 {
-    if (!c) return fail(-1, "hfem_sync: null context");
+    if (!c) return hfem_fail(-1, "hfem_sync: null context");
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
@@ -455,7 +404,7 @@ extern "C" int hfem_read_condensed(hfem_ctx *c, int e0, int n, double *DinvRu, d
     return 0;
 }
 
-static int reduce_sum(hfem_ctx *c, long n, const double *x, int square, double *out)
+int hfem_reduce_sum(hfem_ctx *c, long n, const double *x, int square, double *out)
 {
     const int nblk = 256;
     HIPCHK(launch_sumsq(n, x, square, c->partial, nblk, c->stream));
@@ -467,7 +416,7 @@ static int reduce_sum(hfem_ctx *c, long n, const double *x, int square, double *
 
 extern "C" int hfem_assemble_linear_system(hfem_ctx *c, int precision, double *rNorm)
 {
-    if (!c) return fail(-1, "hfem_assemble_linear_system: null context");
+    if (!c) return hfem_fail(-1, "hfem_assemble_linear_system: null context");
     const int ne = c->hd.ne, nf = c->hd.nf;
     const Dims &d = c->d;
     int rc = hfem_assemble(c, 0, ne);
@@ -477,8 +426,8 @@ extern "C" int hfem_assemble_linear_system(hfem_ctx *c, int precision, double *r
     HIPCHK(launch_face_gather(nf, bsz, d.nch, d.npf, d.nb, d.nb, c->f2e, c->einv, c->Rh, c->r, nullptr, nullptr,
                               c->stream));
     double s1 = 0, s2 = 0;
-    if ((rc = reduce_sum(c, ne, c->ru2, 0, &s1))) return rc;
-    if ((rc = reduce_sum(c, (long)nf * bsz, c->r, 1, &s2))) return rc;
+    if ((rc = hfem_reduce_sum(c, ne, c->ru2, 0, &s1))) return rc;
+    if ((rc = hfem_reduce_sum(c, (long)nf * bsz, c->r, 1, &s2))) return rc;
     if (rNorm) *rNorm = sqrt(s1 + s2);
     if ((rc = hfem_condense(c, 0, ne, precision))) return rc;
     const long es = (long)d.nb * (d.nb + 1);
@@ -492,7 +441,7 @@ extern "C" int hfem_assemble_linear_system(hfem_ctx *c, int precision, double *r
 
 extern "C" int hfem_read_system(hfem_ctx *c, double *Hg, double *Rg)
 {
-    if (!c || !c->has_mesh) return fail(-3, "hfem_read_system: mesh not set");
+    if (!c || !c->has_mesh) return hfem_fail(-3, "hfem_read_system: mesh not set");
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipStreamSynchronize(c->stream));
     const long bsz = (long)c->d.nch * c->d.npf;
@@ -503,7 +452,7 @@ extern "C" int hfem_read_system(hfem_ctx *c, double *Hg, double *Rg)
 
 extern "C" int hfem_bsr_matvec(hfem_ctx *c, const double *x, double *y)
 {
-    if (!c || !c->has_mesh || !x || !y) return fail(-1, "hfem_bsr_matvec: bad argument or mesh not set");
+    if (!c || !c->has_mesh || !x || !y) return hfem_fail(-1, "hfem_bsr_matvec: bad argument or mesh not set");
     HIPCHK(hipSetDevice(c->device));
     const long bsz = (long)c->d.nch * c->d.npf, n = c->hd.nf * bsz;
     HIPCHK(hipMemcpyAsync(c->xv, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));

@@ -9,10 +9,13 @@
 //
 // Phases (barriers between them; inside a phase threads own disjoint outputs):
 //   P0 element data and the master tables in LDS  P5 traces of M^{-1}C, M^{-1}E;
-//   P1 volume/face geometry                          solution at quadrature points
-//   P2 q operators M, C, E (qint)                  P6 flux Jacobians by forward-mode AD,
-//   P3 Cholesky of M                                  one Jacobian column per thread
-//   P4 M^{-1}C, M^{-1}E (column per thread)        P7 wrl = -[s_u jac ; f_u . Xx]
+//   P1 volume/face geometry                          solution at volume points
+//   P2 q operators M, C, E (qint)                  P7 wrl = -[s_u jac ; f_u . Xx]
+//   P3 M^{-1} (Gauss-Jordan)
+//   P4 Q = [shapvt; face traces] M^{-1}, so that P5 forms the traces of M^{-1}C
+//      and M^{-1}E as Q C and Q E without solving per column
+//   (the pointwise fluxes and their Jacobians come from flux_kernel, launched
+//   before this kernel on the same element chunk)
 //   P8 element rows: one thread per column of D or F, all NPV rows in registers;
 //      the q-folded integrand (elementint :510-549, faceint :962-1072) is formed
 //      on the fly and contracted with the test functions, volume then faces
@@ -21,7 +24,11 @@
 //      Ru and Rh entries.
 #include "hfem_device.h"
 
+#include <algorithm>
+
 namespace hfem {
+
+constexpr int ASM_THREADS = 512;
 
 // LDS copies of the master tables, transposed so that the thread-varying index is
 // fastest: svt[n + npv*(g + ngv*dd)], svg[i + npv*(g + ngv*dd)] (as shapvg),
@@ -32,16 +39,17 @@ struct LdsTables {
 };
 
 template <int ND, int NPV, int NPF>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(ASM_THREADS)
 assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, const double *__restrict__ UDG,
                 const double *__restrict__ UH, const double *__restrict__ SH, const int *__restrict__ elcon,
                 const int *__restrict__ bf, int e0, int n, double *__restrict__ scratch, long sstride, int ldx,
-                double *__restrict__ Ru, double *__restrict__ Dm, double *__restrict__ Fx,
-                double *__restrict__ Km, double *__restrict__ Hx, double *__restrict__ Rh,
-                double *__restrict__ ru2)
+                const double *__restrict__ fluxbuf, long fstride, double *__restrict__ Ru,
+                double *__restrict__ Dm, double *__restrict__ Fx, double *__restrict__ Km, double *__restrict__ Hx,
+                double *__restrict__ Rh, double *__restrict__ ru2)
 {
     extern __shared__ double lds[];
     const int tid = threadIdx.x, nth = blockDim.x;
+    const FluxLayout FL = flux_layout(d);
     constexpr int NCU = ND + 2, NC = NCU * (ND + 1), ND1 = ND + 1;
     const int npv = NPV, ngv = d.ngv, npf = NPF, ngf = d.ngf, nfe = d.nfe, ndf = d.ndf;
     const int ncu = NCU, nch = NCU, nc = NC, na = d.na, nb = d.nb, nd1 = ND1, np = d.np;
@@ -60,13 +68,14 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
     double *Xx = p; p += ngv * ND * ND;        // Xx(g,x,r) = jac d xi_r / d x_x
     double *jacf = p; p += np;
     double *nl = p; p += np * ND;
-    double *M = p; p += npv * npv;
+    double *M = p; p += npv * npv;             // mass matrix, then its inverse
+    double *Qm = p; p += (ngv + np) * npv;     // [k + npv*r]: rows of shapvt / face traces times M^{-1}
+    double *rowk = p; p += npv;
+    double *colk = p; p += npv;
     double *shapMiC = p; p += npv * ngv * ND;  // [n + npv*(g + ngv*x)]
     double *shapMiE = p; p += ndf * ngv * ND;  // [l + ndf*(g + ngv*x)], includes the minus sign
     double *udgg = p; p += ngv * nc;           // [g + ngv*c]
-    double *ufg = p; p += np * nc;             // u, q traced on the faces [p + np*c]
-    double *uhgl = p; p += np * nch;           // uh at face points [p + np*c]
-    double *red = p; p += 256;
+    double *red = p; p += ASM_THREADS;
     int *perm = (int *)p;
     int *invperm = perm + npf * nfe;
     int *bnd = invperm + npv * nfe;
@@ -74,9 +83,7 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
     // ---- per-workgroup scratch (global) ----
     double *S = scratch + (long)blockIdx.x * sstride;
     const Scratch L = scratch_layout(d);
-    double *C = S + L.C, *E = S + L.E, *shapMiCf = S + L.shapMiCf, *shapMiEf = S + L.shapMiEf;
-    double *f = S + L.f, *fu = S + L.fu, *fh = S + L.fh, *fhu = S + L.fhu, *fhuh = S + L.fhuh;
-    double *fb = S + L.fb, *fbu = S + L.fbu, *fbuh = S + L.fbuh, *wrl = S + L.wrl;
+    double *C = S + L.C, *E = S + L.E, *shapMiCf = S + L.shapMiCf, *shapMiEf = S + L.shapMiEf, *wrl = S + L.wrl;
     const double tau = ph.param[5];
 
     // master tables once per workgroup
@@ -101,6 +108,9 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
 
     for (int t = blockIdx.x; t < n; t += gridDim.x) {
         const long e = e0 + t;
+        const double *FB = fluxbuf + (long)t * fstride;
+        const double *f = FB + FL.f, *fu = FB + FL.fu, *fh = FB + FL.fh, *fhu = FB + FL.fhu, *fhuh = FB + FL.fhuh;
+        const double *fb = FB + FL.fb, *fbu = FB + FL.fbu, *fbuh = FB + FL.fbuh;
         // ---------------- P0: gather ----------------
         for (int i = tid; i < npv * ND; i += nth) pn[i] = dgnodes[e * npv * ND + i];
         for (int i = tid; i < npv * nc; i += nth) udg[i] = UDG[e * npv * nc + i];
@@ -194,150 +204,74 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
         }
         __syncthreads();
 
-        // ---------------- P3: Cholesky M = U^T U in LDS ----------------
-        for (int j = 0; j < npv; j++) {
-            if (tid == 0) M[j + npv * j] = sqrt(M[j + npv * j]);
+        // ---------------- P3: M^{-1} by Gauss-Jordan in LDS (M is SPD: no pivoting) ----------------
+        for (int k = 0; k < npv; k++) {
+            for (int i = tid; i < 2 * npv; i += nth) {
+                if (i < npv) rowk[i] = M[k + npv * i];      // M(k, i)
+                else colk[i - npv] = M[(i - npv) + npv * k];  // M(i, k)
+            }
             __syncthreads();
-            const double ujj = M[j + npv * j];
-            for (int i = j + 1 + tid; i < npv; i += nth) M[j + npv * i] /= ujj;
-            __syncthreads();
-            const int w = npv - j - 1;
-            for (int idx = tid; idx < w * w; idx += nth) {
-                int k = j + 1 + idx % w, i = j + 1 + idx / w;
-                if (k <= i) M[k + npv * i] -= M[j + npv * k] * M[j + npv * i];
+            const double pinv = 1.0 / rowk[k];
+            for (int idx = tid; idx < npv * npv; idx += nth) {
+                const int i = idx % npv, j = idx / npv;       // M(i, j) at i + npv*j
+                double v;
+                if (i == k) v = (j == k) ? pinv : rowk[j] * pinv;
+                else if (j == k) v = -colk[i] * pinv;
+                else v = M[idx] - colk[i] * rowk[j] * pinv;
+                M[idx] = v;
             }
             __syncthreads();
         }
 
-        // ---------------- P4: M^{-1} C, M^{-1} E (a column per thread) ----------------
-        for (int col = tid; col < ND * (npv + ndf); col += nth) {
-            double *b = col < ND * npv ? C + (long)col * npv : E + (long)(col - ND * npv) * npv;
-            double x[NPV];
-#pragma unroll
-            for (int i = 0; i < NPV; i++) x[i] = b[i];
-#pragma unroll
-            for (int i = 0; i < NPV; i++) {
-                double s = x[i];
-#pragma unroll
-                for (int k = 0; k < i; k++) s -= M[k + npv * i] * x[k];
-                x[i] = s / M[i + npv * i];
+        // ---------------- P4: Q = [shapvt ; face traces] M^{-1} ----------------
+        for (int idx = tid; idx < (ngv + np) * npv; idx += nth) {
+            const int k = idx % npv, r = idx / npv;
+            double s = 0;
+            if (r < ngv) {
+                for (int l = 0; l < npv; l++) s += SVT(r, l, 0) * M[l + npv * k];
+            } else {
+                const int pp = r - ngv, g = pp % ngf, a = pp / ngf;
+                for (int q = 0; q < npf; q++) s += SFT(g, q, 0) * M[PERM(q, a) + npv * k];
             }
-#pragma unroll
-            for (int i = NPV - 1; i >= 0; i--) {
-                double s = x[i];
-#pragma unroll
-                for (int k = i + 1; k < NPV; k++) s -= M[i + npv * k] * x[k];
-                x[i] = s / M[i + npv * i];
-            }
-#pragma unroll
-            for (int i = 0; i < NPV; i++) b[i] = x[i];
+            Qm[k + npv * r] = s;
         }
         __syncthreads();
 
         // ---------------- P5: traces of M^{-1}C, M^{-1}E; solution at points ----------------
         {
             const int n1 = ngv * npv * ND, n2 = ngv * ndf * ND, n3 = np * npv * ND, n4 = np * ndf * ND;
-            const int n5 = ngv * nc, n6 = np * nc, n7 = np * nch;
-            const int tot = n1 + n2 + n3 + n4 + n5 + n6 + n7;
+            const int n5 = ngv * nc;
+            const int tot = n1 + n2 + n3 + n4 + n5;
             for (int i = tid; i < tot; i += nth) {
                 if (i < n1) {
                     int nn = i % npv, r = i / npv, g = r % ngv, x = r / ngv;
+                    const double *q = Qm + npv * g, *c = C + x * npv * npv + nn * npv;
                     double s = 0;
-                    for (int l = 0; l < npv; l++) s += SVT(g, l, 0) * C[x * npv * npv + nn * npv + l];
+                    for (int l = 0; l < npv; l++) s += q[l] * c[l];
                     shapMiC[i] = s;
                 } else if (i < n1 + n2) {
                     int r0 = i - n1, l = r0 % ndf, r = r0 / ndf, g = r % ngv, x = r / ngv;
+                    const double *q = Qm + npv * g, *c = E + (long)x * npv * ndf + l * npv;
                     double s = 0;
-                    for (int k = 0; k < npv; k++) s += SVT(g, k, 0) * E[(long)x * npv * ndf + l * npv + k];
+                    for (int k = 0; k < npv; k++) s += q[k] * c[k];
                     shapMiE[r0] = -s;
                 } else if (i < n1 + n2 + n3) {
                     int r0 = i - n1 - n2, nn = r0 % npv, r = r0 / npv, pp = r % np, x = r / np;
-                    int g = pp % ngf, a = pp / ngf;
+                    const double *q = Qm + npv * (ngv + pp), *c = C + x * npv * npv + nn * npv;
                     double s = 0;
-                    for (int k = 0; k < npf; k++) s += SFT(g, k, 0) * C[x * npv * npv + nn * npv + PERM(k, a)];
+                    for (int k = 0; k < npv; k++) s += q[k] * c[k];
                     shapMiCf[r0] = s;
                 } else if (i < n1 + n2 + n3 + n4) {
                     int r0 = i - n1 - n2 - n3, l = r0 % ndf, r = r0 / ndf, pp = r % np, x = r / np;
-                    int g = pp % ngf, a = pp / ngf;
+                    const double *q = Qm + npv * (ngv + pp), *c = E + (long)x * npv * ndf + l * npv;
                     double s = 0;
-                    for (int k = 0; k < npf; k++) s += SFT(g, k, 0) * E[(long)x * npv * ndf + l * npv + PERM(k, a)];
+                    for (int k = 0; k < npv; k++) s += q[k] * c[k];
                     shapMiEf[r0] = s;
-                } else if (i < n1 + n2 + n3 + n4 + n5) {
+                } else {
                     int r0 = i - n1 - n2 - n3 - n4, g = r0 % ngv, c = r0 / ngv;
                     double s = 0;
                     for (int q = 0; q < npv; q++) s += SVT(g, q, 0) * udg[c * npv + q];
                     udgg[r0] = s;
-                } else if (i < n1 + n2 + n3 + n4 + n5 + n6) {
-                    int r0 = i - n1 - n2 - n3 - n4 - n5, pp = r0 % np, c = r0 / np, g = pp % ngf, a = pp / ngf;
-                    double s = 0;
-                    for (int q = 0; q < npf; q++) s += SFT(g, q, 0) * udg[c * npv + PERM(q, a)];
-                    ufg[r0] = s;
-                } else {
-                    int r0 = i - n1 - n2 - n3 - n4 - n5 - n6, pp = r0 % np, c = r0 / np, g = pp % ngf, a = pp / ngf;
-                    double s = 0;
-                    for (int q = 0; q < npf; q++) s += SFT(g, q, 0) * uh[c * ndf + a * npf + q];
-                    uhgl[r0] = s;
-                }
-            }
-        }
-        __syncthreads();
-
-        // ---------------- P6: fluxes and Jacobian columns ----------------
-        {
-            const int nvol = ngv * NC, nfac = np * NC, nbnd = np * NCU;
-            for (int i = tid; i < nvol + nfac + nbnd; i += nth) {
-                if (i < nvol) {
-                    const int g = i % ngv, k = i / ngv;
-                    dual w[NC], fl[NCU * ND];
-                    for (int c = 0; c < NC; c++) w[c] = mk(udgg[g + ngv * c], c == k ? 1.0 : 0.0);
-                    ns_flux<ND>(ph.param, w, fl);
-                    for (int jx = 0; jx < NCU * ND; jx++) {
-                        fu[g + ngv * (jx + NCU * ND * k)] = fl[jx].d;
-                        if (k == 0) f[g + ngv * jx] = fl[jx].v;
-                    }
-                } else if (i < nvol + nfac) {
-                    const int r = i - nvol, pp = r % np, k = r / np;
-                    dual w[NC], fl[NCU * ND];
-                    for (int c = 0; c < NC; c++)
-                        w[c] = mk(c < NCU ? uhgl[pp + np * c] : ufg[pp + np * c], c == k ? 1.0 : 0.0);
-                    ns_flux<ND>(ph.param, w, fl);
-                    for (int j = 0; j < NCU; j++) {
-                        dual fn = mk(0.0);
-                        for (int x = 0; x < ND; x++) fn = fn + nl[pp + np * x] * fl[j + NCU * x];
-                        fhu[pp + np * (j + NCU * k)] = (k < NCU) ? (k == j ? tau : 0.0) : fn.d;
-                        if (k < NCU) fhuh[pp + np * (j + NCU * k)] = fn.d - (k == j ? tau : 0.0);
-                        if (k == 0) fh[pp + np * j] = fn.v + tau * (ufg[pp + np * j] - w[j].v);
-                    }
-                } else {
-                    const int r = i - nvol - nfac, pp = r % np, k = r / np, a = pp / ngf;
-                    if (!bnd[a]) continue;
-                    // fb = (A+|A|)(u-uh) - (A-|A|)(ui-uh) = A (u-ui) + |A| (u+ui-2uh): one
-                    // Jacobian matrix live at a time
-                    dual uhd[NCU], Am[NCU * NCU], acc[NCU];
-                    double nrm[ND];
-                    for (int c = 0; c < NCU; c++) uhd[c] = mk(uhgl[pp + np * c], c == k ? 1.0 : 0.0);
-                    for (int x = 0; x < ND; x++) nrm[x] = nl[pp + np * x];
-                    for (int j = 0; j < NCU; j++) acc[j] = mk(0.0);
-                    for (int absflag = 0; absflag < 2; absflag++) {
-                        ns_normal_jac<ND>(ph.param[0], ph.param[1], uhd, nrm, absflag, Am);
-                        for (int j = 0; j < NCU; j++)
-                            for (int q = 0; q < NCU; q++) {
-                                const dual uq = mk(ufg[pp + np * q]);
-                                const dual w = absflag ? uq + mk(ph.ui[q]) - 2.0 * uhd[q] : uq - mk(ph.ui[q]);
-                                acc[j] = acc[j] + Am[q * NCU + j] * w;
-                                if (k == 0) {
-                                    double *o = &fbu[pp + np * (j + NCU * q)];
-                                    *o = absflag ? *o + Am[q * NCU + j].v : Am[q * NCU + j].v;
-                                }
-                            }
-                    }
-                    for (int j = 0; j < NCU; j++) {
-                        fbuh[pp + np * (j + NCU * k)] = acc[j].d;
-                        if (k == 0) {
-                            fb[pp + np * j] = acc[j].v;
-                            for (int q = NCU; q < NC; q++) fbu[pp + np * (j + NCU * q)] = 0.0;
-                        }
-                    }
                 }
             }
         }
@@ -384,8 +318,10 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                         for (int x = 0; x < ND; x++)
                             w += wg[cs * (ncu * (x + 1) + kc)] *
                                  (isD ? shapMiC[cc + npv * (g + ngv * x)] : shapMiE[cc + ndf * (g + ngv * x)]);
+                        // uniform index: the test functions come through the scalar cache
+                        const double *sv = m.shapvg + npv * (g + ngv * mq);
 #pragma unroll
-                        for (int i = 0; i < NPV; i++) acc[i] += SVG(i, g, mq) * w;
+                        for (int i = 0; i < NPV; i++) acc[i] += sv[i] * w;
                     }
                 // faces: rows on face a get sum_g shapfg(kk,g) jacf z(p)
                 for (int a = 0; a < nfe; a++) {
@@ -407,8 +343,8 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                         const double jz = jacf[pp] * z;
 #pragma unroll
                         for (int i = 0; i < NPV; i++) {
-                            const int kk = INVP(i, a);
-                            if (kk >= 0) acc[i] += SFG(kk, g, 0) * jz;
+                            const int kk = m.invperm[i + npv * a];
+                            if (kk >= 0) acc[i] += m.shapfg[kk + npf * g] * jz;
                         }
                     }
                 }
@@ -457,7 +393,7 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                         }
                         const double jz = jacf[pp] * z;
 #pragma unroll
-                        for (int k = 0; k < NPF; k++) acc[k] += SFG(k, g, 0) * jz;
+                        for (int k = 0; k < NPF; k++) acc[k] += m.shapfg[k + npf * g] * jz;
                     }
                     if (isK) {
                         double *Kc = Km + e * (long)nb * na + (long)nb * (kc * npv + cc) + (a * npf) * nch + j;
@@ -525,13 +461,361 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
 #undef INVP
 }
 
+
+// Far-field boundary flux at one face point (application/FM/fbou.cpp:126-217, ib 1):
+// fb = (A+|A|)(u-uh) - (A-|A|)(ui-uh) = A(uh)(u-ui) + |A|(uh)(u+ui-2uh), with A the
+// normal flux Jacobian at uh (getan).  Column k of d fb / d uh goes to fbuh; the
+// direction-0 call also writes fb and d fb / d u (zero in the q columns).  Point
+// arrays are strided by np: uh[np*c], u[np*c], nl[np*x], outputs [np*(j + NCU*col)].
+template <int ND>
+__device__ __forceinline__ void boundary_point(const Phys &ph, const double *uh, const double *u, const double *nl,
+                                            int np, int k, double *fb, double *fbu, double *fbuh)
+{
+    constexpr int NCU = ND + 2, NC = NCU * (ND + 1);
+    dual uhd[NCU], Am[NCU * NCU], acc[NCU];
+    double nrm[ND];
+    for (int c = 0; c < NCU; c++) uhd[c] = mk(uh[np * c], c == k ? 1.0 : 0.0);
+    for (int x = 0; x < ND; x++) nrm[x] = nl[np * x];
+    for (int j = 0; j < NCU; j++) acc[j] = mk(0.0);
+    for (int absflag = 0; absflag < 2; absflag++) {
+        ns_normal_jac<ND>(ph.param[0], ph.param[1], uhd, nrm, absflag, Am);
+        for (int q = 0; q < NCU; q++) {
+            const dual uq = mk(u[np * q]);
+            const dual w = absflag ? uq + mk(ph.ui[q]) - 2.0 * uhd[q] : uq - mk(ph.ui[q]);
+            for (int j = 0; j < NCU; j++) acc[j] = acc[j] + Am[q * NCU + j] * w;
+        }
+        if (k == 0)
+            for (int q = 0; q < NCU; q++)
+                for (int j = 0; j < NCU; j++) {
+                    double *o = &fbu[np * (j + NCU * q)];
+                    *o = absflag ? *o + Am[q * NCU + j].v : Am[q * NCU + j].v;
+                }
+    }
+    for (int j = 0; j < NCU; j++) fbuh[np * (j + NCU * k)] = acc[j].d;
+    if (k == 0)
+        for (int j = 0; j < NCU; j++) {
+            fb[np * j] = acc[j].v;
+            for (int q = NCU; q < NC; q++) fbu[np * (j + NCU * q)] = 0.0;
+        }
+}
+
+// Pointwise fluxes and Jacobian columns (elementflux, :239-305; faceflux/fhat,
+// :575-677 and application/fhatDriver.cpp:54-179; far-field fbou,
+// application/FM/fbou.cpp:126-217).  Split from the assembly kernel so that the
+// dual-number register pressure does not limit the contraction phases.
+template <int ND, int NPV, int NPF>
+__global__ void __launch_bounds__(256)
+flux_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, const double *__restrict__ UDG,
+            const double *__restrict__ UH, const int *__restrict__ elcon, const int *__restrict__ bf, int e0, int n,
+            double *__restrict__ fluxbuf, long fstride)
+{
+    extern __shared__ double lds[];
+    const int tid = threadIdx.x, nth = blockDim.x;
+    constexpr int NCU = ND + 2, NC = NCU * (ND + 1);
+    const int npv = NPV, ngv = d.ngv, npf = NPF, ngf = d.ngf, nfe = d.nfe, ndf = d.ndf, nch = NCU, nc = NC;
+    const int np = d.np;
+    double *p = lds;
+    double *pn = p; p += npv * ND;
+    double *udg = p; p += npv * nc;
+    double *uh = p; p += ndf * nch;
+    double *nl = p; p += np * ND;
+    double *udgg = p; p += ngv * nc;
+    double *ufg = p; p += np * nc;
+    double *uhgl = p; p += np * nch;
+    int *bnd = (int *)p;
+    const FluxLayout FL = flux_layout(d);
+    const double tau = ph.param[5];
+#define SVT(g, nn, dd) m.shapvt[(g) + ngv * ((nn) + npv * (dd))]
+#define SFT(g, k, dd) m.shapft[(g) + ngf * ((k) + npf * (dd))]
+#define PERM(k, a) m.perm[(k) + npf * (a)]
+    for (int t = blockIdx.x; t < n; t += gridDim.x) {
+        const long e = e0 + t;
+        double *F = fluxbuf + (long)t * fstride;
+        double *f = F + FL.f, *fu = F + FL.fu, *fh = F + FL.fh, *fhu = F + FL.fhu, *fhuh = F + FL.fhuh;
+        double *fb = F + FL.fb, *fbu = F + FL.fbu, *fbuh = F + FL.fbuh;
+        for (int i = tid; i < npv * ND; i += nth) pn[i] = dgnodes[e * npv * ND + i];
+        for (int i = tid; i < npv * nc; i += nth) udg[i] = UDG[e * npv * nc + i];
+        for (int i = tid; i < ndf * nch; i += nth) {
+            int j = i / nch, k = i % nch;
+            uh[k * ndf + j] = UH[(long)elcon[e * ndf + j] * nch + k];
+        }
+        for (int a = tid; a < nfe; a += nth) bnd[a] = bf[e * nfe + a] < 0;
+        __syncthreads();
+        // unit normals at face points, solution at volume and face points
+        for (int i = tid; i < np + ngv * nc + np * nc + np * nch; i += nth) {
+            if (i < np) {
+                const int pp = i, g = pp % ngf, a = pp / ngf;
+                double tg[2][3];
+                for (int r = 0; r < ND - 1; r++)
+                    for (int x = 0; x < ND; x++) {
+                        double s = 0;
+                        for (int k = 0; k < npf; k++) s += SFT(g, k, 1 + r) * pn[x * npv + PERM(k, a)];
+                        tg[r][x] = s;
+                    }
+                double nv[3];
+                if (ND == 2) { nv[0] = tg[0][1]; nv[1] = -tg[0][0]; nv[2] = 0; }
+                else {
+                    nv[0] = tg[0][1] * tg[1][2] - tg[0][2] * tg[1][1];
+                    nv[1] = tg[0][2] * tg[1][0] - tg[0][0] * tg[1][2];
+                    nv[2] = tg[0][0] * tg[1][1] - tg[0][1] * tg[1][0];
+                }
+                double len = 0;
+                for (int x = 0; x < ND; x++) len += nv[x] * nv[x];
+                len = sqrt(len);
+                for (int x = 0; x < ND; x++) nl[pp + np * x] = nv[x] / len;
+            } else if (i < np + ngv * nc) {
+                int r0 = i - np, g = r0 % ngv, c = r0 / ngv;
+                double s = 0;
+                for (int q = 0; q < npv; q++) s += SVT(g, q, 0) * udg[c * npv + q];
+                udgg[r0] = s;
+            } else if (i < np + ngv * nc + np * nc) {
+                int r0 = i - np - ngv * nc, pp = r0 % np, c = r0 / np, g = pp % ngf, a = pp / ngf;
+                double s = 0;
+                for (int q = 0; q < npf; q++) s += SFT(g, q, 0) * udg[c * npv + PERM(q, a)];
+                ufg[r0] = s;
+            } else {
+                int r0 = i - np - ngv * nc - np * nc, pp = r0 % np, c = r0 / np, g = pp % ngf, a = pp / ngf;
+                double s = 0;
+                for (int q = 0; q < npf; q++) s += SFT(g, q, 0) * uh[c * ndf + a * npf + q];
+                uhgl[r0] = s;
+            }
+        }
+        __syncthreads();
+        {
+            const int nvol = ngv * NC, nfac = np * NC, nbnd = np * NCU;
+            for (int i = tid; i < nvol; i += nth) {
+                const int g = i % ngv, k = i / ngv;
+                dual w[NC], fl[NCU * ND];
+                for (int c = 0; c < NC; c++) w[c] = mk(udgg[g + ngv * c], c == k ? 1.0 : 0.0);
+                ns_flux<ND>(ph.param, w, fl);
+                for (int jx = 0; jx < NCU * ND; jx++) {
+                    fu[g + ngv * (jx + NCU * ND * k)] = fl[jx].d;
+                    if (k == 0) f[g + ngv * jx] = fl[jx].v;
+                }
+            }
+            for (int r = tid; r < nfac; r += nth) {
+                const int pp = r % np, k = r / np;
+                dual w[NC], fl[NCU * ND];
+                for (int c = 0; c < NC; c++)
+                    w[c] = mk(c < NCU ? uhgl[pp + np * c] : ufg[pp + np * c], c == k ? 1.0 : 0.0);
+                ns_flux<ND>(ph.param, w, fl);
+                for (int j = 0; j < NCU; j++) {
+                    dual fn = mk(0.0);
+                    for (int x = 0; x < ND; x++) fn = fn + nl[pp + np * x] * fl[j + NCU * x];
+                    fhu[pp + np * (j + NCU * k)] = (k < NCU) ? (k == j ? tau : 0.0) : fn.d;
+                    if (k < NCU) fhuh[pp + np * (j + NCU * k)] = fn.d - (k == j ? tau : 0.0);
+                    if (k == 0) fh[pp + np * j] = fn.v + tau * (ufg[pp + np * j] - w[j].v);
+                }
+            }
+            for (int r = tid; r < nbnd; r += nth) {
+                const int pp = r % np, k = r / np, a = pp / ngf;
+                if (bnd[a]) boundary_point<ND>(ph, uhgl + pp, ufg + pp, nl + pp, np, k, fb + pp, fbu + pp, fbuh + pp);
+            }
+        }
+
+        __syncthreads();
+    }
+#undef SVT
+#undef SFT
+#undef PERM
+}
+
+
+// getQ (assembly/elementassembly.cpp:1414-1479): q = M^{-1}(M sh_q + C u - E uh) for
+// every element of [e0, e0+n), written into the q components of UDG.  C u and E uh
+// are contracted at the quadrature points without forming C and E.  One
+// workgroup per element.
+template <int ND, int NPV, int NPF>
+__global__ void __launch_bounds__(256)
+getq_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, double *__restrict__ UDG,
+            const double *__restrict__ UH, const double *__restrict__ SH, const int *__restrict__ elcon,
+            const int *__restrict__ bf, int e0, int n)
+{
+    extern __shared__ double lds[];
+    const int tid = threadIdx.x, nth = blockDim.x;
+    constexpr int NCU = ND + 2, NC = NCU * (ND + 1);
+    const int npv = NPV, ngv = d.ngv, npf = NPF, ngf = d.ngf, nfe = d.nfe, ndf = d.ndf, np = d.np;
+    double *p = lds;
+    double *pn = p; p += npv * ND;
+    double *u = p; p += npv * NCU;
+    double *uh = p; p += ndf * NCU;            // uh[k*ndf + j]
+    double *jac = p; p += ngv;
+    double *Xx = p; p += ngv * ND * ND;
+    double *jacf = p; p += np;
+    double *nl = p; p += np * ND;
+    double *M = p; p += npv * npv;
+    double *Mi = p; p += npv * npv;
+    double *rowk = p; p += npv;
+    double *colk = p; p += npv;
+    double *ug = p; p += ngv * NCU;            // [g + ngv*c]
+    double *uhg = p; p += np * NCU;            // [pp + np*c]
+    double *rhs = p; p += npv * NCU * ND;      // [n + npv*(c + NCU*x)]
+#define SVT(g, nn, dd) m.shapvt[(g) + ngv * ((nn) + npv * (dd))]
+#define SVG(i, g, dd) m.shapvg[(i) + npv * ((g) + ngv * (dd))]
+#define SFT(g, k, dd) m.shapft[(g) + ngf * ((k) + npf * (dd))]
+#define SFG(k, g, dd) m.shapfg[(k) + npf * ((g) + ngf * (dd))]
+#define PERM(k, a) m.perm[(k) + npf * (a)]
+#define INVP(nn, a) m.invperm[(nn) + npv * (a)]
+    for (int t = blockIdx.x; t < n; t += gridDim.x) {
+        const long e = e0 + t;
+        double *ue = UDG + e * npv * NC;
+        const double *she = SH + e * npv * NC;
+        for (int i = tid; i < npv * ND; i += nth) pn[i] = dgnodes[e * npv * ND + i];
+        for (int i = tid; i < npv * NCU; i += nth) u[i] = ue[i];
+        for (int i = tid; i < ndf * NCU; i += nth) {
+            const int j = i / NCU, k = i % NCU;
+            uh[k * ndf + j] = UH[(long)elcon[e * ndf + j] * NCU + k];
+        }
+        __syncthreads();
+        // geometry (elementgeom :74-124, facegeom :206-232)
+        for (int i = tid; i < ngv + np; i += nth) {
+            if (i < ngv) {
+                const int g = i;
+                double J[ND][ND];
+                for (int x = 0; x < ND; x++)
+                    for (int r = 0; r < ND; r++) {
+                        double s = 0;
+                        for (int k = 0; k < npv; k++) s += SVT(g, k, 1 + r) * pn[x * npv + k];
+                        J[x][r] = s;
+                    }
+                double adj[ND][ND], det;
+                if (ND == 2) {
+                    det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+                    adj[0][0] = J[1][1]; adj[0][1] = -J[0][1]; adj[1][0] = -J[1][0]; adj[1][1] = J[0][0];
+                } else {
+                    for (int r = 0; r < ND; r++)
+                        for (int x = 0; x < ND; x++) {
+                            int x1 = (x + 1) % 3, x2 = (x + 2) % 3, r1 = (r + 1) % 3, r2 = (r + 2) % 3;
+                            adj[r][x] = J[x1][r1] * J[x2][r2] - J[x1][r2] * J[x2][r1];
+                        }
+                    det = J[0][0] * adj[0][0] + J[1][0] * adj[0][1] + J[2][0] * adj[0][2];
+                }
+                jac[g] = det;
+                for (int x = 0; x < ND; x++)
+                    for (int r = 0; r < ND; r++) Xx[g + ngv * (x + ND * r)] = adj[r][x];
+            } else {
+                const int pp = i - ngv, g = pp % ngf, a = pp / ngf;
+                double tg[2][3];
+                for (int r = 0; r < ND - 1; r++)
+                    for (int x = 0; x < ND; x++) {
+                        double s = 0;
+                        for (int k = 0; k < npf; k++) s += SFT(g, k, 1 + r) * pn[x * npv + PERM(k, a)];
+                        tg[r][x] = s;
+                    }
+                double nv[3];
+                if (ND == 2) { nv[0] = tg[0][1]; nv[1] = -tg[0][0]; nv[2] = 0; }
+                else {
+                    nv[0] = tg[0][1] * tg[1][2] - tg[0][2] * tg[1][1];
+                    nv[1] = tg[0][2] * tg[1][0] - tg[0][0] * tg[1][2];
+                    nv[2] = tg[0][0] * tg[1][1] - tg[0][1] * tg[1][0];
+                }
+                double len = 0;
+                for (int x = 0; x < ND; x++) len += nv[x] * nv[x];
+                len = sqrt(len);
+                jacf[pp] = len;
+                for (int x = 0; x < ND; x++) nl[pp + np * x] = nv[x] / len;
+            }
+        }
+        __syncthreads();
+        // mass matrix (qint :1374) and the traces of u and uh
+        for (int i = tid; i < npv * npv + ngv * NCU + np * NCU; i += nth) {
+            if (i < npv * npv) {
+                const int jj = i % npv, ii = i / npv;
+                double s = 0;
+                for (int g = 0; g < ngv; g++) s += (SVG(jj, g, 0) * SVT(g, ii, 0)) * jac[g];
+                M[i] = ph.fc_q * s;
+                Mi[i] = M[i];
+            } else if (i < npv * npv + ngv * NCU) {
+                const int r = i - npv * npv, g = r % ngv, c = r / ngv;
+                double s = 0;
+                for (int q = 0; q < npv; q++) s += SVT(g, q, 0) * u[c * npv + q];
+                ug[r] = s;
+            } else {
+                const int r = i - npv * npv - ngv * NCU, pp = r % np, c = r / np, g = pp % ngf, a = pp / ngf;
+                double s = 0;
+                for (int q = 0; q < npf; q++) s += SFT(g, q, 0) * uh[c * ndf + a * npf + q];
+                uhg[r] = s;
+            }
+        }
+        __syncthreads();
+        // rhs = M sh_q + C u - E uh
+        for (int i = tid; i < npv * NCU * ND; i += nth) {
+            const int nn = i % npv, r = i / npv, c = r % NCU, x = r / NCU;
+            double s = 0;
+            const double *shq = she + (long)(NCU + x * NCU + c) * npv;
+            for (int l = 0; l < npv; l++) s += M[nn + npv * l] * shq[l];
+            for (int g = 0; g < ngv; g++) {
+                double cx = 0;
+                for (int q = 0; q < ND; q++) cx += SVG(nn, g, q + 1) * Xx[g + ngv * (x + ND * q)];
+                s += cx * ug[g + ngv * c];
+            }
+            for (int a = 0; a < nfe; a++) {
+                const int k = INVP(nn, a);
+                if (k < 0) continue;
+                for (int g = 0; g < ngf; g++) {
+                    const int pp = g + ngf * a;
+                    s -= SFG(k, g, 0) * uhg[pp + np * c] * nl[pp + np * x] * jacf[pp];
+                }
+            }
+            rhs[i] = s;
+        }
+        // M^{-1} by Gauss-Jordan (M is SPD)
+        for (int k = 0; k < npv; k++) {
+            __syncthreads();
+            for (int i = tid; i < 2 * npv; i += nth) {
+                if (i < npv) rowk[i] = Mi[k + npv * i];
+                else colk[i - npv] = Mi[(i - npv) + npv * k];
+            }
+            __syncthreads();
+            const double pinv = 1.0 / rowk[k];
+            for (int idx = tid; idx < npv * npv; idx += nth) {
+                const int i = idx % npv, j = idx / npv;
+                double v;
+                if (i == k) v = (j == k) ? pinv : rowk[j] * pinv;
+                else if (j == k) v = -colk[i] * pinv;
+                else v = Mi[idx] - colk[i] * rowk[j] * pinv;
+                Mi[idx] = v;
+            }
+        }
+        __syncthreads();
+        for (int i = tid; i < npv * NCU * ND; i += nth) {
+            const int nn = i % npv, r = i / npv;
+            double s = 0;
+            for (int l = 0; l < npv; l++) s += Mi[nn + npv * l] * rhs[l + npv * r];
+            ue[NCU * npv + i] = s;
+        }
+        __syncthreads();
+    }
+#undef SVT
+#undef SVG
+#undef SFT
+#undef SFG
+#undef PERM
+#undef INVP
+}
+
+size_t getq_lds_bytes(const Dims &d)
+{
+    const size_t n = (size_t)d.npv * d.nd + (size_t)d.npv * d.ncu + (size_t)d.ndf * d.nch + d.ngv +
+                     (size_t)d.ngv * d.nd * d.nd + d.np + (size_t)d.np * d.nd + 2 * (size_t)d.npv * d.npv +
+                     2 * (size_t)d.npv + (size_t)d.ngv * d.ncu + (size_t)d.np * d.nch + (size_t)d.npv * d.ncu * d.nd;
+    return n * sizeof(double);
+}
+
+size_t flux_lds_bytes(const Dims &d)
+{
+    size_t n = (size_t)d.npv * d.nd + (size_t)d.npv * d.nc + (size_t)d.ndf * d.nch + (size_t)d.np * d.nd +
+               (size_t)d.ngv * d.nc + (size_t)d.np * d.nc + (size_t)d.np * d.nch;
+    return n * sizeof(double) + 16 * sizeof(int);
+}
+
 size_t assemble_lds_bytes(const Dims &d)
 {
     size_t n = 2 * (size_t)d.npv * d.ngv * d.nd1 + 2 * (size_t)d.npf * d.ngf * d.nd + (size_t)d.npv * d.nd +
                (size_t)d.npv * d.nc + (size_t)d.ndf * d.nch + (size_t)d.npv * d.ncu + d.ngv +
                (size_t)d.ngv * d.nd * d.nd + d.np + (size_t)d.np * d.nd + (size_t)d.npv * d.npv +
-               (size_t)d.npv * d.ngv * d.nd + (size_t)d.ndf * d.ngv * d.nd + (size_t)d.ngv * d.nc +
-               (size_t)d.np * d.nc + (size_t)d.np * d.nch + 256;
+               (size_t)(d.ngv + d.np) * d.npv + 2 * (size_t)d.npv + (size_t)d.npv * d.ngv * d.nd + (size_t)d.ndf * d.ngv * d.nd + (size_t)d.ngv * d.nc +
+               ASM_THREADS;
     size_t ints = (size_t)d.npf * d.nfe + (size_t)d.npv * d.nfe + d.nfe + 8;
     return n * sizeof(double) + ints * sizeof(int);
 }
@@ -551,17 +835,50 @@ bool assemble_supported(int nd, int npv, int npf)
 
 hipError_t launch_assemble(const Dims &d, const Master &m, const Phys &ph, const double *dgnodes, const double *UDG,
                            const double *UH, const double *SH, const int *elcon, const int *bf, int e0, int n,
-                           double *scratch, long sstride, int grid, int ldx, double *Ru, double *D, double *Fx,
-                           double *K, double *Hx, double *Rh, double *ru2, hipStream_t stream)
+                           double *scratch, long sstride, int grid, int ldx, double *fluxbuf, long fstride, int chunk,
+                           double *Ru, double *D, double *Fx, double *K, double *Hx, double *Rh, double *ru2,
+                           hipStream_t stream)
 {
-    size_t lds = assemble_lds_bytes(d);
-#define LAUNCH(a, b, c)                                                                                            \
-    if (d.nd == a && d.npv == b && d.npf == c) {                                                                   \
-        hipFuncSetAttribute((const void *)assemble_kernel<a, b, c>, hipFuncAttributeMaxDynamicSharedMemorySize,   \
-                            (int)lds);                                                                             \
-        hipLaunchKernelGGL((assemble_kernel<a, b, c>), dim3(grid), dim3(256), lds, stream, d, m, ph, dgnodes, UDG, \
-                           UH, SH, elcon, bf, e0, n, scratch, sstride, ldx, Ru, D, Fx, K, Hx, Rh, ru2);            \
-        return hipGetLastError();                                                                                  \
+    const size_t lds = assemble_lds_bytes(d), flds = flux_lds_bytes(d);
+    for (int c0 = e0; c0 < e0 + n; c0 += chunk) {
+        const int cn = std::min(chunk, e0 + n - c0);
+        const int fgrid = std::min(cn, 2048), agrid = std::min(cn, grid);
+        bool done = false;
+#define LAUNCH(a, b, c)                                                                                             \
+        if (!done && d.nd == a && d.npv == b && d.npf == c) {                                                       \
+            hipFuncSetAttribute((const void *)flux_kernel<a, b, c>, hipFuncAttributeMaxDynamicSharedMemorySize,    \
+                                (int)flds);                                                                         \
+            hipLaunchKernelGGL((flux_kernel<a, b, c>), dim3(fgrid), dim3(256), flds, stream, d, m, ph, dgnodes,     \
+                               UDG, UH, elcon, bf, c0, cn, fluxbuf, fstride);                                       \
+            hipFuncSetAttribute((const void *)assemble_kernel<a, b, c>, hipFuncAttributeMaxDynamicSharedMemorySize,\
+                                (int)lds);                                                                          \
+            hipLaunchKernelGGL((assemble_kernel<a, b, c>), dim3(agrid), dim3(ASM_THREADS), lds, stream, d, m, ph,   \
+                               dgnodes, UDG, UH, SH, elcon, bf, c0, cn, scratch, sstride, ldx, fluxbuf, fstride, Ru, \
+                               D, Fx, K, Hx, Rh, ru2);                                                              \
+            done = true;                                                                                            \
+        }
+        HFEM_FAMILIES(LAUNCH)
+#undef LAUNCH
+        if (!done) return hipErrorInvalidValue;
+        hipError_t err = hipGetLastError();
+        if (err != hipSuccess) return err;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_getq(const Dims &d, const Master &m, const Phys &ph, const double *dgnodes, double *UDG,
+                       const double *UH, const double *SH, const int *elcon, const int *bf, int e0, int n,
+                       hipStream_t stream)
+{
+    const size_t lds = getq_lds_bytes(d);
+    const int grid = std::min(n, 4096);
+#define LAUNCH(a, b, c)                                                                                             \
+    if (d.nd == a && d.npv == b && d.npf == c) {                                                                    \
+        hipFuncSetAttribute((const void *)getq_kernel<a, b, c>, hipFuncAttributeMaxDynamicSharedMemorySize,         \
+                            (int)lds);                                                                              \
+        hipLaunchKernelGGL((getq_kernel<a, b, c>), dim3(grid), dim3(256), lds, stream, d, m, ph, dgnodes, UDG, UH,  \
+                           SH, elcon, bf, e0, n);                                                                   \
+        return hipGetLastError();                                                                                   \
     }
     HFEM_FAMILIES(LAUNCH)
 #undef LAUNCH

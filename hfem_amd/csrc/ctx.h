@@ -1,0 +1,147 @@
+// Private definitions shared by the hfem_amd host sources (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/hfem_amd.h"
+#include "hfem_device.h"
+
+namespace hfem {
+hipError_t launch_assemble(const Dims &, const Master &, const Phys &, const double *, const double *, const double *,
+                           const double *, const int *, const int *, int, int, double *, long, int, int, double *, long,
+                           int, double *, double *, double *, double *, double *, double *, double *, hipStream_t);
+size_t flux_lds_bytes(const Dims &d);
+hipError_t launch_condense_fp64(int na, int nb, int ldx, int e0, int n, const double *D, double *Dinv_g,
+                                const double *Fx, double *X, const double *K, double *Hx, int *info, int grid,
+                                bool dinv_lds, hipStream_t s);
+hipError_t launch_condense_mixed(int na, int nb, int ldx, int e0, int n, const double *D, const double *Fx, double *X,
+                                 double *Rw, const double *K, double *Hx, int *info, int grid, int sweeps,
+                                 hipStream_t s);
+size_t condense_lds_bytes(int na, bool dinv_in_lds, int elt);
+size_t assemble_lds_bytes(const Dims &d);
+bool assemble_supported(int nd, int npv, int npf);
+int condense_dinv_ld(int na);
+hipError_t launch_bsr_gather(int, int, int, int, int, long, const int *, const int *, const int *, const double *,
+                             double *, hipStream_t);
+hipError_t launch_face_gather(int, int, int, int, int, long, const int *, const int *, const double *, double *,
+                              const double *, double *, hipStream_t);
+hipError_t launch_sumsq(long, const double *, int, double *, int, hipStream_t);
+hipError_t launch_bsr_matvec(int, int, const int *, const int *, const double *, const double *, double *,
+                             hipStream_t);
+hipError_t launch_getq(const Dims &, const Master &, const Phys &, const double *, double *, const double *,
+                       const double *, const int *, const int *, int, int, hipStream_t);
+size_t getq_lds_bytes(const Dims &d);
+// solver.hip
+size_t block_inverse_lds(int bsz);
+size_t bilu0_factor_lds(int bsz);
+hipError_t launch_block_inverse(int nrows, int bsz, const int *rowpts, const double *Hg, double *Dinv, int *info,
+                                hipStream_t s);
+hipError_t launch_mdf_weights(int nrows, int bsz, const int *rowpts, const double *Hg, const double *Dinv,
+                              double *Cw, hipStream_t s);
+hipError_t launch_bilu0_factor_level(int bsz, const int *rowpts, const int *colind, const int *u2o, const int *rows,
+                                     int nlev, double *Mg, int *info, hipStream_t s);
+hipError_t launch_bilu0_lsolve_level(int bsz, const int *rowpts, const int *colind, const int *u2o, const int *rows,
+                                     int nlev, const double *Mg, const double *x, double *y, hipStream_t s);
+hipError_t launch_bilu0_usolve_level(int bsz, const int *rowpts, const int *colind, const int *u2o, const int *rows,
+                                     int nlev, const double *Mg, const double *y, double *v, hipStream_t s);
+hipError_t launch_block_jacobi(int nrows, int bsz, const double *Dinv, const double *x, double *y, hipStream_t s);
+int multidot_blocks(long n);
+hipError_t launch_multidot(long n, int k, const double *V, long ldv, const double *w, double *partial, double *out,
+                           hipStream_t s);
+hipError_t launch_multi_axpy(long n, int k, const double *V, long ldv, const double *coef, double sign, double *w,
+                             hipStream_t s);
+hipError_t launch_axpby(long n, double a, const double *x, double b, double *y, hipStream_t s);
+hipError_t launch_fill(long n, double v, double *y, hipStream_t s);
+hipError_t launch_update_u(int ne, int npv, int nc, int nch, int ndf, int na, int nb, int ldx, const int *elcon,
+                           const double *X, const double *duh, double alpha, double *UDG, hipStream_t s);
+hipError_t launch_stage_source(long ne, int npv, int nc, int ncu, double a, const double *UDGi, double b,
+                               const double *UDG, double c, double *SH, hipStream_t s);
+}  // namespace hfem
+
+
+using namespace hfem;
+
+int hfem_fail(int code, const char *fmt, ...);
+
+#define HIPCHK(x)                                                                                     \
+    do {                                                                                              \
+        hipError_t e_ = (x);                                                                          \
+        if (e_ != hipSuccess) return hfem_fail(-2, "%s failed: %s", #x, hipGetErrorString(e_));       \
+    } while (0)
+
+// Device state of the trace-system solver: preconditioner blocks, ordering and
+// level schedules, Krylov basis.  Allocated on first use.
+struct SolverState {
+    int prec = -2;                      // -2 nothing factored, -1 none, 0 BILU0, 2 block-Jacobi
+    int order_method = -1;              // ordering behind o2u/u2o: 0 natural, 1 approximate MDF, 3 caller's
+    std::vector<int> rowpts, colind;    // host copy of the trace graph
+    std::vector<int> o2u, u2o;          // host copies
+    int *d_o2u = nullptr, *d_u2o = nullptr;
+    std::vector<int> lptr, uptr;        // level pointers of the L (forward) and U (backward) sweeps
+    int *d_lrows = nullptr, *d_urows = nullptr;  // rows of each level, concatenated
+    double *Mg = nullptr;               // BILU0 factors (nblocks blocks) or block-Jacobi inverses (nf blocks)
+    double *Dinv = nullptr;             // inverted diagonal blocks of Hg (MDF weights)
+    double *Cw = nullptr;               // MDF block weights ||inv(H_ii) H_ij||_F
+    int *d_info = nullptr;
+    double *x = nullptr;                // GMRES solution (trace update duh)
+    double *V = nullptr;                // Krylov basis, (restart+1) columns of length N
+    int vcols = 0;
+    double *b = nullptr, *t = nullptr;  // preconditioned rhs, work vector
+    double *dots = nullptr;             // partial sums of the multi-dot kernel
+    double *coef = nullptr;             // device copy of Gram-Schmidt coefficients
+    double *UDGi = nullptr;             // DIRK: solution at the start of the time step
+    // timings of the last calls (milliseconds; device events inside GMRES)
+    double ms_order = 0, ms_factor = 0, ms_gmres = 0, ms_matvec = 0, ms_prec = 0;
+    int n_matvec = 0, n_prec = 0;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+struct hfem_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hfem_dims hd{};
+    Dims d{};
+    Phys ph{};
+    bool has_master = false, has_phys = false, has_mesh = false, has_sol = false;
+    // master
+    double *shapvt = nullptr, *shapvg = nullptr, *shapft = nullptr, *shapfg = nullptr;
+    int *perm = nullptr, *invperm = nullptr;
+    // mesh
+    double *dgnodes = nullptr;
+    int *elcon = nullptr, *bf = nullptr, *rowstart = nullptr, *cols = nullptr, *einv = nullptr, *f2e = nullptr;
+    int *blkptr = nullptr, *blklist = nullptr;
+    int nblocks = 0;
+    // state
+    double *UDG = nullptr, *UH = nullptr, *SH = nullptr;
+    // element blocks: D (na x na), Fx and X (na x ldx, row-major, column nb = Ru / inv(D)Ru),
+    // K (nb x na), Hx (nb x (nb+1), column nb = Rh / FH), Ru, Rh (uncondensed)
+    double *D = nullptr, *Fx = nullptr, *X = nullptr, *K = nullptr, *Hx = nullptr, *Ru = nullptr, *Rh = nullptr,
+           *ru2 = nullptr;
+    int *info = nullptr;
+    int ldx = 0;
+    // work
+    double *scratch = nullptr, *dinv = nullptr, *rwork = nullptr, *fluxbuf = nullptr;
+    long fstride = 0;
+    int chunk = 0;
+    long sstride = 0;
+    int grid_asm = 0, grid_cond = 0;
+    bool dinv_lds = true;
+    int maxlds = 0;
+    // trace system
+    double *Hg = nullptr, *Rg = nullptr, *r = nullptr, *partial = nullptr, *xv = nullptr, *yv = nullptr;
+    float ms_asm = 0, ms_cond = 0;
+    // trace-system solver (solve.hip)
+    SolverState sv;
+};
+
+
+int hfem_reduce_sum(hfem_ctx *c, long n, const double *x, int square, double *out);
+
+template <class T>
+static inline hipError_t hfem_dalloc(T **p, size_t n)
+{
+    return hipMalloc((void **)p, (n > 0 ? n : 1) * sizeof(T));
+}

@@ -92,6 +92,17 @@ __device__ __forceinline__ void ns_flux(const double *param, const dual *w, dual
     }
 }
 
+// |lambda| with the smooth fix 0.5 (lambda^2 / (eps c) + eps c) below eps c (getan.c).
+__device__ __forceinline__ dual abs_eig(dual l, double eps, dual cs)
+{
+    dual a = dfabs(l);
+    if (eps > 0) {
+        dual ec = eps * cs;
+        if (a.v < ec.v) a = 0.5 * (a * a / ec + ec);
+    }
+    return a;
+}
+
 // Normal flux Jacobian A_n(uh) (absflag 0) or |A_n|(uh) (absflag 1) in the
 // eigen-split form of application/FM/getan.c: column k of M (M[k*nch+row]) is
 // lam3 e_k + cc1_k (1, v, h) + cc2_k (0, n, un).
@@ -111,16 +122,9 @@ __device__ __forceinline__ void ns_normal_jac(double gam, double eps, const dual
     dual c2 = gam * (p * ri), cs = dsqrt(c2);
     dual l1 = un + cs, l2 = un - cs, l3 = un;
     if (absflag) {
-        dual *ls[3] = {&l1, &l2, &l3};
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            dual a = dfabs(*ls[k]);
-            if (eps > 0) {
-                dual ec = eps * cs;
-                if (a.v < ec.v) a = 0.5 * (a * a / ec + ec);
-            }
-            *ls[k] = a;
-        }
+        l1 = abs_eig(l1, eps, cs);
+        l2 = abs_eig(l2, eps, cs);
+        l3 = abs_eig(l3, eps, cs);
     }
     dual s1 = 0.5 * (l1 + l2), s2 = 0.5 * (l1 - l2), s13 = s1 - l3;
 #pragma unroll
@@ -148,7 +152,7 @@ __device__ __forceinline__ void ns_normal_jac(double gam, double eps, const dual
 
 // Per-workgroup scratch offsets (doubles) used by the assembly kernel.
 struct Scratch {
-    long C, E, shapMiCf, shapMiEf, f, fu, fh, fhu, fhuh, fb, fbu, fbuh, wrl, total;
+    long C, E, shapMiCf, shapMiEf, wrl, total;
 };
 
 __host__ __device__ inline Scratch scratch_layout(const Dims &d)
@@ -160,6 +164,21 @@ __host__ __device__ inline Scratch scratch_layout(const Dims &d)
     s.E = take((long)d.nd * d.npv * d.ndf);
     s.shapMiCf = take((long)d.np * d.npv * d.nd);
     s.shapMiEf = take((long)d.np * d.ndf * d.nd);
+    s.wrl = take((long)d.ngv * d.nd1 * d.ncu * d.nc);
+    s.total = o;
+    return s;
+}
+
+// Per-element pointwise flux data written by the flux kernel, read by assembly.
+struct FluxLayout {
+    long f, fu, fh, fhu, fhuh, fb, fbu, fbuh, total;
+};
+
+__host__ __device__ inline FluxLayout flux_layout(const Dims &d)
+{
+    FluxLayout s;
+    long o = 0;
+    auto take = [&](long n) { long r = o; o += (n + 31) / 32 * 32; return r; };
     s.f = take((long)d.ngv * d.ncu * d.nd);
     s.fu = take((long)d.ngv * d.ncu * d.nd * d.nc);
     s.fh = take((long)d.np * d.nch);
@@ -168,7 +187,6 @@ __host__ __device__ inline Scratch scratch_layout(const Dims &d)
     s.fb = take((long)d.np * d.nch);
     s.fbu = take((long)d.np * d.nch * d.nc);
     s.fbuh = take((long)d.np * d.nch * d.nch);
-    s.wrl = take((long)d.ngv * d.nd1 * d.ncu * d.nc);
     s.total = o;
     return s;
 }

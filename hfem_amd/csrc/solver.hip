@@ -1,0 +1,547 @@
+// Trace-system solver kernels (gfx950): inverses of BSR diagonal blocks, MDF
+// block weights, BILU0 factorisation and its triangular sweeps one level of the
+// ordering at a time, block-Jacobi, the vector kernels of GMRES, the solution
+// update (getU) and DIRK stage sources.
+//
+// BSR blocks are bsz x bsz column-major, rows = faces, the diagonal block first in
+// each row (the reference's Hg layout, assembly/globalassembly.cpp:42-86).
+#include "hfem_device.h"
+
+#include <algorithm>
+
+namespace hfem {
+
+constexpr int SV_THREADS = 256;
+
+// In-place Gauss-Jordan inverse of an n x n column-major block in LDS with
+// partial pivoting: rows are interchanged as pivots are chosen and the columns of
+// the result unscrambled in reverse order at the end.  All threads of the
+// workgroup call it; piv (n ints), rowk/colk (n doubles each) and red (1 double)
+// are LDS.  Returns 0, or 1 + the step of the first zero pivot (workgroup-uniform).
+__device__ int lds_inverse(double *A, int n, int *piv, double *rowk, double *colk, double *red)
+{
+    const int tid = threadIdx.x, nth = blockDim.x;
+    for (int k = 0; k < n; k++) {
+        if (tid < 64) {
+            double best = -1.0;
+            int bi = k;
+            for (int i = k + tid; i < n; i += 64) {
+                const double v = fabs(A[i + n * k]);
+                if (v > best) { best = v; bi = i; }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ob = __shfl_down(best, off);
+                const int oi = __shfl_down(bi, off);
+                if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+            }
+            if (tid == 0) { piv[k] = bi; red[0] = best; }
+        }
+        __syncthreads();
+        if (red[0] == 0.0) return k + 1;
+        const int p = piv[k];
+        for (int j = tid; j < n; j += nth) {
+            double a = A[k + n * j], b = A[p + n * j];
+            rowk[j] = b;                         // pivot row after the interchange
+            if (p != k) { A[k + n * j] = b; A[p + n * j] = a; }
+        }
+        __syncthreads();
+        for (int i = tid; i < n; i += nth) colk[i] = A[i + n * k];
+        __syncthreads();
+        const double pinv = 1.0 / rowk[k];
+        for (int idx = tid; idx < n * n; idx += nth) {
+            const int i = idx % n, j = idx / n;
+            double v;
+            if (i == k) v = (j == k) ? pinv : rowk[j] * pinv;
+            else if (j == k) v = -colk[i] * pinv;
+            else v = A[idx] - colk[i] * (rowk[j] * pinv);
+            A[idx] = v;
+        }
+        __syncthreads();
+    }
+    for (int k = n - 1; k >= 0; k--) {
+        const int p = piv[k];
+        if (p != k)
+            for (int i = tid; i < n; i += nth) {
+                double t = A[i + n * k];
+                A[i + n * k] = A[i + n * p];
+                A[i + n * p] = t;
+            }
+        __syncthreads();
+    }
+    return 0;
+}
+
+struct LdsInv {
+    double *rowk, *colk, *red;
+    int *piv;
+};
+
+__device__ LdsInv lds_inv_area(double *p, int n)
+{
+    LdsInv w;
+    w.rowk = p;
+    w.colk = p + n;
+    w.red = p + 2 * n;
+    w.piv = (int *)(p + 2 * n + 2);
+    return w;
+}
+
+size_t lds_inv_bytes(int n) { return sizeof(double) * (2 * n + 2) + sizeof(int) * (n + 2); }
+
+// Inverses of the diagonal blocks of Hg (block-Jacobi, mkBlockJacobi.m:9-11, and
+// the MDF weights).  One workgroup per row.
+__global__ void __launch_bounds__(SV_THREADS)
+block_inverse_kernel(int nrows, int bsz, const int *__restrict__ rowpts, const double *__restrict__ Hg,
+                     double *__restrict__ Dinv, int *__restrict__ info)
+{
+    extern __shared__ double lds[];
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const long b2 = (long)bsz * bsz;
+    double *A = lds;
+    LdsInv w = lds_inv_area(A + b2, bsz);
+    for (int row = blockIdx.x; row < nrows; row += gridDim.x) {
+        const double *src = Hg + (long)rowpts[row] * b2;
+        for (long i = tid; i < b2; i += nth) A[i] = src[i];
+        __syncthreads();
+        const int r = lds_inverse(A, bsz, w.piv, w.rowk, w.colk, w.red);
+        for (long i = tid; i < b2; i += nth) Dinv[row * b2 + i] = A[i];
+        if (tid == 0 && info) info[row] = r;
+        __syncthreads();
+    }
+}
+
+// MDF block weights C(n) = ||inv(H_ii) H_n||_F for every block n of row i
+// (kernel/computeMDFordering.m:14-20).  One workgroup per row.
+__global__ void __launch_bounds__(SV_THREADS)
+mdf_weight_kernel(int nrows, int bsz, const int *__restrict__ rowpts, const double *__restrict__ Hg,
+                  const double *__restrict__ Dinv, double *__restrict__ Cw)
+{
+    extern __shared__ double lds[];
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const long b2 = (long)bsz * bsz;
+    double *A = lds, *B = lds + b2, *red = lds + 2 * b2;
+    for (int row = blockIdx.x; row < nrows; row += gridDim.x) {
+        for (long i = tid; i < b2; i += nth) A[i] = Dinv[row * b2 + i];
+        for (int n = rowpts[row]; n < rowpts[row + 1]; n++) {
+            for (long i = tid; i < b2; i += nth) B[i] = Hg[(long)n * b2 + i];
+            __syncthreads();
+            double s2 = 0.0;
+            for (long idx = tid; idx < b2; idx += nth) {
+                const int i = idx % bsz, j = idx / bsz;
+                double s = 0.0;
+                for (int k = 0; k < bsz; k++) s += A[i + bsz * k] * B[k + bsz * j];
+                s2 += s * s;
+            }
+            red[tid] = s2;
+            __syncthreads();
+            for (int st = nth / 2; st > 0; st >>= 1) {
+                if (tid < st) red[tid] += red[tid + st];
+                __syncthreads();
+            }
+            if (tid == 0) Cw[n] = sqrt(red[0]);
+            __syncthreads();
+        }
+    }
+}
+
+// BILU0 factorisation of the rows of one level (kernel/computeBILU0.m), in the
+// row-oriented form: row ri takes, for each neighbour rk numbered before it in
+// increasing order, L_ik = A_ik inv(U_kk), then A_ii -= L_ik A_ki and A_il -=
+// L_ik A_kl for the later common neighbours rl; finally A_ii is inverted.  Each
+// block of row ri receives the same updates in the same order as in the
+// reference's right-looking loop; rows of one level do not depend on each other.
+// The rows' own blocks are read and written with one fixed entry -> thread map,
+// so a thread only ever re-reads global entries it wrote itself.
+__global__ void __launch_bounds__(SV_THREADS)
+bilu0_factor_kernel(int bsz, const int *__restrict__ rowpts, const int *__restrict__ colind,
+                    const int *__restrict__ u2o, const int *__restrict__ rows, int nlev, double *__restrict__ Mg,
+                    int *__restrict__ info)
+{
+    extern __shared__ double lds[];
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const long b2 = (long)bsz * bsz;
+    double *X = lds, *B = lds + b2, *L = lds + 2 * b2;
+    LdsInv w = lds_inv_area(lds + 3 * b2, bsz);
+    int *lst = w.piv + bsz + 2;            // [0]: count, then pairs (block position, row)
+    for (int t = blockIdx.x; t < nlev; t += gridDim.x) {
+        const int ri = rows[t], oi = u2o[ri], d0 = rowpts[ri], d1 = rowpts[ri + 1];
+        if (tid == 0) {
+            int cnt = 0;
+            for (int k = d0 + 1; k < d1; k++) {
+                const int rk = colind[k];
+                if (u2o[rk] >= oi) continue;
+                int q = cnt++;                   // insertion by increasing order number
+                while (q > 0 && u2o[lst[1 + 2 * (q - 1) + 1]] > u2o[rk]) {
+                    lst[1 + 2 * q] = lst[1 + 2 * (q - 1)];
+                    lst[2 + 2 * q] = lst[2 + 2 * (q - 1)];
+                    q--;
+                }
+                lst[1 + 2 * q] = k;
+                lst[2 + 2 * q] = rk;
+            }
+            lst[0] = cnt;
+        }
+        __syncthreads();
+        const int cnt = lst[0];
+        for (int q = 0; q < cnt; q++) {
+            const int ik = lst[1 + 2 * q], rk = lst[2 + 2 * q];
+            const int k0 = rowpts[rk], k1 = rowpts[rk + 1];
+            for (long i = tid; i < b2; i += nth) {
+                X[i] = Mg[(long)ik * b2 + i];
+                B[i] = Mg[(long)k0 * b2 + i];     // inv(U_kk)
+            }
+            __syncthreads();
+            for (long idx = tid; idx < b2; idx += nth) {
+                const int i = idx % bsz, j = idx / bsz;
+                double s = 0.0;
+                for (int k = 0; k < bsz; k++) s += X[i + bsz * k] * B[k + bsz * j];
+                L[idx] = s;
+                Mg[(long)ik * b2 + idx] = s;
+            }
+            __syncthreads();
+            // targets in row ri: the diagonal (source: ri's position in row rk) and the
+            // later common neighbours rl (source: rl's position in row rk)
+            for (int l = d0; l < d1; l++) {
+                const int rl = colind[l];
+                if (l != d0 && u2o[rl] <= u2o[rk]) continue;
+                int km = -1;
+                for (int m = k0 + 1; m < k1; m++)
+                    if (colind[m] == rl) { km = m; break; }
+                if (km < 0) continue;
+                for (long i = tid; i < b2; i += nth) B[i] = Mg[(long)km * b2 + i];
+                __syncthreads();
+                for (long idx = tid; idx < b2; idx += nth) {
+                    const int i = idx % bsz, j = idx / bsz;
+                    double s = 0.0;
+                    for (int k = 0; k < bsz; k++) s += L[i + bsz * k] * B[k + bsz * j];
+                    Mg[(long)l * b2 + idx] -= s;
+                }
+                __syncthreads();
+            }
+        }
+        for (long i = tid; i < b2; i += nth) X[i] = Mg[(long)d0 * b2 + i];
+        __syncthreads();
+        const int r = lds_inverse(X, bsz, w.piv, w.rowk, w.colk, w.red);
+        for (long i = tid; i < b2; i += nth) Mg[(long)d0 * b2 + i] = X[i];
+        if (tid == 0) info[ri] = r;
+        __syncthreads();
+    }
+}
+
+// Forward sweep L y = x over the rows of one level (kernel/applyBILU0.m:12-27):
+// y_r = x_r - sum over earlier neighbours k of L_rk y_k.  One wave per row.
+__global__ void __launch_bounds__(SV_THREADS)
+bilu0_lsolve_kernel(int bsz, const int *__restrict__ rowpts, const int *__restrict__ colind,
+                    const int *__restrict__ u2o, const int *__restrict__ rows, int nlev, const double *__restrict__ Mg,
+                    const double *__restrict__ x, double *__restrict__ y)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, wpb = blockDim.x >> 6;
+    const long b2 = (long)bsz * bsz;
+    for (int t = blockIdx.x * wpb + wv; t < nlev; t += gridDim.x * wpb) {
+        const int r = rows[t], o = u2o[r];
+        for (int i = lane; i < bsz; i += 64) {
+            double s = x[(long)r * bsz + i];
+            for (int k = rowpts[r] + 1; k < rowpts[r + 1]; k++) {
+                const int c = colind[k];
+                if (u2o[c] >= o) continue;
+                const double *blk = Mg + (long)k * b2 + i, *yc = y + (long)c * bsz;
+                for (int cc = 0; cc < bsz; cc++) s -= blk[(long)bsz * cc] * yc[cc];
+            }
+            y[(long)r * bsz + i] = s;
+        }
+    }
+}
+
+// Backward sweep U v = y over the rows of one level (kernel/applyBILU0.m:29-48):
+// v_r = inv(U_rr) (y_r - sum over later neighbours k of U_rk v_k).  One wave per
+// row; every wave of the workgroup runs the same number of iterations so the
+// barriers around the LDS staging are uniform.
+__global__ void __launch_bounds__(SV_THREADS)
+bilu0_usolve_kernel(int bsz, const int *__restrict__ rowpts, const int *__restrict__ colind,
+                    const int *__restrict__ u2o, const int *__restrict__ rows, int nlev, const double *__restrict__ Mg,
+                    const double *__restrict__ y, double *__restrict__ v)
+{
+    extern __shared__ double lds[];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, wpb = blockDim.x >> 6;
+    const long b2 = (long)bsz * bsz;
+    double *tb = lds + (long)wv * bsz;
+    for (int base = blockIdx.x * wpb; base < nlev; base += gridDim.x * wpb) {
+        const int t = base + wv;
+        const bool valid = t < nlev;
+        const int r = valid ? rows[t] : 0;
+        if (valid) {
+            const int o = u2o[r];
+            for (int i = lane; i < bsz; i += 64) {
+                double s = y[(long)r * bsz + i];
+                for (int k = rowpts[r] + 1; k < rowpts[r + 1]; k++) {
+                    const int c = colind[k];
+                    if (u2o[c] <= o) continue;
+                    const double *blk = Mg + (long)k * b2 + i, *vc = v + (long)c * bsz;
+                    for (int cc = 0; cc < bsz; cc++) s -= blk[(long)bsz * cc] * vc[cc];
+                }
+                tb[i] = s;
+            }
+        }
+        __syncthreads();
+        if (valid) {
+            const double *Ui = Mg + (long)rowpts[r] * b2;
+            for (int i = lane; i < bsz; i += 64) {
+                double s = 0.0;
+                for (int cc = 0; cc < bsz; cc++) s += Ui[i + (long)bsz * cc] * tb[cc];
+                v[(long)r * bsz + i] = s;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Block-Jacobi application y_i = inv(H_ii) x_i.  One wave per row.
+__global__ void __launch_bounds__(SV_THREADS)
+block_jacobi_kernel(int nrows, int bsz, const double *__restrict__ Dinv, const double *__restrict__ x,
+                    double *__restrict__ y)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, wpb = blockDim.x >> 6;
+    const long b2 = (long)bsz * bsz;
+    for (int r = blockIdx.x * wpb + wv; r < nrows; r += gridDim.x * wpb) {
+        const double *Di = Dinv + (long)r * b2, *xr = x + (long)r * bsz;
+        for (int i = lane; i < bsz; i += 64) {
+            double s = 0.0;
+            for (int cc = 0; cc < bsz; cc++) s += Di[i + (long)bsz * cc] * xr[cc];
+            y[(long)r * bsz + i] = s;
+        }
+    }
+}
+
+// ---- GMRES vector kernels ----
+
+constexpr int MDOT_GROUP = 8;
+
+// partial[blk*k + i] = sum over the workgroup's slice of V_i . w  (V_i at V + i*ldv)
+__global__ void __launch_bounds__(SV_THREADS)
+multidot_kernel(long n, int k, const double *__restrict__ V, long ldv, const double *__restrict__ w,
+                double *__restrict__ partial)
+{
+    __shared__ double red[SV_THREADS * MDOT_GROUP];
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const long chunk = (n + gridDim.x - 1) / gridDim.x;
+    const long q0 = blockIdx.x * chunk, q1 = q0 + chunk < n ? q0 + chunk : n;
+    for (int ig = 0; ig < k; ig += MDOT_GROUP) {
+        const int g = k - ig < MDOT_GROUP ? k - ig : MDOT_GROUP;
+        double acc[MDOT_GROUP];
+#pragma unroll
+        for (int j = 0; j < MDOT_GROUP; j++) acc[j] = 0.0;
+        for (long q = q0 + tid; q < q1; q += nth) {
+            const double wq = w[q];
+#pragma unroll
+            for (int j = 0; j < MDOT_GROUP; j++)
+                if (j < g) acc[j] += V[(ig + j) * ldv + q] * wq;
+        }
+#pragma unroll
+        for (int j = 0; j < MDOT_GROUP; j++) red[j * SV_THREADS + tid] = acc[j];
+        __syncthreads();
+        for (int st = nth / 2; st > 0; st >>= 1) {
+            if (tid < st)
+#pragma unroll
+                for (int j = 0; j < MDOT_GROUP; j++) red[j * SV_THREADS + tid] += red[j * SV_THREADS + tid + st];
+            __syncthreads();
+        }
+        if (tid < g) partial[(long)blockIdx.x * k + ig + tid] = red[tid * SV_THREADS];
+        __syncthreads();
+    }
+}
+
+// out[i] = sum_b partial[b*k + i]
+__global__ void multidot_reduce_kernel(int nblk, int k, const double *__restrict__ partial, double *__restrict__ out)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int b = 0; b < nblk; b++) s += partial[(long)b * k + i];
+        out[i] = s;
+    }
+}
+
+// w += sign * sum_i coef[i] V_i
+__global__ void __launch_bounds__(SV_THREADS)
+multi_axpy_kernel(long n, int k, const double *__restrict__ V, long ldv, const double *__restrict__ coef,
+                  double sign, double *__restrict__ w)
+{
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int i = 0; i < k; i++) s += coef[i] * V[i * ldv + q];
+        w[q] += sign * s;
+    }
+}
+
+// y = a*x + b*y
+__global__ void axpby_kernel(long n, double a, const double *__restrict__ x, double b, double *__restrict__ y)
+{
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x)
+        y[q] = a * x[q] + b * y[q];
+}
+
+__global__ void fill_kernel(long n, double v, double *__restrict__ y)
+{
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) y[q] = v;
+}
+
+// ---- solution update (getU, solvers/solutionupdate.cpp:6-67) ----
+// u_e <- u_e - alpha inv(D)F duh_e + alpha inv(D)Ru, with X = [inv(D)F  inv(D)Ru]
+// row-major (na x ldx).  One workgroup per element.
+__global__ void __launch_bounds__(SV_THREADS)
+update_u_kernel(int ne, int npv, int nc, int nch, int ndf, int na, int nb, int ldx, const int *__restrict__ elcon,
+                const double *__restrict__ X, const double *__restrict__ duh, double alpha, double *__restrict__ UDG)
+{
+    extern __shared__ double de[];
+    const int tid = threadIdx.x, nth = blockDim.x;
+    for (int e = blockIdx.x; e < ne; e += gridDim.x) {
+        for (int q = tid; q < nb; q += nth) {
+            const int j = q / nch, k = q % nch;
+            de[q] = duh[(long)elcon[(long)e * ndf + j] * nch + k];
+        }
+        __syncthreads();
+        const double *Xe = X + (long)e * na * ldx;
+        for (int i = tid; i < na; i += nth) {
+            const double *row = Xe + (long)i * ldx;
+            double s = 0.0;
+            for (int c = 0; c < nb; c++) s += row[c] * de[c];
+            double *u = UDG + (long)e * npv * nc + i;
+            *u = (*u - alpha * s) + alpha * row[nb];
+        }
+        __syncthreads();
+    }
+}
+
+// DIRK stage source (solvers/dgsolvers.cpp:316-349): SH = a*UDGi + b*UDG + c*SH on the
+// u components, 0 on the q components.
+__global__ void stage_source_kernel(long ne, int npv, int nc, int ncu, double a, const double *__restrict__ UDGi,
+                                    double b, const double *__restrict__ UDG, double c, double *__restrict__ SH)
+{
+    const long n = ne * npv * nc;
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const int comp = (int)((q / npv) % nc);
+        SH[q] = comp < ncu ? (a * UDGi[q] + b * UDG[q]) + c * SH[q] : 0.0;
+    }
+}
+
+// ---- launchers ----
+
+static int grid_for(long n, int per = SV_THREADS)
+{
+    long g = (n + per - 1) / per;
+    return (int)(g < 1 ? 1 : (g > 4096 ? 4096 : g));
+}
+
+static hipError_t set_lds(const void *fn, size_t bytes)
+{
+    return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+size_t block_inverse_lds(int bsz) { return sizeof(double) * bsz * bsz + lds_inv_bytes(bsz); }
+size_t bilu0_factor_lds(int bsz) { return sizeof(double) * 3 * bsz * bsz + lds_inv_bytes(bsz) + sizeof(int) * 80; }
+
+hipError_t launch_block_inverse(int nrows, int bsz, const int *rowpts, const double *Hg, double *Dinv, int *info,
+                                hipStream_t s)
+{
+    const size_t lds = block_inverse_lds(bsz);
+    hipError_t e = set_lds((const void *)block_inverse_kernel, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(block_inverse_kernel, dim3(std::min(nrows, 8192)), dim3(SV_THREADS), lds, s, nrows, bsz,
+                       rowpts, Hg, Dinv, info);
+    return hipGetLastError();
+}
+
+hipError_t launch_mdf_weights(int nrows, int bsz, const int *rowpts, const double *Hg, const double *Dinv,
+                              double *Cw, hipStream_t s)
+{
+    const size_t lds = sizeof(double) * (2 * bsz * bsz + SV_THREADS);
+    hipError_t e = set_lds((const void *)mdf_weight_kernel, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(mdf_weight_kernel, dim3(std::min(nrows, 8192)), dim3(SV_THREADS), lds, s, nrows, bsz, rowpts,
+                       Hg, Dinv, Cw);
+    return hipGetLastError();
+}
+
+hipError_t launch_bilu0_factor_level(int bsz, const int *rowpts, const int *colind, const int *u2o, const int *rows,
+                                     int nlev, double *Mg, int *info, hipStream_t s)
+{
+    const size_t lds = bilu0_factor_lds(bsz);
+    hipError_t e = set_lds((const void *)bilu0_factor_kernel, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(bilu0_factor_kernel, dim3(std::min(nlev, 8192)), dim3(SV_THREADS), lds, s, bsz, rowpts, colind,
+                       u2o, rows, nlev, Mg, info);
+    return hipGetLastError();
+}
+
+hipError_t launch_bilu0_lsolve_level(int bsz, const int *rowpts, const int *colind, const int *u2o, const int *rows,
+                                     int nlev, const double *Mg, const double *x, double *y, hipStream_t s)
+{
+    const int wpb = SV_THREADS / 64;
+    hipLaunchKernelGGL(bilu0_lsolve_kernel, dim3(std::min((nlev + wpb - 1) / wpb, 8192)), dim3(SV_THREADS), 0, s, bsz,
+                       rowpts, colind, u2o, rows, nlev, Mg, x, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_bilu0_usolve_level(int bsz, const int *rowpts, const int *colind, const int *u2o, const int *rows,
+                                     int nlev, const double *Mg, const double *y, double *v, hipStream_t s)
+{
+    const int wpb = SV_THREADS / 64;
+    hipLaunchKernelGGL(bilu0_usolve_kernel, dim3(std::min((nlev + wpb - 1) / wpb, 8192)), dim3(SV_THREADS),
+                       sizeof(double) * wpb * bsz, s, bsz, rowpts, colind, u2o, rows, nlev, Mg, y, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_block_jacobi(int nrows, int bsz, const double *Dinv, const double *x, double *y, hipStream_t s)
+{
+    const int wpb = SV_THREADS / 64;
+    hipLaunchKernelGGL(block_jacobi_kernel, dim3(std::min((nrows + wpb - 1) / wpb, 8192)), dim3(SV_THREADS), 0, s,
+                       nrows, bsz, Dinv, x, y);
+    return hipGetLastError();
+}
+
+int multidot_blocks(long n) { return (int)std::min<long>(512, std::max<long>(1, n / 2048)); }
+
+hipError_t launch_multidot(long n, int k, const double *V, long ldv, const double *w, double *partial, double *out,
+                           hipStream_t s)
+{
+    const int nblk = multidot_blocks(n);
+    hipLaunchKernelGGL(multidot_kernel, dim3(nblk), dim3(SV_THREADS), 0, s, n, k, V, ldv, w, partial);
+    hipLaunchKernelGGL(multidot_reduce_kernel, dim3(1), dim3(128), 0, s, nblk, k, partial, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_multi_axpy(long n, int k, const double *V, long ldv, const double *coef, double sign, double *w,
+                             hipStream_t s)
+{
+    hipLaunchKernelGGL(multi_axpy_kernel, dim3(grid_for(n)), dim3(SV_THREADS), 0, s, n, k, V, ldv, coef, sign, w);
+    return hipGetLastError();
+}
+
+hipError_t launch_axpby(long n, double a, const double *x, double b, double *y, hipStream_t s)
+{
+    hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(n)), dim3(SV_THREADS), 0, s, n, a, x, b, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(long n, double v, double *y, hipStream_t s)
+{
+    hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n)), dim3(SV_THREADS), 0, s, n, v, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_update_u(int ne, int npv, int nc, int nch, int ndf, int na, int nb, int ldx, const int *elcon,
+                           const double *X, const double *duh, double alpha, double *UDG, hipStream_t s)
+{
+    hipLaunchKernelGGL(update_u_kernel, dim3(std::min(ne, 8192)), dim3(SV_THREADS), sizeof(double) * nb, s, ne, npv,
+                       nc, nch, ndf, na, nb, ldx, elcon, X, duh, alpha, UDG);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_source(long ne, int npv, int nc, int ncu, double a, const double *UDGi, double b,
+                               const double *UDG, double c, double *SH, hipStream_t s)
+{
+    hipLaunchKernelGGL(stage_source_kernel, dim3(grid_for(ne * npv * nc)), dim3(SV_THREADS), 0, s, ne, npv, nc, ncu,
+                       a, UDGi, b, UDG, c, SH);
+    return hipGetLastError();
+}
+
+}  // namespace hfem

@@ -25,6 +25,10 @@ _vp = ctypes.c_void_p
 FP64 = 0
 MIXED = 1
 
+PREC_NONE, PREC_BILU0, PREC_BLOCK_JACOBI = -1, 0, 2
+ORDER_NATURAL, ORDER_MDF = 0, 1
+MGS, ICGS = 0, 1
+
 
 class HfemError(RuntimeError):
     pass
@@ -50,10 +54,38 @@ EXPORTS = {
     "hfem_assemble_linear_system": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "hfem_read_system": (ctypes.c_int, [_vp, _dp, _dp]),
     "hfem_bsr_matvec": (ctypes.c_int, [_vp, _dp, _dp]),
+    "hfem_compute_ordering": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "hfem_set_ordering": (ctypes.c_int, [_vp, _ip]),
+    "hfem_get_ordering": (ctypes.c_int, [_vp, _ip, _ip]),
+    "hfem_precond_factor": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "hfem_precond_apply": (ctypes.c_int, [_vp, _dp, _dp]),
+    "hfem_gmres": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _ip, _dp, _ip]),
+    "hfem_read_trace_solution": (ctypes.c_int, [_vp, _dp]),
+    "hfem_update_solution": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "hfem_get_q": (ctypes.c_int, [_vp]),
+    "hfem_residual_norm": (ctypes.c_int, [_vp, _dp]),
+    "hfem_solve_steady": (ctypes.c_int, [_vp, ctypes.c_void_p, _ip, _ip, _dp]),
+    "hfem_dirk_step": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, _ip, _ip]),
+    "hfem_solver_stats": (ctypes.c_int, [_vp, _dp]),
     "hfem_last_kernel_ms": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
     "hfem_sync": (ctypes.c_int, [_vp]),
 }
 EXTRA_EXPORTS = {}
+
+
+class hfem_solver_params(ctypes.Structure):
+    """Newton / GMRES parameters (include/hfem_amd.h); defaults follow the reference's
+    MATLAB-side defaults (preprocessing/digasopre.m:424-438) with GMRES restart 30."""
+    _fields_ = [("newton_tol", ctypes.c_double), ("newton_maxiter", ctypes.c_int), ("precond", ctypes.c_int),
+                ("ordering", ctypes.c_int), ("reuse_ordering", ctypes.c_int), ("restart", ctypes.c_int),
+                ("gmres_tol", ctypes.c_double), ("gmres_maxit", ctypes.c_int), ("orthog", ctypes.c_int),
+                ("precision", ctypes.c_int)]
+
+    @classmethod
+    def make(cls, newton_tol=1e-7, newton_maxiter=10, precond=0, ordering=1, reuse_ordering=1, restart=30,
+             gmres_tol=1e-6, gmres_maxit=1000, orthog=1, precision=0):
+        return cls(newton_tol, newton_maxiter, precond, ordering, reuse_ordering, restart, gmres_tol, gmres_maxit,
+                   orthog, precision)
 
 
 def load():
@@ -199,3 +231,73 @@ class HDGContext:
         y = np.zeros_like(x)
         _check(load().hfem_bsr_matvec(self.h, _d(x), _d(y)))
         return y
+
+    # -- trace-system solver (computeOrdering / computePreconditioner / gmres) ---
+    def compute_ordering(self, method=ORDER_MDF):
+        _check(load().hfem_compute_ordering(self.h, method))
+
+    def set_ordering(self, o2u):
+        _check(load().hfem_set_ordering(self.h, _i(_i32(o2u))))
+
+    def get_ordering(self):
+        o2u = np.zeros(self.prob.mesh.nf, dtype=np.int32)
+        nlev = np.zeros(2, dtype=np.int32)
+        _check(load().hfem_get_ordering(self.h, _i(o2u), _i(nlev)))
+        return o2u, tuple(int(v) for v in nlev)
+
+    def compute_preconditioner(self, kind=PREC_BILU0):
+        _check(load().hfem_precond_factor(self.h, kind))
+
+    def apply_preconditioner(self, x):
+        x = _f64(x)
+        y = np.zeros_like(x)
+        _check(load().hfem_precond_apply(self.h, _d(x), _d(y)))
+        return y
+
+    def gmres(self, restart=30, tol=1e-6, maxit=1000, orthog=ICGS):
+        """Returns (flag, iterations, relative residual history); x via trace_solution()."""
+        it = ctypes.c_int()
+        fl = ctypes.c_int()
+        rev = np.zeros(maxit)
+        _check(load().hfem_gmres(self.h, restart, tol, maxit, orthog, ctypes.byref(it), _d(rev), ctypes.byref(fl)))
+        return fl.value, it.value, rev[:it.value]
+
+    def trace_solution(self):
+        x = np.zeros(self.prob.UH.size)
+        _check(load().hfem_read_trace_solution(self.h, _d(x)))
+        return x
+
+    def update_solution(self, alpha=1.0):
+        _check(load().hfem_update_solution(self.h, alpha))
+
+    def get_q(self):
+        _check(load().hfem_get_q(self.h))
+
+    def residual_norm(self):
+        r = ctypes.c_double()
+        _check(load().hfem_residual_norm(self.h, ctypes.byref(r)))
+        return r.value
+
+    def solver_stats(self):
+        v = np.zeros(7)
+        _check(load().hfem_solver_stats(self.h, _d(v)))
+        return dict(order_ms=v[0], factor_ms=v[1], gmres_ms=v[2], matvec_ms=v[3], matvecs=int(v[4]),
+                    precond_ms=v[5], preconds=int(v[6]))
+
+    # -- drivers (solveNonlinearProblem, solveUnsteadyProblem) -------------------
+    def solve_steady(self, params=None, **kw):
+        """Newton with line search; returns (converged, iterations, history[it, 4])."""
+        p = params or hfem_solver_params.make(**kw)
+        conv = ctypes.c_int()
+        nit = ctypes.c_int()
+        hist = np.zeros(4 * p.newton_maxiter)
+        _check(load().hfem_solve_steady(self.h, ctypes.byref(p), ctypes.byref(conv), ctypes.byref(nit), _d(hist)))
+        return bool(conv.value), nit.value, hist.reshape(-1, 4)[:nit.value]
+
+    def dirk_step(self, dt, nstage=3, order=3, params=None, **kw):
+        p = params or hfem_solver_params.make(**kw)
+        conv = ctypes.c_int()
+        nit = ctypes.c_int()
+        _check(load().hfem_dirk_step(self.h, dt, nstage, order, ctypes.byref(p), ctypes.byref(conv),
+                                     ctypes.byref(nit)))
+        return bool(conv.value), nit.value

@@ -27,13 +27,17 @@ from .master import make_master
 __all__ = ["Problem", "make_problem", "CONFIGS", "freestream_state"]
 
 
-def freestream_state(nd, gamma, minf, angle=0.0):
-    """rho = 1, |v| = 1 at `angle` (radians) from x in the x-y plane, p = 1/(gamma Minf^2)."""
+def freestream_state(nd, gamma, minf, angle=0.0, elevation=0.0):
+    """rho = 1, |v| = 1 at `angle` (radians) from x in the x-y plane and `elevation`
+    out of it (3D), p = 1/(gamma Minf^2)."""
     pinf = 1.0 / (gamma * minf ** 2)
     u = np.zeros(nd + 2)
     u[0] = 1.0
-    u[1] = np.cos(angle)
-    u[2] += np.sin(angle)
+    ce = np.cos(elevation) if nd == 3 else 1.0
+    u[1] = np.cos(angle) * ce
+    u[2] += np.sin(angle) * ce
+    if nd == 3:
+        u[3] += np.sin(elevation)
     u[nd + 1] = 0.5 + pinf / (gamma - 1.0)
     return u
 
@@ -93,7 +97,7 @@ def _uhat_average(mesh, UDG, ncu):
 
 
 def make_problem(kind, ncell, porder, seed=0, tau=2.0, re=1000.0, pr=0.72, minf=0.3, gamma=1.4,
-                 amp=1e-2, qamp=1e-2, jitter=0.05, angle=0.0, name=None):
+                 amp=1e-2, qamp=1e-2, jitter=0.05, angle=0.0, elevation=0.0, name=None):
     """Build a seeded case.  kind: 'tri' (2D), 'tet' (3D), 'hex' (3D periodic).
 
     ``angle`` turns the freestream in the x-y plane; at angle 0 the flow is
@@ -112,7 +116,7 @@ def make_problem(kind, ncell, porder, seed=0, tau=2.0, re=1000.0, pr=0.72, minf=
     nc = ncu * (nd + 1)
     m = make_master(nd, msh.elemtype, porder)
     rng = np.random.default_rng(seed + 7919)
-    u0 = freestream_state(nd, gamma, minf, angle)
+    u0 = freestream_state(nd, gamma, minf, angle, elevation)
     x = msh.dgnodes                                   # (ne, npv, nd)
     ne = msh.ne
     UDG = np.zeros((ne, nc, m.npv))

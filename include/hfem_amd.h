@@ -103,6 +103,71 @@ int hfem_read_system(hfem_ctx *ctx, double *Hg, double *Rg);
 /* y = Hg x with host vectors of length nf*bsz. */
 int hfem_bsr_matvec(hfem_ctx *ctx, const double *x, double *y);
 
+/* ---- trace-system solver --------------------------------------------------
+ * The reference binds these from closed libraries (solvers/dgsolvers.cpp:16-20:
+ * computeOrdering, computePreconditioner, gmres); their semantics follow its
+ * MATLAB specifications.  They act on the Hg, Rg of the last
+ * hfem_assemble_linear_system. */
+
+/* computeOrdering: method 0 natural, 1 approximate minimum discarded fill
+ * (kernel/computeMDFordering.m:1-150; reorderMethod 1). */
+int hfem_compute_ordering(hfem_ctx *ctx, int method);
+/* Use the caller's ordering: o2u[l] = face pivoted at step l. */
+int hfem_set_ordering(hfem_ctx *ctx, const int *o2u);
+/* Current ordering and the number of levels of the forward / backward sweeps
+ * (nlevels[2]); either pointer may be NULL. */
+int hfem_get_ordering(hfem_ctx *ctx, int *o2u, int *nlevels);
+/* computePreconditioner: kind -1 none, 0 BILU0 in the current ordering
+ * (kernel/computeBILU0.m:1-46), 2 block-Jacobi (kernel/mkBlockJacobi.m:1-16). */
+int hfem_precond_factor(hfem_ctx *ctx, int kind);
+/* y = M^{-1} x with host vectors of length nf*bsz (kernel/applyBILU0.m:1-52). */
+int hfem_precond_apply(hfem_ctx *ctx, const double *x, double *y);
+/* gmres: left-preconditioned restarted GMRES on Hg x = Rg from x = 0
+ * (kernel/gmres_bilu0.m:1-105); orthog 0 modified, 1 iterated classical
+ * Gram-Schmidt (orthogMethod).  rev (maxit doubles, may be NULL) receives the
+ * relative preconditioned residual of every iteration; flag 0 converged,
+ * 1 iteration limit.  x stays on the device (hfem_read_trace_solution). */
+int hfem_gmres(hfem_ctx *ctx, int restart, double tol, int maxit, int orthog, int *iters, double *rev, int *flag);
+int hfem_read_trace_solution(hfem_ctx *ctx, double *x);
+
+/* updateSol (solvers/solutionupdate.cpp:69-98): UH += alpha x, getU on every
+ * element with the inv(D)F, inv(D)Ru of the last condensation, then getQ. */
+int hfem_update_solution(hfem_ctx *ctx, double alpha);
+/* getQ on every element (assembly/elementassembly.cpp:1414-1479). */
+int hfem_get_q(hfem_ctx *ctx);
+/* computeResidualNorm (assembly/globalassembly.cpp:618-678). */
+int hfem_residual_norm(hfem_ctx *ctx, double *rNorm);
+
+/* Newton / GMRES parameters (sysstruct flags, preprocessing/digasopre.h:749-778). */
+typedef struct {
+    double newton_tol;   /* NewtonTol */
+    int newton_maxiter;  /* NewtonMaxiter */
+    int precond;         /* -1 none, 0 BILU0, 2 block-Jacobi */
+    int ordering;        /* 0 natural, 1 approximate MDF */
+    int reuse_ordering;  /* 1: keep an existing ordering (reuseOrdering) */
+    int restart;         /* GMRES restart */
+    double gmres_tol;    /* GMREStol */
+    int gmres_maxit;
+    int orthog;          /* 0 MGS, 1 ICGS */
+    int precision;       /* condensation precision, HFEM_FP64 or HFEM_MIXED */
+} hfem_solver_params;
+
+/* solveNonlinearProblem (solvers/dgsolvers.cpp:142-205): Newton with the
+ * reference's halving line search.  history (4 doubles per iteration, may be
+ * NULL): residual before, residual after, GMRES iterations, accumulated step. */
+int hfem_solve_steady(hfem_ctx *ctx, const hfem_solver_params *p, int *converged, int *niter, double *history);
+/* One time step of solveUnsteadyProblem (solvers/dgsolvers.cpp:221-444) with
+ * DIRK(nstage, order) of solvers/DIRKcoeff.cpp; each stage is a steady solve
+ * with fc_u = d_ii/dt and the stage source SH. */
+int hfem_dirk_step(hfem_ctx *ctx, double dt, int nstage, int order, const hfem_solver_params *p, int *converged,
+                   int *newton_iters);
+
+/* Timings of the last solver calls, like the breakdown solveLinearProblem prints
+ * (solvers/dgsolvers.cpp:107-113): out[7] = {ordering ms, factorisation ms,
+ * GMRES ms, matvec ms (device), matvec count, preconditioner ms (device),
+ * preconditioner count}. */
+int hfem_solver_stats(hfem_ctx *ctx, double *out);
+
 /* Milliseconds spent in the last hfem_assemble / hfem_condense launch (HIP events). */
 int hfem_last_kernel_ms(hfem_ctx *ctx, float *assemble_ms, float *condense_ms);
 /* Synchronise the context stream. */

@@ -39,10 +39,16 @@ def update_solution(prob, UDG, UH, SH, blocks, duh, alpha):
     return UDG, UH
 
 
-def linear_solve(prob, M, Rg, prec="bilu0", restart=30, gmres_tol=1e-10, maxit=1000, orthog=1, exact_mdf=False):
-    """computeOrdering + computePreconditioner + gmres (solvers/dgsolvers.cpp:79-105)."""
+def linear_solve(prob, M, Rg, prec="bilu0", restart=30, gmres_tol=1e-10, maxit=1000, orthog=1, exact_mdf=False,
+                 order=None):
+    """computeOrdering + computePreconditioner + gmres (solvers/dgsolvers.cpp:79-105).
+    ``order`` is a dict caching the ordering (reuseOrdering): computed when empty."""
     if prec == "bilu0":
-        o2u, u2o = O.mdf_order(M, exact=exact_mdf)
+        if order is None:
+            order = {}
+        if "o2u" not in order:
+            order["o2u"], order["u2o"] = O.mdf_order(M, exact=exact_mdf)
+        o2u, u2o = order["o2u"], order["u2o"]
         Mg, _ = O.bilu0_factor(M, o2u, u2o)
         return O.gmres(M, Rg, O.PREC_BILU0, Mg, o2u, u2o, restart=restart, tol=gmres_tol, maxit=maxit, orthog=orthog)
     if prec == "bj":
@@ -51,8 +57,11 @@ def linear_solve(prob, M, Rg, prec="bilu0", restart=30, gmres_tol=1e-10, maxit=1
     return O.gmres(M, Rg, O.PREC_NONE, restart=restart, tol=gmres_tol, maxit=maxit, orthog=orthog)
 
 
-def newton_solve(prob, UDG, UH, SH, newton_tol=1e-7, maxiter=10, alpha_min=1e-4, **lin):
-    """Returns (UDG, UH, converged, history); history holds one dict per Newton iteration."""
+def newton_solve(prob, UDG, UH, SH, newton_tol=1e-7, maxiter=10, alpha_min=1e-4, order=None, **lin):
+    """Returns (UDG, UH, converged, history); history holds one dict per Newton iteration.
+    The ordering is computed at the first iteration and reused (reuseOrdering,
+    solvers/dgsolvers.cpp:87-92); pass ``order`` to share it between solves."""
+    order = {} if order is None else order
     UDG = np.array(UDG, dtype=float)
     UH = np.array(UH, dtype=float)
     rnorm = newton_tol + 1.0
@@ -63,7 +72,7 @@ def newton_solve(prob, UDG, UH, SH, newton_tol=1e-7, maxiter=10, alpha_min=1e-4,
         M, Rg, old, blocks = O.trace_system(prob, UDG=UDG, UH=UH, SH=SH)
         if not np.isfinite(old):
             raise FloatingPointError("residual norm is NaN or Inf")
-        x, flag, iters, rev = linear_solve(prob, M, Rg, **lin)
+        x, flag, iters, rev = linear_solve(prob, M, Rg, order=order, **lin)
         rec = dict(old=old, gmres_iters=iters, gmres_flag=flag, alphas=[])
         history.append(rec)
         if it == 1 and old < newton_tol:
@@ -132,13 +141,14 @@ def dirk_step(prob, UDG, UH, SH, dt, nstage=3, order=3, **newton):
     ncu = prob.ncu
     UDGi = np.array(UDG, dtype=float)
     hist = []
+    ordering = newton.pop("ordering_cache", {})
     for s in range(nstage):
         fc_u = d[s * nstage + s] / dt
         SH = stage_source(s, nstage, d, dt, UDGi, UDG, SH)
         SH = np.array(SH)
         SH[:, ncu:, :] = 0.0                       # no q source for non-wave problems (:344-349)
         ps = dataclasses.replace(prob, fc_u=fc_u, fc_q=1.0, tdep=1)
-        UDG, UH, ok, h = newton_solve(ps, UDG, UH, SH, **newton)
+        UDG, UH, ok, h = newton_solve(ps, UDG, UH, SH, order=ordering, **newton)
         if not ok:
             raise RuntimeError(f"DIRK stage {s + 1} did not converge")
         hist.append(h)

@@ -1,0 +1,161 @@
+"""HIP trace-system solver and Newton / DIRK drivers vs the CPU oracle (GPU).
+
+The oracle restates the reference's MATLAB specifications of the closed solver
+pieces (oracle/solver_oracle.c) and its C++ drivers (oracle/newton_oracle.py);
+those are pinned by the properties in tests/test_solver_oracle.py.
+Tolerances: FP64 throughout; orderings and GMRES iteration counts must agree
+exactly, vectors to 1e-10 relative (different summation orders), Newton
+residual histories to 1e-8 relative above the 1e-12 level.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from hfem_amd.problem import make_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+CASES = [("tri", 3, 2), ("tet", 1, 2)]
+
+
+@pytest.fixture(params=CASES, ids=lambda c: f"{c[0]}{c[1]}p{c[2]}")
+def system(request, oracle_mod):
+    from hfem_amd.lib import HDGContext
+    kind, nc, p = request.param
+    prob = make_problem(kind, nc, p, seed=11, angle=0.3, elevation=0.2)
+    say(f"case {kind}{nc} p{p}: nf={prob.mesh.nf}")
+    ctx = HDGContext(prob)
+    ctx.assemble_linear_system()
+    Hg, Rg = ctx.read_system()
+    M = oracle_mod.BSR.from_problem(prob, Hg)
+    yield prob, ctx, M, Rg
+    ctx.close()
+
+
+def test_get_q_matches_oracle(oracle_mod):
+    from hfem_amd.lib import HDGContext
+    for kind, nc, p in [("tri", 2, 3), ("tet", 1, 3)]:
+        prob = make_problem(kind, nc, p, seed=4)
+        say(f"getq {kind}{nc} p{p}")
+        ctx = HDGContext(prob)
+        ctx.get_q()
+        U, _ = ctx.get_solution()
+        ref = oracle_mod.Oracle(prob).getq(prob.UDG)
+        assert rel(U, ref) < 1e-12
+        ctx.close()
+
+
+def test_mdf_ordering_matches_oracle(oracle_mod, system):
+    prob, ctx, M, Rg = system
+    ctx.compute_ordering(1)
+    o2u, nlev = ctx.get_ordering()
+    ref, _ = oracle_mod.mdf_order(M)
+    say(f"  levels {nlev}")
+    assert np.array_equal(o2u, ref)
+
+
+@pytest.mark.parametrize("kind", ["bilu0", "bj"])
+def test_preconditioner_matches_oracle(oracle_mod, system, kind):
+    from hfem_amd.lib import PREC_BILU0, PREC_BLOCK_JACOBI
+    O = oracle_mod
+    prob, ctx, M, Rg = system
+    x = np.random.default_rng(3).standard_normal(len(Rg))
+    if kind == "bilu0":
+        o2u, u2o = O.mdf_order(M)
+        ctx.set_ordering(o2u)
+        ctx.compute_preconditioner(PREC_BILU0)
+        Mg, _ = O.bilu0_factor(M, o2u, u2o)
+        want = O.bilu0_apply(M, o2u, u2o, Mg, x)
+    else:
+        ctx.compute_preconditioner(PREC_BLOCK_JACOBI)
+        Dinv, _ = O.block_jacobi_factor(M)
+        blocks = Dinv.reshape(-1, M.bsz, M.bsz).transpose(0, 2, 1)
+        want = np.concatenate([blocks[i] @ x[i * M.bsz:(i + 1) * M.bsz] for i in range(M.nrows)])
+    got = ctx.apply_preconditioner(x)
+    say(f"  {kind} apply rel err {rel(got, want):.2e}")
+    assert rel(got, want) < 1e-10
+
+
+@pytest.mark.parametrize("orthog", [0, 1])
+def test_gmres_matches_oracle(oracle_mod, system, orthog):
+    from hfem_amd.lib import PREC_BILU0
+    O = oracle_mod
+    prob, ctx, M, Rg = system
+    o2u, u2o = O.mdf_order(M)
+    ctx.set_ordering(o2u)
+    ctx.compute_preconditioner(PREC_BILU0)
+    flag, it, rev = ctx.gmres(restart=10, tol=1e-10, maxit=500, orthog=orthog)
+    x = ctx.trace_solution()
+    Mg, _ = O.bilu0_factor(M, o2u, u2o)
+    xr, fr, itr, revr = O.gmres(M, Rg, O.PREC_BILU0, Mg, o2u, u2o, restart=10, tol=1e-10, maxit=500, orthog=orthog)
+    say(f"  gmres orthog={orthog}: gpu {it} it flag {flag}, oracle {itr} it flag {fr}, x rel {rel(x, xr):.2e}")
+    assert flag == fr == 0 and it == itr
+    np.testing.assert_allclose(rev, revr, rtol=1e-6, atol=1e-12)
+    assert rel(x, xr) < 1e-8
+    assert rel(M.matvec(x), Rg) < 1e-6
+
+
+def test_update_and_residual_match_oracle(oracle_mod, system):
+    import newton_oracle as N
+    from hfem_amd.lib import PREC_BILU0
+    O = oracle_mod
+    prob, ctx, M, Rg = system
+    ctx.compute_ordering(1)
+    ctx.compute_preconditioner(PREC_BILU0)
+    ctx.gmres(restart=30, tol=1e-12, maxit=500)
+    x = ctx.trace_solution()
+    ctx.update_solution(0.5)
+    U, UH = ctx.get_solution()
+    _, _, _, blocks = O.trace_system(prob)
+    Ur, UHr = N.update_solution(prob, prob.UDG, prob.UH, prob.SH, blocks, x, 0.5)
+    say(f"  update: UDG {rel(U, Ur):.2e} UH {rel(UH, UHr):.2e}")
+    assert rel(UH, UHr) < 1e-14
+    assert rel(U, Ur) < 1e-11
+    r = ctx.residual_norm()
+    rr = N.residual_norm(prob, Ur, UHr, prob.SH)
+    say(f"  residual norm gpu {r:.6e} oracle {rr:.6e}")
+    assert abs(r - rr) <= 1e-8 * rr
+
+
+def test_newton_matches_oracle(oracle_mod):
+    import newton_oracle as N
+    from hfem_amd.lib import HDGContext
+    prob = make_problem("tri", 3, 2, seed=5, angle=0.3)
+    ctx = HDGContext(prob)
+    conv, nit, hist = ctx.solve_steady(newton_tol=1e-11, newton_maxiter=6, restart=30, gmres_tol=1e-12, ordering=1)
+    U, UH = ctx.get_solution()
+    Ur, UHr, okr, hr = N.newton_solve(prob, prob.UDG, prob.UH, prob.SH, newton_tol=1e-11, maxiter=6, restart=30,
+                                      gmres_tol=1e-12)
+    say(f"  newton gpu {nit} it {hist[:, :2].tolist()}; oracle {[(h['old'], h['new']) for h in hr]}")
+    assert conv and okr and nit == len(hr)
+    for k, h in enumerate(hr):
+        assert abs(hist[k, 0] - h["old"]) <= 1e-8 * h["old"] + 1e-12
+        assert hist[k, 2] == h["gmres_iters"]
+    assert rel(U, Ur) < 1e-10 and rel(UH, UHr) < 1e-10
+    ctx.close()
+
+
+def test_dirk_step_matches_oracle(oracle_mod):
+    import newton_oracle as N
+    from hfem_amd.lib import HDGContext
+    prob = make_problem("tri", 2, 2, seed=9, angle=0.3)
+    ctx = HDGContext(prob)
+    conv, nit = ctx.dirk_step(0.05, 3, 3, newton_tol=1e-11, newton_maxiter=8, restart=30, gmres_tol=1e-12)
+    U, UH = ctx.get_solution()
+    Ur, UHr, SHr, hist = N.dirk_step(prob, prob.UDG, prob.UH, prob.SH, 0.05, newton_tol=1e-11, maxiter=8,
+                                     restart=30, gmres_tol=1e-12)
+    say(f"  dirk: newton its gpu {nit} oracle {sum(len(h) for h in hist)}; UDG {rel(U, Ur):.2e} UH {rel(UH, UHr):.2e}")
+    assert conv
+    assert rel(U, Ur) < 1e-9 and rel(UH, UHr) < 1e-9
+    ctx.close()
