@@ -81,7 +81,6 @@ extern "C" int hfem_create(hfem_ctx **out, const hfem_dims *hd, int device)
         if (hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
             maxlds = 65536;
         c->maxlds = maxlds;
-        c->dinv_lds = condense_lds_bytes(d.na, true, sizeof(double)) <= (size_t)maxlds;
         if (assemble_lds_bytes(d) > (size_t)maxlds) {
             delete c;
             return hfem_fail(-1, "hfem_create: element family needs %zu bytes of LDS, device offers %d",
@@ -96,10 +95,13 @@ extern "C" int hfem_create(hfem_ctx **out, const hfem_dims *hd, int device)
             return hfem_fail(-1, "hfem_create: na=%d needs more LDS than the device offers (%d bytes)", d.na, maxlds);
         }
     }
-    if (!c->dinv_lds) ALLOC(dinv, (long)c->grid_cond * d.na * condense_dinv_ld(d.na));
+    ALLOC(dinv, ne * d.na * d.na);
     ALLOC(rwork, (long)c->grid_cond * d.na * c->ldx);
 #undef ALLOC
     if (e == hipSuccess) e = hipStreamCreate(&c->stream);
+    if (e == hipSuccess && (rocblas_create_handle(&c->blas) != rocblas_status_success ||
+                            rocblas_set_stream(c->blas, c->stream) != rocblas_status_success))
+        e = hipErrorInitializationError;
     for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
     if (e != hipSuccess) {
         hfem_destroy(c);
@@ -126,6 +128,7 @@ extern "C" int hfem_destroy(hfem_ctx *c)
         if (ev) hipEventDestroy(ev);
     for (auto &ev : c->sv.ev)
         if (ev) hipEventDestroy(ev);
+    if (c->blas) rocblas_destroy_handle(c->blas);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -314,8 +317,23 @@ extern "C" int hfem_condense(hfem_ctx *c, int e0, int n, int precision)
     const int grid = std::min(n, c->grid_cond);
     HIPCHK(hipEventRecord(c->ev[2], c->stream));
     if (precision == HFEM_FP64)
-        HIPCHK(launch_condense_fp64(d.na, d.nb, c->ldx, e0, n, c->D, c->dinv, c->Fx, c->X, c->K, c->Hx, c->info, grid,
-                                    c->dinv_lds, c->stream));
+    {
+        // inv(D) per element, then X^T = [F Ru]^T inv(D)^T (X row-major = column-major X^T,
+        // ld ldx) and [AH FH] = [H Rh] - K X as strided-batched DGEMMs
+        HIPCHK(launch_dinv(d.na, e0, n, c->D, c->dinv, c->info, c->stream));
+        const long sx = (long)d.na * c->ldx, sd = (long)d.na * d.na, sk = (long)d.nb * d.na,
+                   sh = (long)d.nb * (d.nb + 1);
+        const double one = 1.0, zero = 0.0, mone = -1.0;
+        rocblas_status st = rocblas_dgemm_strided_batched(
+            c->blas, rocblas_operation_none, rocblas_operation_transpose, d.nb + 1, d.na, d.na, &one,
+            c->Fx + e0 * sx, c->ldx, sx, c->dinv + e0 * sd, d.na, sd, &zero, c->X + e0 * sx, c->ldx, sx, n);
+        if (st == rocblas_status_success)
+            st = rocblas_dgemm_strided_batched(c->blas, rocblas_operation_none, rocblas_operation_transpose, d.nb,
+                                               d.nb + 1, d.na, &mone, c->K + e0 * sk, d.nb, sk, c->X + e0 * sx,
+                                               c->ldx, sx, &one, c->Hx + e0 * sh, d.nb, sh, n);
+        if (st != rocblas_status_success)
+            return hfem_fail(-2, "hfem_condense: rocblas_dgemm_strided_batched failed: %s", rocblas_status_to_string(st));
+    }
     else
         HIPCHK(launch_condense_mixed(d.na, d.nb, c->ldx, e0, n, c->D, c->Fx, c->X, c->rwork, c->K, c->Hx, c->info,
                                      grid, 2, c->stream));

@@ -1,11 +1,15 @@
 // Static condensation of element blocks (assembly/elementschur.cpp:6-329) on gfx950.
 //
-// Per element (one 256-thread workgroup, grid-stride over elements):
-//   1. inv(D) by Gauss-Jordan elimination with partial pivoting, the na x na
-//      matrix distributed cyclically over a 16 x 16 thread grid in registers
-//      (pivot column and rows exchanged through LDS, double-buffered so each
-//      elimination step needs two barriers);
-//   2. X = inv(D) [F | Ru]                  (FP64 MFMA 16x16x4, inv(D) from LDS)
+// FP64 path: inv(D) by Gauss-Jordan elimination with partial pivoting, one
+// 256-thread workgroup per element with the na x na matrix distributed
+// cyclically over a 16 x 16 thread grid in registers (pivot column and rows
+// exchanged through LDS, double-buffered so each elimination step needs two
+// barriers); then X = inv(D) [F | Ru] and [AH | FH] = [H | Rh] - K X as
+// strided-batched DGEMMs over all elements.
+// Mixed path (one workgroup per element, all stages fused):
+//   1. inv(D) by the same Gauss-Jordan in FP32;
+//   2. X = inv(D) [F | Ru]                  (FP32 MFMA 16x16x4, inv(D) from LDS)
+//      refined against the FP64 D (FP64 MFMA residuals);
 //   3. [AH | FH] = [H | Rh] - K X           (FP64 MFMA 16x16x4, in place)
 // Internal layouts: D column-major (na x na); Fx row-major (na x ldx) holding
 // F then Ru in column nb; X the same layout; K column-major (nb x na); Hx
@@ -15,6 +19,8 @@
 // and refines X against the FP64 D with FP64 residuals; the Schur product stays
 // FP64 because AH must carry the FP64 accuracy.
 #include "hfem_device.h"
+
+#include <algorithm>
 
 namespace hfem {
 
@@ -214,20 +220,18 @@ __device__ __forceinline__ void schur_update(int na, int nb, int ldx, const doub
     }
 }
 
+// inv(D) of every element of [e0, e0+n) by Gauss-Jordan with partial pivoting
+// (schurImplementation 0 semantics), written column-major (ld na) to Dinv; the
+// products X = inv(D)[F Ru] and [AH FH] = [H Rh] - K X then run as batched
+// GEMMs.  One workgroup per element.
 template <int R>
 __global__ void __launch_bounds__(256)
-condense_fp64_kernel(int na, int nb, int ldx, int e0, int n, const double *__restrict__ Dm,
-                     double *__restrict__ Dinv_g, const double *__restrict__ Fx, double *__restrict__ Xm,
-                     const double *__restrict__ Km, double *__restrict__ Hx, int *__restrict__ info, int dinv_lds)
+dinv_kernel(int na, int e0, int n, const double *__restrict__ Dm, double *__restrict__ Dinv, int *__restrict__ info)
 {
     extern __shared__ double lds[];
     double *colbuf = lds, *rowJ = lds + 2 * 192, *rowP = lds + 4 * 192;
     int *perm = (int *)(lds + 6 * 192);
-    double *Ml = (double *)((char *)lds + lds_head(sizeof(double)));   // inv(D), column-major, ld ldl
-    const int ldl = dinv_ld(na);
     const int tid = threadIdx.x, ti = tid & 15, tj = tid >> 4;
-    const int wave = tid >> 6, nwave = blockDim.x >> 6;
-    const int lane = tid & 63, lr = lane & 15, lg = lane >> 4;
     for (int t = blockIdx.x; t < n; t += gridDim.x) {
         const long e = e0 + t;
         const double *D = Dm + e * (long)na * na;
@@ -241,41 +245,14 @@ condense_fp64_kernel(int na, int nb, int ldx, int e0, int n, const double *__res
             }
         int sing = gauss_jordan<R, double>(na, a, colbuf, rowJ, rowP, perm);
         if (tid == 0 && info) info[e] = sing;
-        // inv(D)[:, perm[m]] = a[:, m]
-        double *Mi = dinv_lds ? Ml : Dinv_g + (long)blockIdx.x * na * ldl;
+        double *Mi = Dinv + e * (long)na * na;
 #pragma unroll
         for (int c = 0; c < R; c++)
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 int i = ti + 16 * r, m = tj + 16 * c;
-                if (i < na && m < na) Mi[i + (long)ldl * perm[m]] = a[r][c];
+                if (i < na && m < na) Mi[i + (long)na * perm[m]] = a[r][c];
             }
-        __syncthreads();
-        // X = inv(D) Fx  (na x (nb+1))
-        const double *F = Fx + e * (long)na * ldx;
-        double *X = Xm + e * (long)na * ldx;
-        const int nbx = nb + 1, tm = (na + 31) / 32, tn = (nbx + 31) / 32;
-        for (int tile = wave; tile < tm * tn; tile += nwave) {
-            const int i0 = (tile % tm) * 32, j0 = (tile / tm) * 32;
-            f64x4 acc[2][2];
-            mfma_tile64(
-                na, i0, j0, [&](int i, int k) { return (i < na && k < na) ? Mi[i + (long)ldl * k] : 0.0; },
-                [&](int k, int j) { return (k < na && j < nbx) ? F[(long)k * ldx + j] : 0.0; }, acc);
-#pragma unroll
-            for (int x = 0; x < 2; x++)
-#pragma unroll
-                for (int y = 0; y < 2; y++) {
-                    const int j = j0 + 16 * y + lr;
-                    if (j >= nbx) continue;
-#pragma unroll
-                    for (int v = 0; v < 4; v++) {
-                        const int i = i0 + 16 * x + lg + 4 * v;
-                        if (i < na) X[(long)i * ldx + j] = acc[x][y][v];
-                    }
-                }
-        }
-        __syncthreads();   // X complete (global writes of this workgroup) before the Schur product
-        schur_update(na, nb, ldx, Km + e * (long)nb * na, X, Hx + e * (long)nb * (nb + 1));
         __syncthreads();
     }
 }
@@ -383,19 +360,15 @@ int condense_rows_per_thread(int na) { return na <= 32 ? 2 : (na + 15) / 16; }
 
 #define HFEM_R_CASES(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11)
 
-hipError_t launch_condense_fp64(int na, int nb, int ldx, int e0, int n, const double *D, double *Dinv_g,
-                                const double *Fx, double *X, const double *K, double *Hx, int *info, int grid,
-                                bool dinv_lds, hipStream_t s)
+hipError_t launch_dinv(int na, int e0, int n, const double *D, double *Dinv, int *info, hipStream_t s)
 {
     const int R = condense_rows_per_thread(na);
-    const size_t lds = condense_lds_bytes(na, dinv_lds, sizeof(double));
+    const size_t lds = condense_lds_bytes(na, false, sizeof(double));
+    const int grid = std::min(n, 32768);
     switch (R) {
 #define CASE(RR)                                                                                                   \
     case RR:                                                                                                       \
-        hipFuncSetAttribute((const void *)condense_fp64_kernel<RR>, hipFuncAttributeMaxDynamicSharedMemorySize,   \
-                            (int)lds);                                                                             \
-        hipLaunchKernelGGL(condense_fp64_kernel<RR>, dim3(grid), dim3(256), lds, s, na, nb, ldx, e0, n, D, Dinv_g, \
-                           Fx, X, K, Hx, info, dinv_lds ? 1 : 0);                                                  \
+        hipLaunchKernelGGL(dinv_kernel<RR>, dim3(grid), dim3(256), lds, s, na, e0, n, D, Dinv, info);             \
         break;
         HFEM_R_CASES(CASE)
 #undef CASE

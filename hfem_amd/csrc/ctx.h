@@ -5,6 +5,8 @@
 #include <string>
 #include <vector>
 
+#include <rocblas/rocblas.h>
+
 #include "../../include/hfem_amd.h"
 #include "hfem_device.h"
 
@@ -13,16 +15,13 @@ hipError_t launch_assemble(const Dims &, const Master &, const Phys &, const dou
                            const double *, const int *, const int *, int, int, double *, long, int, int, double *, long,
                            int, double *, double *, double *, double *, double *, double *, double *, hipStream_t);
 size_t flux_lds_bytes(const Dims &d);
-hipError_t launch_condense_fp64(int na, int nb, int ldx, int e0, int n, const double *D, double *Dinv_g,
-                                const double *Fx, double *X, const double *K, double *Hx, int *info, int grid,
-                                bool dinv_lds, hipStream_t s);
+hipError_t launch_dinv(int na, int e0, int n, const double *D, double *Dinv, int *info, hipStream_t s);
 hipError_t launch_condense_mixed(int na, int nb, int ldx, int e0, int n, const double *D, const double *Fx, double *X,
                                  double *Rw, const double *K, double *Hx, int *info, int grid, int sweeps,
                                  hipStream_t s);
 size_t condense_lds_bytes(int na, bool dinv_in_lds, int elt);
 size_t assemble_lds_bytes(const Dims &d);
 bool assemble_supported(int nd, int npv, int npf);
-int condense_dinv_ld(int na);
 hipError_t launch_bsr_gather(int, int, int, int, int, long, const int *, const int *, const int *, const double *,
                              double *, hipStream_t);
 hipError_t launch_face_gather(int, int, int, int, int, long, const int *, const int *, const double *, double *,
@@ -123,12 +122,12 @@ struct hfem_ctx {
     int *info = nullptr;
     int ldx = 0;
     // work
-    double *scratch = nullptr, *dinv = nullptr, *rwork = nullptr, *fluxbuf = nullptr;
+    double *scratch = nullptr, *dinv = nullptr, *rwork = nullptr, *fluxbuf = nullptr;   // dinv: ne x na x na
+    rocblas_handle blas = nullptr;
     long fstride = 0;
     int chunk = 0;
     long sstride = 0;
     int grid_asm = 0, grid_cond = 0;
-    bool dinv_lds = true;
     int maxlds = 0;
     // trace system
     double *Hg = nullptr, *Rg = nullptr, *r = nullptr, *partial = nullptr, *xv = nullptr, *yv = nullptr;
