@@ -228,67 +228,102 @@ bilu0_factor_kernel(int bsz, const int *__restrict__ rowpts, const int *__restri
     }
 }
 
-// Forward sweep L y = x over the rows of one level (kernel/applyBILU0.m:12-27):
-// y_r = x_r - sum over earlier neighbours k of L_rk y_k.  One wave per row.
-__global__ void __launch_bounds__(SV_THREADS)
-bilu0_lsolve_kernel(int bsz, const int *__restrict__ rowpts, const int *__restrict__ colind,
-                    const int *__restrict__ u2o, const int *__restrict__ rows, int nlev, const double *__restrict__ Mg,
-                    const double *__restrict__ x, double *__restrict__ y)
+// Broadcast lane `src` (wave-uniform) of a double to the whole wave.
+__device__ __forceinline__ double wave_bcast(double v, int src)
 {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, wpb = blockDim.x >> 6;
-    const long b2 = (long)bsz * bsz;
-    for (int t = blockIdx.x * wpb + wv; t < nlev; t += gridDim.x * wpb) {
-        const int r = rows[t], o = u2o[r];
-        for (int i = lane; i < bsz; i += 64) {
-            double s = x[(long)r * bsz + i];
-            for (int k = rowpts[r] + 1; k < rowpts[r + 1]; k++) {
-                const int c = colind[k];
-                if (u2o[c] >= o) continue;
-                const double *blk = Mg + (long)k * b2 + i, *yc = y + (long)c * bsz;
-                for (int cc = 0; cc < bsz; cc++) s -= blk[(long)bsz * cc] * yc[cc];
-            }
-            y[(long)r * bsz + i] = s;
-        }
-    }
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, src);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Backward sweep U v = y over the rows of one level (kernel/applyBILU0.m:29-48):
-// v_r = inv(U_rr) (y_r - sum over later neighbours k of U_rk v_k).  One wave per
-// row; every wave of the workgroup runs the same number of iterations so the
-// barriers around the LDS staging are uniform.
-__global__ void __launch_bounds__(SV_THREADS)
-bilu0_usolve_kernel(int bsz, const int *__restrict__ rowpts, const int *__restrict__ colind,
-                    const int *__restrict__ u2o, const int *__restrict__ rows, int nlev, const double *__restrict__ Mg,
-                    const double *__restrict__ y, double *__restrict__ v)
+// acc_i += sum_cc B[i + bsz cc] z[cc] for the lane's rows i = lane, lane + 64, with z
+// held one entry per lane (z0 = z[lane], z1 = z[lane + 64]) and broadcast by readlane.
+__device__ __forceinline__ void wave_block_matvec(int bsz, const double *__restrict__ B, double z0, double z1,
+                                                  double &acc0, double &acc1)
 {
-    extern __shared__ double lds[];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, wpb = blockDim.x >> 6;
-    const long b2 = (long)bsz * bsz;
-    double *tb = lds + (long)wv * bsz;
-    for (int base = blockIdx.x * wpb; base < nlev; base += gridDim.x * wpb) {
-        const int t = base + wv;
-        const bool valid = t < nlev;
-        const int r = valid ? rows[t] : 0;
-        if (valid) {
-            const int o = u2o[r];
-            for (int i = lane; i < bsz; i += 64) {
-                double s = y[(long)r * bsz + i];
-                for (int k = rowpts[r] + 1; k < rowpts[r + 1]; k++) {
-                    const int c = colind[k];
-                    if (u2o[c] <= o) continue;
-                    const double *blk = Mg + (long)k * b2 + i, *vc = v + (long)c * bsz;
-                    for (int cc = 0; cc < bsz; cc++) s -= blk[(long)bsz * cc] * vc[cc];
-                }
-                tb[i] = s;
-            }
+    const int lane = threadIdx.x & 63;
+    const bool two = bsz > 64;
+    double s0 = 0.0, s1 = 0.0, u0 = 0.0, u1 = 0.0;
+    int cc = 0;
+    for (; cc + 2 <= bsz; cc += 2) {
+        const double za = wave_bcast(cc < 64 ? z0 : z1, cc & 63);
+        const double zb = wave_bcast(cc + 1 < 64 ? z0 : z1, (cc + 1) & 63);
+        if (lane < bsz) {
+            s0 += B[lane + (long)bsz * cc] * za;
+            u0 += B[lane + (long)bsz * (cc + 1)] * zb;
         }
+        if (two && lane + 64 < bsz) {
+            s1 += B[lane + 64 + (long)bsz * cc] * za;
+            u1 += B[lane + 64 + (long)bsz * (cc + 1)] * zb;
+        }
+    }
+    if (cc < bsz) {
+        const double za = wave_bcast(cc < 64 ? z0 : z1, cc & 63);
+        if (lane < bsz) s0 += B[lane + (long)bsz * cc] * za;
+        if (two && lane + 64 < bsz) s1 += B[lane + 64 + (long)bsz * cc] * za;
+    }
+    acc0 += s0 + u0;
+    acc1 += s1 + u1;
+}
+
+// One BILU0 sweep over the rows of one level (kernel/applyBILU0.m).  For row r,
+// t = b_r - sum over the neighbours k taken by the sweep (numbered before r in the
+// forward sweep L y = x, after r in the backward sweep U v = y) of M_rk z_k; the
+// forward sweep stores z_r = t, the backward sweep z_r = inv(U_rr) t.  One
+// 256-thread workgroup per row: the four waves take the neighbour blocks in turn
+// (lane = block row, the neighbour vector broadcast from registers), their partial
+// sums meet in LDS.
+template <bool BACKWARD>
+__global__ void __launch_bounds__(SV_THREADS)
+bilu0_sweep_kernel(int bsz, const int *__restrict__ rowpts, const int *__restrict__ colind,
+                   const int *__restrict__ u2o, const int *__restrict__ rows, int nlev, const double *__restrict__ Mg,
+                   const double *__restrict__ b, double *__restrict__ z)
+{
+    __shared__ double red[SV_THREADS / 64][128];
+    __shared__ double tb[128];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+    const long b2 = (long)bsz * bsz;
+    for (int t = blockIdx.x; t < nlev; t += gridDim.x) {
+        const int r = rows[t], o = u2o[r], k0 = rowpts[r], k1 = rowpts[r + 1];
+        double acc0 = 0.0, acc1 = 0.0;
+        int sel = 0;
+        for (int k = k0 + 1; k < k1; k++) {
+            const int c = colind[k];
+            const bool take = BACKWARD ? (u2o[c] > o) : (u2o[c] < o);
+            if (!take) continue;
+            if ((sel++ % nw) != wv) continue;
+            const double *zc = z + (long)c * bsz;
+            const double z0 = lane < bsz ? zc[lane] : 0.0, z1 = lane + 64 < bsz ? zc[lane + 64] : 0.0;
+            wave_block_matvec(bsz, Mg + (long)k * b2, z0, z1, acc0, acc1);
+        }
+        red[wv][lane] = acc0;
+        red[wv][lane + 64] = acc1;
         __syncthreads();
-        if (valid) {
-            const double *Ui = Mg + (long)rowpts[r] * b2;
-            for (int i = lane; i < bsz; i += 64) {
-                double s = 0.0;
-                for (int cc = 0; cc < bsz; cc++) s += Ui[i + (long)bsz * cc] * tb[cc];
-                v[(long)r * bsz + i] = s;
+        for (int i = tid; i < bsz; i += blockDim.x) {
+            double sum = 0.0;
+            for (int w = 0; w < nw; w++) sum += red[w][i];
+            const double v = b[(long)r * bsz + i] - sum;
+            if (BACKWARD) tb[i] = v;
+            else z[(long)r * bsz + i] = v;
+        }
+        if (BACKWARD) {
+            __syncthreads();
+            // z_r = inv(U_rr) t: wave w takes the columns cc = w, w + nw, ...
+            const double *Ui = Mg + (long)k0 * b2;
+            double a0 = 0.0, a1 = 0.0;
+            for (int cc = wv; cc < bsz; cc += nw) {
+                const double tc = tb[cc];
+                if (lane < bsz) a0 += Ui[lane + (long)bsz * cc] * tc;
+                if (lane + 64 < bsz) a1 += Ui[lane + 64 + (long)bsz * cc] * tc;
+            }
+            red[wv][lane] = a0;
+            red[wv][lane + 64] = a1;
+            __syncthreads();
+            for (int i = tid; i < bsz; i += blockDim.x) {
+                double sum = 0.0;
+                for (int w = 0; w < nw; w++) sum += red[w][i];
+                z[(long)r * bsz + i] = sum;
             }
         }
         __syncthreads();
@@ -303,12 +338,12 @@ block_jacobi_kernel(int nrows, int bsz, const double *__restrict__ Dinv, const d
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, wpb = blockDim.x >> 6;
     const long b2 = (long)bsz * bsz;
     for (int r = blockIdx.x * wpb + wv; r < nrows; r += gridDim.x * wpb) {
-        const double *Di = Dinv + (long)r * b2, *xr = x + (long)r * bsz;
-        for (int i = lane; i < bsz; i += 64) {
-            double s = 0.0;
-            for (int cc = 0; cc < bsz; cc++) s += Di[i + (long)bsz * cc] * xr[cc];
-            y[(long)r * bsz + i] = s;
-        }
+        const double *xr = x + (long)r * bsz;
+        const double z0 = lane < bsz ? xr[lane] : 0.0, z1 = lane + 64 < bsz ? xr[lane + 64] : 0.0;
+        double a0 = 0.0, a1 = 0.0;
+        wave_block_matvec(bsz, Dinv + (long)r * b2, z0, z1, a0, a1);
+        if (lane < bsz) y[(long)r * bsz + lane] = a0;
+        if (lane + 64 < bsz) y[(long)r * bsz + lane + 64] = a1;
     }
 }
 
@@ -475,23 +510,24 @@ hipError_t launch_bilu0_factor_level(int bsz, const int *rowpts, const int *coli
 hipError_t launch_bilu0_lsolve_level(int bsz, const int *rowpts, const int *colind, const int *u2o, const int *rows,
                                      int nlev, const double *Mg, const double *x, double *y, hipStream_t s)
 {
-    const int wpb = SV_THREADS / 64;
-    hipLaunchKernelGGL(bilu0_lsolve_kernel, dim3(std::min((nlev + wpb - 1) / wpb, 8192)), dim3(SV_THREADS), 0, s, bsz,
-                       rowpts, colind, u2o, rows, nlev, Mg, x, y);
+    if (bsz > 128) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(bilu0_sweep_kernel<false>, dim3(std::min(nlev, 8192)), dim3(SV_THREADS), 0, s, bsz, rowpts,
+                       colind, u2o, rows, nlev, Mg, x, y);
     return hipGetLastError();
 }
 
 hipError_t launch_bilu0_usolve_level(int bsz, const int *rowpts, const int *colind, const int *u2o, const int *rows,
                                      int nlev, const double *Mg, const double *y, double *v, hipStream_t s)
 {
-    const int wpb = SV_THREADS / 64;
-    hipLaunchKernelGGL(bilu0_usolve_kernel, dim3(std::min((nlev + wpb - 1) / wpb, 8192)), dim3(SV_THREADS),
-                       sizeof(double) * wpb * bsz, s, bsz, rowpts, colind, u2o, rows, nlev, Mg, y, v);
+    if (bsz > 128) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(bilu0_sweep_kernel<true>, dim3(std::min(nlev, 8192)), dim3(SV_THREADS), 0, s, bsz, rowpts,
+                       colind, u2o, rows, nlev, Mg, y, v);
     return hipGetLastError();
 }
 
 hipError_t launch_block_jacobi(int nrows, int bsz, const double *Dinv, const double *x, double *y, hipStream_t s)
 {
+    if (bsz > 128) return hipErrorInvalidValue;
     const int wpb = SV_THREADS / 64;
     hipLaunchKernelGGL(block_jacobi_kernel, dim3(std::min((nrows + wpb - 1) / wpb, 8192)), dim3(SV_THREADS), 0, s,
                        nrows, bsz, Dinv, x, y);
