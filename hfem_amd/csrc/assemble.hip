@@ -30,6 +30,8 @@ namespace hfem {
 
 constexpr int ASM_THREADS = 512;
 
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
 // LDS copies of the master tables, transposed so that the thread-varying index is
 // fastest: svt[n + npv*(g + ngv*dd)], svg[i + npv*(g + ngv*dd)] (as shapvg),
 // sft[k + npf*(g + ngf*dd)], sfg[k + npf*(g + ngf*dd)] (as shapfg).
@@ -56,7 +58,8 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
 
     // ---- LDS layout ----
     double *p = lds;
-    double *svt = p; p += npv * ngv * nd1;
+    // shapvt; its derivative part (dead after P1) later holds fu of one row block
+    double *svt = p; p += npv * ngv + (npv * ND > ND * NC ? npv : NC) * ngv * ND;
     double *svg = p; p += npv * ngv * nd1;
     double *sft = p; p += npf * ngf * ND;
     double *sfg = p; p += npf * ngf * ND;
@@ -68,13 +71,17 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
     double *Xx = p; p += ngv * ND * ND;        // Xx(g,x,r) = jac d xi_r / d x_x
     double *jacf = p; p += np;
     double *nl = p; p += np * ND;
-    double *M = p; p += npv * npv;             // mass matrix, then its inverse
-    double *Qm = p; p += (ngv + np) * npv;     // [k + npv*r]: rows of shapvt / face traces times M^{-1}
-    double *rowk = p; p += npv;
-    double *colk = p; p += npv;
+    // mass matrix / its inverse, Q and the Gauss-Jordan rows; later G (P8v)
+    double *M = p;
+    double *Qm = M + npv * npv;                // [k + npv*r]: rows of shapvt / face traces times M^{-1}
+    double *rowk = Qm + (ngv + np) * npv;
+    double *colk = rowk + npv;
+    p += (npv + ngv + np + 2) * npv > npv * ngv * ND ? (npv + ngv + np + 2) * npv : npv * ngv * ND;
     double *shapMiC = p; p += npv * ngv * ND;  // [n + npv*(g + ngv*x)]
     double *shapMiE = p; p += ndf * ngv * ND;  // [l + ndf*(g + ngv*x)], includes the minus sign
     double *udgg = p; p += ngv * nc;           // [g + ngv*c]
+    double *fhuj = p; p += np * nc;            // fhu of one row block [pp + np*c]
+    double *fhuhj = p; p += np * nch;          // fhuh of one row block [pp + np*kc]
     double *red = p; p += ASM_THREADS;
     int *perm = (int *)p;
     int *invperm = perm + npf * nfe;
@@ -83,7 +90,7 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
     // ---- per-workgroup scratch (global) ----
     double *S = scratch + (long)blockIdx.x * sstride;
     const Scratch L = scratch_layout(d);
-    double *C = S + L.C, *E = S + L.E, *shapMiCf = S + L.shapMiCf, *shapMiEf = S + L.shapMiEf, *wrl = S + L.wrl;
+    double *C = S + L.C, *E = S + L.E, *shapMiCf = S + L.shapMiCf, *shapMiEf = S + L.shapMiEf;
     const double tau = ph.param[5];
 
     // master tables once per workgroup
@@ -277,88 +284,154 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
         }
         __syncthreads();
 
-        // ---------------- P7: wrl ----------------
-        for (int i = tid; i < ngv * nd1 * ncu * nc; i += nth) {
-            int g = i % ngv, r = i / ngv, mq = r % nd1, r2 = r / nd1, j = r2 % ncu, c = r2 / ncu;
-            double v;
-            if (mq == 0) {
-                double su = (ph.tdep && c == j) ? -ph.fc_u : 0.0;
-                v = -su * jac[g];
-            } else {
-                double s = 0;
-                for (int x = 0; x < ND; x++) s += fu[g + ngv * (j + ncu * (x + ND * c))] * Xx[g + ngv * (x + ND * (mq - 1))];
-                v = -s;
+        // ---------------- P8: element rows of D and F on the FP64 MFMA ----------------
+        // Row block j of D and F as one product over volume and face points.  With
+        // G(i,g,x) = sum_r shapvg(i,g,r+1) Xx(g,x,r) (jac-weighted test-function
+        // gradients), the q-folded volume integrand of elementint (:418-549) is
+        //   D_j[(kc,i), cc] += sum_{x',g} P[(kc,i),(x',g)] Bd[(x',g), cc]
+        //   F_j[(kc,i), l]  += sum_{x>=1,g} P[(kc,i),(x,g)] Be[(x,g), l]
+        //   P = -sum_x fu(g,j,x, c) G(i,g,x) (+ fc_u jac shapvg(i,g,0) for c = j, tdep),
+        //   c = x' ncu + kc;  Bd = [shapvt(g,cc); M^{-1}C traces], Be = M^{-1}E traces;
+        // the face integrand of faceint (:862-1072), the gather over the faces that
+        // contain node i, is the same product over face points pp = (a, g):
+        //   A = [i on a] shapfg(k(i,a), g) jacf(pp) fhu(pp, j, c),
+        //   D: B = [cc on a] shapft(g, k(cc,a)) (c = kc), M^{-1}C face traces (c = q);
+        //   F: B = [l on a] shapft(g, l) with fhuh in A (u part), -M^{-1}E traces (q).
+        // One wave per (j, 16-row tile of (kc,i)), all 16-column tiles of D (cc) and
+        // F (l) accumulated together; A formed on the fly from LDS operands.
+        {
+            double *G = M;                                   // M, Qm dead after P5
+            double *fuj = svt + npv * ngv;                   // shapvt derivatives dead after P1
+            for (int i = tid; i < npv * ngv * ND; i += nth) {
+                const int nn = i % npv, r = i / npv, g = r % ngv, x = r / ngv;
+                double sg = 0;
+#pragma unroll
+                for (int q = 0; q < ND; q++) sg += SVG(nn, g, q + 1) * Xx[g + ngv * (x + ND * q)];
+                G[i] = sg;
             }
-            wrl[i] = v;
+            const int lane = tid & 63, wave = tid >> 6, nwave = nth >> 6, lr = lane & 15, lk = lane >> 4;
+            const int nrt = (ncu * npv + 15) / 16;
+            // column tiles: TD of D (cc < npv), TF of F (l < ndf); faces per element from
+            // the face size (tets: npf triangular, hexes: npf square)
+            constexpr int NFE = ND == 2 ? 3 : ((NPF == 4 || NPF == 9 || NPF == 16) ? 6 : 4);
+            constexpr int TD = (NPV + 15) / 16, TF = (NPF * NFE + 15) / 16, NT = TD + TF;
+            for (int j = 0; j < ncu; j++) {
+                __syncthreads();
+                for (int i = tid; i < ngv * ND * NC; i += nth) {
+                    const int g = i % ngv, r = i / ngv, x = r % ND, c = r / ND;
+                    fuj[i] = fu[g + ngv * (j + ncu * (x + ND * c))];
+                }
+                for (int i = tid; i < np * NC; i += nth) {
+                    const int pp = i % np, c = i / np;
+                    fhuj[i] = fhu[pp + np * (j + nch * c)];
+                    if (c < nch) fhuhj[i] = fhuh[pp + np * (j + nch * c)];
+                }
+                __syncthreads();
+                for (int rt = wave; rt < nrt; rt += nwave) {
+                    // this lane's A row and its accumulator rows
+                    const int ra = rt * 16 + lr;
+                    const bool va = ra < ncu * npv;
+                    const int kca = va ? ra / npv : 0, ia = va ? ra % npv : 0;
+                    f64x4 acc[NT];
+#pragma unroll
+                    for (int t = 0; t < NT; t++) acc[t] = (f64x4){0.0, 0.0, 0.0, 0.0};
+                    for (int xq = 0; xq < nd1; xq++) {
+                        for (int g0 = 0; g0 < ngv; g0 += 4) {
+                            const int g = g0 + lk;
+                            double a = 0.0;
+                            if (va && g < ngv) {
+                                const int c = xq * ncu + kca;
+                                double sa = 0.0;
+#pragma unroll
+                                for (int x = 0; x < ND; x++) sa += fuj[g + ngv * (x + ND * c)] * G[ia + npv * (g + ngv * x)];
+                                a = -sa;
+                                if (xq == 0 && ph.tdep && kca == j) a += SVG(ia, g, 0) * ph.fc_u * jac[g];
+                            }
+#pragma unroll
+                            for (int t = 0; t < TD; t++) {
+                                const int cc = t * 16 + lr;
+                                double b = 0.0;
+                                if (cc < npv && g < ngv)
+                                    b = xq == 0 ? SVT(g, cc, 0) : shapMiC[cc + npv * (g + ngv * (xq - 1))];
+                                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+                            }
+                            if (xq > 0) {
+#pragma unroll
+                                for (int t = TD; t < NT; t++) {
+                                    const int l = (t - TD) * 16 + lr;
+                                    const double b = (l < ndf && g < ngv) ? shapMiE[l + ndf * (g + ngv * (xq - 1))] : 0.0;
+                                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+                                }
+                            }
+                        }
+                    }
+                    // face points
+                    for (int p0 = 0; p0 < np; p0 += 4) {
+                        const int pp = p0 + lk;
+                        const bool vp = pp < np;
+                        const int a = vp ? pp / ngf : 0, gf = vp ? pp % ngf : 0;
+                        double af = 0.0;
+                        if (va && vp) {
+                            const int k = INVP(ia, a);
+                            if (k >= 0) af = SFG(k, gf, 0) * jacf[pp];
+                        }
+                        // u part: D with fhu, F with fhuh
+                        {
+                            const double ad = af * fhuj[(vp ? pp : 0) + np * kca];
+                            const double ah = af * fhuhj[(vp ? pp : 0) + np * kca];
+#pragma unroll
+                            for (int t = 0; t < TD; t++) {
+                                const int cc = t * 16 + lr;
+                                double b = 0.0;
+                                if (cc < npv && vp) {
+                                    const int k = INVP(cc, a);
+                                    if (k >= 0) b = SFT(gf, k, 0);
+                                }
+                                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, b, acc[t], 0, 0, 0);
+                            }
+#pragma unroll
+                            for (int t = TD; t < NT; t++) {
+                                const int l = (t - TD) * 16 + lr;
+                                const double b = (l < ndf && vp && l / npf == a) ? SFT(gf, l % npf, 0) : 0.0;
+                                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ah, b, acc[t], 0, 0, 0);
+                            }
+                        }
+                        // q parts
+                        for (int x = 0; x < ND; x++) {
+                            const double aq = af * fhuj[(vp ? pp : 0) + np * (ncu * (x + 1) + kca)];
+#pragma unroll
+                            for (int t = 0; t < TD; t++) {
+                                const int cc = t * 16 + lr;
+                                const double b = (cc < npv && vp) ? shapMiCf[cc + npv * (pp + np * x)] : 0.0;
+                                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(aq, b, acc[t], 0, 0, 0);
+                            }
+#pragma unroll
+                            for (int t = TD; t < NT; t++) {
+                                const int l = (t - TD) * 16 + lr;
+                                const double b = (l < ndf && vp) ? -shapMiEf[l + ndf * (pp + np * x)] : 0.0;
+                                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(aq, b, acc[t], 0, 0, 0);
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int t = 0; t < NT; t++)
+#pragma unroll
+                        for (int v = 0; v < 4; v++) {
+                            const int r = rt * 16 + lk + 4 * v;
+                            if (r >= ncu * npv) continue;
+                            const int kcr = r / npv, ir = r % npv;
+                            if (t < TD) {
+                                const int cc = t * 16 + lr;
+                                if (cc < npv) Dm[e * (long)na * na + (long)na * (kcr * npv + cc) + j * npv + ir] = acc[t][v];
+                            } else {
+                                const int l = (t - TD) * 16 + lr;
+                                if (l < ndf) Fx[e * (long)na * ldx + (long)(j * npv + ir) * ldx + l * nch + kcr] = acc[t][v];
+                            }
+                        }
+                }
+            }
         }
         __syncthreads();
-
-        // ---------------- P8: element rows of D and F ----------------
-        // thread column: D column (kc, cc) for row block j  -> cc = element node
-        //                F column (cc, kc) for row block j  -> cc = trace node
-        {
-            const int nDc = ncu * ncu * npv, nFc = ncu * nch * ndf;
-            const long cs = (long)ngv * nd1 * ncu;          // stride of wrl's column index
-            for (int col = tid; col < nDc + nFc; col += nth) {
-                const bool isD = col < nDc;
-                int cc, kc, j;
-                if (isD) { cc = col % npv; int r = col / npv; kc = r % ncu; j = r / ncu; }
-                else { int c2 = col - nDc; cc = c2 % ndf; int r = c2 / ndf; kc = r % nch; j = r / nch; }
-                double acc[NPV];
-#pragma unroll
-                for (int i = 0; i < NPV; i++) acc[i] = 0.0;
-                // volume: sum_{g,m'} shapvg(i,g,m') W(g,m')
-                const double *wr = wrl + ngv * nd1 * (long)j;
-                for (int mq = 0; mq < nd1; mq++)
-                    for (int g = 0; g < ngv; g++) {
-                        const double *wg = wr + g + ngv * mq;
-                        double w = 0.0;
-                        if (isD) w = wg[cs * kc] * SVT(g, cc, 0);
-#pragma unroll
-                        for (int x = 0; x < ND; x++)
-                            w += wg[cs * (ncu * (x + 1) + kc)] *
-                                 (isD ? shapMiC[cc + npv * (g + ngv * x)] : shapMiE[cc + ndf * (g + ngv * x)]);
-                        // uniform index: the test functions come through the scalar cache
-                        const double *sv = m.shapvg + npv * (g + ngv * mq);
-#pragma unroll
-                        for (int i = 0; i < NPV; i++) acc[i] += sv[i] * w;
-                    }
-                // faces: rows on face a get sum_g shapfg(kk,g) jacf z(p)
-                for (int a = 0; a < nfe; a++) {
-                    for (int g = 0; g < ngf; g++) {
-                        const int pp = g + ngf * a;
-                        double z;
-                        if (isD) {
-                            int k = INVP(cc, a);
-                            z = k >= 0 ? fhu[pp + np * (j + nch * kc)] * SFT(g, k, 0) : 0.0;
-#pragma unroll
-                            for (int x = 0; x < ND; x++)
-                                z += fhu[pp + np * (j + nch * (ncu * (x + 1) + kc))] * shapMiCf[cc + npv * (pp + np * x)];
-                        } else {
-                            z = (cc / npf == a) ? fhuh[pp + np * (j + nch * kc)] * SFT(g, cc % npf, 0) : 0.0;
-#pragma unroll
-                            for (int x = 0; x < ND; x++)
-                                z -= fhu[pp + np * (j + nch * (ncu * (x + 1) + kc))] * shapMiEf[cc + ndf * (pp + np * x)];
-                        }
-                        const double jz = jacf[pp] * z;
-#pragma unroll
-                        for (int i = 0; i < NPV; i++) {
-                            const int kk = m.invperm[i + npv * a];
-                            if (kk >= 0) acc[i] += m.shapfg[kk + npf * g] * jz;
-                        }
-                    }
-                }
-                if (isD) {
-                    double *Dcol = Dm + e * (long)na * na + (long)na * (kc * npv + cc) + j * npv;
-#pragma unroll
-                    for (int i = 0; i < NPV; i++) Dcol[i] = acc[i];
-                } else {
-                    double *Fcol = Fx + e * (long)na * ldx + (long)(j * npv) * ldx + cc * nch + kc;
-#pragma unroll
-                    for (int i = 0; i < NPV; i++) Fcol[(long)i * ldx] = acc[i];
-                }
-            }
-        }
 
         // ---------------- P9: trace rows of K and H; Ru, Rh ----------------
         {
@@ -811,12 +884,13 @@ size_t flux_lds_bytes(const Dims &d)
 
 size_t assemble_lds_bytes(const Dims &d)
 {
-    size_t n = 2 * (size_t)d.npv * d.ngv * d.nd1 + 2 * (size_t)d.npf * d.ngf * d.nd + (size_t)d.npv * d.nd +
-               (size_t)d.npv * d.nc + (size_t)d.ndf * d.nch + (size_t)d.npv * d.ncu + d.ngv +
-               (size_t)d.ngv * d.nd * d.nd + d.np + (size_t)d.np * d.nd + (size_t)d.npv * d.npv +
-               (size_t)(d.ngv + d.np) * d.npv + 2 * (size_t)d.npv + (size_t)d.npv * d.ngv * d.nd + (size_t)d.ndf * d.ngv * d.nd + (size_t)d.ngv * d.nc +
-               ASM_THREADS;
-    size_t ints = (size_t)d.npf * d.nfe + (size_t)d.npv * d.nfe + d.nfe + 8;
+    const size_t nc = d.nc, npv = d.npv, ngv = d.ngv, np = d.np, nd = d.nd;
+    const size_t svt = npv * ngv + (npv > nc ? npv : nc) * ngv * nd;
+    const size_t mreg = (npv + ngv + np + 2) * npv > npv * ngv * nd ? (npv + ngv + np + 2) * npv : npv * ngv * nd;
+    size_t n = svt + npv * ngv * d.nd1 + 2 * (size_t)d.npf * d.ngf * nd + npv * nd + npv * nc + (size_t)d.ndf * d.nch +
+               npv * d.ncu + ngv + ngv * nd * nd + np + np * nd + mreg + npv * ngv * nd + (size_t)d.ndf * ngv * nd +
+               ngv * nc + np * nc + np * d.nch + ASM_THREADS;
+    size_t ints = (size_t)d.npf * d.nfe + npv * d.nfe + d.nfe + 8;
     return n * sizeof(double) + ints * sizeof(int);
 }
 

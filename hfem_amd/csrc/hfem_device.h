@@ -152,7 +152,7 @@ __device__ __forceinline__ void ns_normal_jac(double gam, double eps, const dual
 
 // Per-workgroup scratch offsets (doubles) used by the assembly kernel.
 struct Scratch {
-    long C, E, shapMiCf, shapMiEf, wrl, total;
+    long C, E, shapMiCf, shapMiEf, total;
 };
 
 __host__ __device__ inline Scratch scratch_layout(const Dims &d)
@@ -164,7 +164,6 @@ __host__ __device__ inline Scratch scratch_layout(const Dims &d)
     s.E = take((long)d.nd * d.npv * d.ndf);
     s.shapMiCf = take((long)d.np * d.npv * d.nd);
     s.shapMiEf = take((long)d.np * d.ndf * d.nd);
-    s.wrl = take((long)d.ngv * d.nd1 * d.ncu * d.nc);
     s.total = o;
     return s;
 }
