@@ -71,12 +71,19 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
     double *Xx = p; p += ngv * ND * ND;        // Xx(g,x,r) = jac d xi_r / d x_x
     double *jacf = p; p += np;
     double *nl = p; p += np * ND;
-    // mass matrix / its inverse, Q and the Gauss-Jordan rows; later G (P8v)
+    // mass matrix / its inverse, Q and the Gauss-Jordan rows; later G (P8), then the
+    // flux Jacobians of one face (P9k)
     double *M = p;
     double *Qm = M + npv * npv;                // [k + npv*r]: rows of shapvt / face traces times M^{-1}
     double *rowk = Qm + (ngv + np) * npv;
     double *colk = rowk + npv;
-    p += (npv + ngv + np + 2) * npv > npv * ngv * ND ? (npv + ngv + np + 2) * npv : npv * ngv * ND;
+    double *xa = M, *xha = M + ngf * nch * nc;
+    {
+        int mr = (npv + ngv + np + 2) * npv;
+        if (npv * ngv * ND > mr) mr = npv * ngv * ND;
+        if (ngf * nch * (nc + nch) > mr) mr = ngf * nch * (nc + nch);
+        p += mr;
+    }
     double *shapMiC = p; p += npv * ngv * ND;  // [n + npv*(g + ngv*x)]
     double *shapMiE = p; p += ndf * ngv * ND;  // [l + ndf*(g + ngv*x)], includes the minus sign
     double *udgg = p; p += ngv * nc;           // [g + ngv*c]
@@ -433,52 +440,110 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
         }
         __syncthreads();
 
-        // ---------------- P9: trace rows of K and H; Ru, Rh ----------------
+        // ---------------- P9k: trace rows of K and H on the FP64 MFMA ----------------
+        // faceint's trace rows (:1254-1318; boundary faces with the far-field flux,
+        // :1074-1246): for face a, rows (a, kk, j) and columns (kc, cc) of K, (kc, l) of H,
+        //   K^T[cc, (kc,kk,j)] = sum_{x',g} Bk[cc, (x',g)] Pk[(x',g), (kc,kk,j)],
+        //   Pk = shapfg(kk,g) jacf(pp) xu(pp, j, x' ncu + kc)  (xuh for the u part of H),
+        // with Bk / Bh the same face-point operands as the face part of D / F and xu the
+        // interior (fhu) or boundary (fbu) flux Jacobian of the face.  Computed
+        // transposed so that a tile column runs along contiguous (kk, j) rows of K, H.
         {
-            const int nKc = nfe * nch * ncu * npv, nHc = nfe * nch * nch * ndf;
+            const int lane = tid & 63, wave = tid >> 6, nwave = nth >> 6, lr = lane & 15, lk = lane >> 4;
+            constexpr int NFE = ND == 2 ? 3 : ((NPF == 4 || NPF == 9 || NPF == 16) ? 6 : 4);
+            constexpr int TD = (NPV + 15) / 16, TF = (NPF * NFE + 15) / 16, NT = TD + TF;
+            const int ncol = ncu * npf * nch, nct = (ncol + 15) / 16;
+            const long nH = (long)nb * nb;
+            for (int a = 0; a < nfe; a++) {
+                const bool bd = bnd[a];
+                const double *xu = bd ? fbu : fhu, *xuh = bd ? fbuh : fhuh;
+                __syncthreads();
+                // this face's flux Jacobians: xa[g + ngf*(j + nch*c)], xha[g + ngf*(j + nch*kc)]
+                for (int i = tid; i < ngf * nch * (nc + nch); i += nth) {
+                    const int g = i % ngf, r = i / ngf, jj = r % nch, c = r / nch, pp = g + ngf * a;
+                    if (c < nc) xa[i] = xu[pp + np * (jj + nch * c)];
+                    else xha[g + ngf * (jj + nch * (c - nc))] = xuh[pp + np * (jj + nch * (c - nc))];
+                }
+                __syncthreads();
+                for (int ct = wave; ct < nct; ct += nwave) {
+                    const int colb = ct * 16 + lr;              // this lane's B column
+                    const bool vb = colb < ncol;
+                    const int kcb = vb ? colb / (npf * nch) : 0, rb = vb ? colb % (npf * nch) : 0;
+                    const int kkb = rb / nch, jb = rb % nch;
+                    f64x4 acc[NT];
+#pragma unroll
+                    for (int t = 0; t < NT; t++) acc[t] = (f64x4){0.0, 0.0, 0.0, 0.0};
+                    for (int g0 = 0; g0 < ngf; g0 += 4) {
+                        const int g = g0 + lk, pp = g + ngf * a;
+                        const bool vg = g < ngf;
+                        const double wgt = (vb && vg) ? SFG(kkb, g, 0) * jacf[pp] : 0.0;
+                        const int gi = vg ? g : 0;
+                        // u part: K with xu, H with xuh
+                        {
+                            const double bk = wgt * xa[gi + ngf * (jb + nch * kcb)];
+                            const double bh = wgt * xha[gi + ngf * (jb + nch * kcb)];
+#pragma unroll
+                            for (int t = 0; t < TD; t++) {
+                                const int cc = t * 16 + lr;
+                                double av = 0.0;
+                                if (cc < npv && vg) {
+                                    const int k = INVP(cc, a);
+                                    if (k >= 0) av = SFT(g, k, 0);
+                                }
+                                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bk, acc[t], 0, 0, 0);
+                            }
+#pragma unroll
+                            for (int t = TD; t < NT; t++) {
+                                const int l = (t - TD) * 16 + lr;
+                                const double av = (l < ndf && vg && l / npf == a) ? SFT(g, l % npf, 0) : 0.0;
+                                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bh, acc[t], 0, 0, 0);
+                            }
+                        }
+                        for (int x = 0; x < ND; x++) {
+                            const double bq = wgt * xa[gi + ngf * (jb + nch * (ncu * (x + 1) + kcb))];
+#pragma unroll
+                            for (int t = 0; t < TD; t++) {
+                                const int cc = t * 16 + lr;
+                                const double av = (cc < npv && vg) ? shapMiCf[cc + npv * (pp + np * x)] : 0.0;
+                                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bq, acc[t], 0, 0, 0);
+                            }
+#pragma unroll
+                            for (int t = TD; t < NT; t++) {
+                                const int l = (t - TD) * 16 + lr;
+                                const double av = (l < ndf && vg) ? -shapMiEf[l + ndf * (pp + np * x)] : 0.0;
+                                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bq, acc[t], 0, 0, 0);
+                            }
+                        }
+                    }
+                    // result: lane holds C(row = l/16 + 4v, col = l%16) of each tile
+                    const int colc = ct * 16 + lr;
+                    if (colc < ncol) {
+                        const int kcc = colc / (npf * nch), rc = colc % (npf * nch);
+                        const long rowK = (long)(a * npf) * nch + rc;     // (a, kk, j) row of K and H
+#pragma unroll
+                        for (int t = 0; t < NT; t++)
+#pragma unroll
+                            for (int v = 0; v < 4; v++) {
+                                const int rr = t < TD ? t * 16 + lk + 4 * v : (t - TD) * 16 + lk + 4 * v;
+                                if (t < TD) {
+                                    if (rr < npv) Km[e * (long)nb * na + (long)nb * (kcc * npv + rr) + rowK] = acc[t][v];
+                                } else {
+                                    if (rr < ndf) Hx[e * (nH + nb) + (long)nb * (rr * nch + kcc) + rowK] = acc[t][v];
+                                }
+                            }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---------------- P9: Ru, Rh ----------------
+        {
             const long nH = (long)nb * nb;
             double ru2loc = 0.0;
-            for (int col = tid; col < nKc + nHc + na + nb; col += nth) {
-                if (col < nKc + nHc) {
-                    const bool isK = col < nKc;
-                    int a, j, kc, cc;
-                    if (isK) { cc = col % npv; int r = col / npv; kc = r % ncu; r /= ncu; j = r % nch; a = r / nch; }
-                    else { int c2 = col - nKc; cc = c2 % ndf; int r = c2 / ndf; kc = r % nch; r /= nch; j = r % nch; a = r / nch; }
-                    const bool bd = bnd[a];
-                    const double *xu = bd ? fbu : fhu, *xuh = bd ? fbuh : fhuh;
-                    double acc[NPF];
-#pragma unroll
-                    for (int k = 0; k < NPF; k++) acc[k] = 0.0;
-                    for (int g = 0; g < ngf; g++) {
-                        const int pp = g + ngf * a;
-                        double z;
-                        if (isK) {
-                            int k = INVP(cc, a);
-                            z = k >= 0 ? xu[pp + np * (j + nch * kc)] * SFT(g, k, 0) : 0.0;
-#pragma unroll
-                            for (int x = 0; x < ND; x++)
-                                z += xu[pp + np * (j + nch * (ncu * (x + 1) + kc))] * shapMiCf[cc + npv * (pp + np * x)];
-                        } else {
-                            z = (cc / npf == a) ? xuh[pp + np * (j + nch * kc)] * SFT(g, cc % npf, 0) : 0.0;
-#pragma unroll
-                            for (int x = 0; x < ND; x++)
-                                z -= xu[pp + np * (j + nch * (ncu * (x + 1) + kc))] * shapMiEf[cc + ndf * (pp + np * x)];
-                        }
-                        const double jz = jacf[pp] * z;
-#pragma unroll
-                        for (int k = 0; k < NPF; k++) acc[k] += m.shapfg[k + npf * g] * jz;
-                    }
-                    if (isK) {
-                        double *Kc = Km + e * (long)nb * na + (long)nb * (kc * npv + cc) + (a * npf) * nch + j;
-#pragma unroll
-                        for (int k = 0; k < NPF; k++) Kc[k * nch] = acc[k];
-                    } else {
-                        double *Hc = Hx + e * (nH + nb) + (long)nb * (cc * nch + kc) + (a * npf) * nch + j;
-#pragma unroll
-                        for (int k = 0; k < NPF; k++) Hc[k * nch] = acc[k];
-                    }
-                } else if (col < nKc + nHc + na) {
-                    int row = col - nKc - nHc, j = row / npv, ii = row % npv;
+            for (int col = tid; col < na + nb; col += nth) {
+                if (col < na) {
+                    int row = col, j = row / npv, ii = row % npv;
                     double s = 0;
                     for (int g = 0; g < ngv; g++) {
                         if (ph.tdep) {
@@ -504,7 +569,7 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                     Fx[e * (long)na * ldx + (long)row * ldx + nb] = s;
                     ru2loc += s * s;
                 } else {
-                    int row = col - nKc - nHc - na, j = row % nch, tr = row / nch, a = tr / npf, kk = tr % npf;
+                    int row = col - na, j = row % nch, tr = row / nch, a = tr / npf, kk = tr % npf;
                     const double *fx = bnd[a] ? fb : fh;
                     double s = 0;
                     for (int g = 0; g < ngf; g++) {
@@ -886,7 +951,8 @@ size_t assemble_lds_bytes(const Dims &d)
 {
     const size_t nc = d.nc, npv = d.npv, ngv = d.ngv, np = d.np, nd = d.nd;
     const size_t svt = npv * ngv + (npv > nc ? npv : nc) * ngv * nd;
-    const size_t mreg = (npv + ngv + np + 2) * npv > npv * ngv * nd ? (npv + ngv + np + 2) * npv : npv * ngv * nd;
+    size_t mreg = (npv + ngv + np + 2) * npv > npv * ngv * nd ? (npv + ngv + np + 2) * npv : npv * ngv * nd;
+    if ((size_t)d.ngf * d.nch * (nc + d.nch) > mreg) mreg = (size_t)d.ngf * d.nch * (nc + d.nch);
     size_t n = svt + npv * ngv * d.nd1 + 2 * (size_t)d.npf * d.ngf * nd + npv * nd + npv * nc + (size_t)d.ndf * d.nch +
                npv * d.ncu + ngv + ngv * nd * nd + np + np * nd + mreg + npv * ngv * nd + (size_t)d.ndf * ngv * nd +
                ngv * nc + np * nc + np * d.nch + ASM_THREADS;
