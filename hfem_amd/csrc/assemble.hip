@@ -126,6 +126,12 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
         const double *f = FB + FL.f, *fu = FB + FL.fu, *fh = FB + FL.fh, *fhu = FB + FL.fhu, *fhuh = FB + FL.fhuh;
         const double *fb = FB + FL.fb, *fbu = FB + FL.fbu, *fbuh = FB + FL.fbuh;
         // ---------------- P0: gather ----------------
+        // the derivative part of shapvt held the previous element's fu in P8
+        if (t != (int)blockIdx.x)
+            for (int i = npv * ngv + tid; i < npv * ngv * nd1; i += nth) {
+                const int nn = i % npv, r = i / npv, g = r % ngv, dd = r / ngv;
+                svt[i] = m.shapvt[g + ngv * (nn + npv * dd)];
+            }
         for (int i = tid; i < npv * ND; i += nth) pn[i] = dgnodes[e * npv * ND + i];
         for (int i = tid; i < npv * nc; i += nth) udg[i] = UDG[e * npv * nc + i];
         for (int i = tid; i < ndf * nch; i += nth) {

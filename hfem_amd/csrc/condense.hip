@@ -37,81 +37,80 @@ __device__ __forceinline__ f32x4 mfma32(float a, float b, f32x4 c)
 }
 
 // ---------------------------------------------------------------------------
-// Gauss-Jordan inverse in registers.  Thread (ti, tj) = (tid % 16, tid / 16)
-// owns rows ti + 16 r and columns tj + 16 c (r, c < R).  On exit a[r][c] holds
-// inv(P D) and perm[] the composed row permutation, so that
-// inv(D)[:, perm[m]] = a[:, m].  T is the element type of the elimination.
+// Gauss-Jordan inverse in registers with partial pivoting and implicit row
+// exchanges.  Thread (ti, tj) = (tid % 16, tid / 16) owns rows ti + 16 r and
+// columns tj + 16 c (r, c < R).  Step j takes as pivot the row p_j of largest
+// |a(i, j)| among the rows not yet used (lowest index on ties) and applies the
+// exchange step to every row; rows are never moved.  On exit a holds G with
+//   inv(D)[label[i], pof[c]] = G[i][c],   pof[j] = p_j, label[p_j] = j.
+// The column loop is unrolled by 16-column tile so that the owner of column j
+// reads a compile-time register index.  T is the element type.
 // ---------------------------------------------------------------------------
 template <int R, typename T>
-__device__ __forceinline__ int gauss_jordan(int na, T (&a)[R][R], T *colbuf, T *rowJ, T *rowP, int *perm)
+__device__ __forceinline__ int gauss_jordan(int na, T (&a)[R][R], T *colbuf, T *rowbuf, int *pof, int *label)
 {
     const int tid = threadIdx.x, ti = tid & 15, tj = tid >> 4, lane = tid & 63;
-    int sing = 0;
-    for (int i = tid; i < na; i += blockDim.x) perm[i] = i;
+    for (int i = tid; i < na; i += blockDim.x) label[i] = -1;
     __syncthreads();
-    for (int j = 0; j < na; j++) {
-        const int b = j & 1;
-        T *cb = colbuf + b * 192, *rj = rowJ + b * 192, *rp = rowP + b * 192;
-        if (tj == (j & 15)) {
+    int sing = 0;
 #pragma unroll
-            for (int c = 0; c < R; c++)
-                if (c == (j >> 4))
+    for (int c0 = 0; c0 < R; c0++) {
+        for (int jj = 0; jj < 16; jj++) {
+            const int j = 16 * c0 + jj;
+            if (j >= na) break;
+            T *cb = colbuf + (j & 1) * 192, *rb = rowbuf + (j & 1) * 192;
+            if (tj == jj) {
 #pragma unroll
-                    for (int r = 0; r < R; r++) {
-                        int i = ti + 16 * r;
-                        if (i < na) cb[i] = a[r][c];
-                    }
-        }
-        __syncthreads();
-        // pivot: first index of max |cb[i]|, i >= j (every wave computes it)
-        T best = (T)-1;
-        int bi = na;
-        for (int i = j + lane; i < na; i += 64) {
-            T v = fabs(cb[i]);
-            if (v > best) { best = v; bi = i; }
-        }
+                for (int r = 0; r < R; r++) {
+                    const int i = ti + 16 * r;
+                    if (i < na) cb[i] = a[r][c0];
+                }
+            }
+            __syncthreads();
+            // pivot among the rows not yet used (every wave computes it)
+            T best = (T)-1;
+            int bi = na;
+            for (int i = lane; i < na; i += 64) {
+                if (label[i] >= 0) continue;
+                const T v = fabs(cb[i]);
+                if (v > best) { best = v; bi = i; }
+            }
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            T ov = __shfl_xor(best, off);
-            int oi = __shfl_xor(bi, off);
-            if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
-        }
-        const int p = bi;
-        // rows j and p through LDS
-        if (ti == (j & 15) || ti == (p & 15)) {
+            for (int off = 32; off > 0; off >>= 1) {
+                const T ov = __shfl_xor(best, off);
+                const int oi = __shfl_xor(bi, off);
+                if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+            }
+            const int p = bi;
+            if (ti == (p & 15)) {
+                const int rp = p >> 4;
+#pragma unroll
+                for (int c = 0; c < R; c++) {
+                    T v = (T)0;
+#pragma unroll
+                    for (int r = 0; r < R; r++) v = (r == rp) ? a[r][c] : v;
+                    const int k = tj + 16 * c;
+                    if (k < na) rb[k] = v;
+                }
+            }
+            if (tid == 0) { label[p] = j; pof[j] = p; }
+            __syncthreads();
+            const T pv = cb[p];
+            if (pv == (T)0 && !sing) sing = j + 1;
+            const T pinv = pv == (T)0 ? (T)0 : (T)1 / pv;
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                int i = ti + 16 * r;
-                if (i == j || i == p)
+                const int i = ti + 16 * r;
+                const T f = cb[i < na ? i : 0];
 #pragma unroll
-                    for (int c = 0; c < R; c++) {
-                        int k = tj + 16 * c;
-                        if (k < na) {
-                            if (i == j) rj[k] = a[r][c];
-                            if (i == p) rp[k] = a[r][c];
-                        }
-                    }
-            }
-        }
-        if (tid == 0 && p != j) { int t = perm[j]; perm[j] = perm[p]; perm[p] = t; }
-        __syncthreads();
-        const T pv = cb[p];
-        if (pv == (T)0) { if (!sing) sing = j + 1; }
-        const T rinv = pv == (T)0 ? (T)0 : (T)1 / pv;
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            const int i = ti + 16 * r;
-            // column-j value of row i after the swap
-            const T f = (i == p) ? cb[j] : cb[i < na ? i : 0];
-#pragma unroll
-            for (int c = 0; c < R; c++) {
-                const int k = tj + 16 * c;
-                if (i >= na || k >= na) continue;
-                const T prow = rp[k];                       // pivot row (old row p)
-                if (i == j) a[r][c] = (k == j) ? rinv : prow * rinv;
-                else {
-                    const T src = (i == p) ? rj[k] : a[r][c];
-                    a[r][c] = (k == j) ? -f * rinv : src - f * (prow * rinv);
+                for (int c = 0; c < R; c++) {
+                    const int k = tj + 16 * c;
+                    const T prow = rb[k < na ? k : 0] * pinv;
+                    const bool kj = (c == c0) && (tj == jj);
+                    T v;
+                    if (i == p) v = kj ? pinv : prow;
+                    else v = kj ? -f * pinv : a[r][c] - f * prow;
+                    a[r][c] = v;
                 }
             }
         }
@@ -174,8 +173,8 @@ __host__ __device__ inline int dinv_ld(int na)
     return (l % 32 == 0) ? l + 16 : l;
 }
 
-// LDS bytes: three double-buffered exchange vectors of 192, the permutation,
-// and (optionally) inv(D).
+// LDS bytes: the double-buffered pivot column and row (192 each), the pivot
+// order and row labels, and (optionally) inv(D).
 size_t condense_lds_bytes(int na, bool dinv_in_lds, int elt)
 {
     size_t head = (size_t)6 * 192 * elt + 192 * sizeof(int);
@@ -229,8 +228,8 @@ __global__ void __launch_bounds__(256)
 dinv_kernel(int na, int e0, int n, const double *__restrict__ Dm, double *__restrict__ Dinv, int *__restrict__ info)
 {
     extern __shared__ double lds[];
-    double *colbuf = lds, *rowJ = lds + 2 * 192, *rowP = lds + 4 * 192;
-    int *perm = (int *)(lds + 6 * 192);
+    double *colbuf = lds, *rowbuf = lds + 2 * 192;
+    int *pof = (int *)(lds + 4 * 192), *label = pof + 192;
     const int tid = threadIdx.x, ti = tid & 15, tj = tid >> 4;
     for (int t = blockIdx.x; t < n; t += gridDim.x) {
         const long e = e0 + t;
@@ -243,7 +242,7 @@ dinv_kernel(int na, int e0, int n, const double *__restrict__ Dm, double *__rest
                 int i = ti + 16 * r, k = tj + 16 * c;
                 a[r][c] = (i < na && k < na) ? D[i + (long)na * k] : 0.0;
             }
-        int sing = gauss_jordan<R, double>(na, a, colbuf, rowJ, rowP, perm);
+        int sing = gauss_jordan<R, double>(na, a, colbuf, rowbuf, pof, label);
         if (tid == 0 && info) info[e] = sing;
         double *Mi = Dinv + e * (long)na * na;
 #pragma unroll
@@ -251,7 +250,7 @@ dinv_kernel(int na, int e0, int n, const double *__restrict__ Dm, double *__rest
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 int i = ti + 16 * r, m = tj + 16 * c;
-                if (i < na && m < na) Mi[i + (long)na * perm[m]] = a[r][c];
+                if (i < na && m < na) Mi[label[i] + (long)na * pof[m]] = a[r][c];
             }
         __syncthreads();
     }
@@ -268,8 +267,8 @@ condense_mixed_kernel(int na, int nb, int ldx, int e0, int n, const double *__re
 {
     extern __shared__ double lds[];
     float *fl = (float *)lds;
-    float *colbuf = fl, *rowJ = fl + 2 * 192, *rowP = fl + 4 * 192;
-    int *perm = (int *)(fl + 6 * 192);
+    float *colbuf = fl, *rowbuf = fl + 2 * 192;
+    int *pof = (int *)(fl + 4 * 192), *label = pof + 192;
     float *Ml = (float *)((char *)lds + lds_head(sizeof(float)));
     const int ldl = dinv_ld(na);
     const int tid = threadIdx.x, ti = tid & 15, tj = tid >> 4;
@@ -287,14 +286,14 @@ condense_mixed_kernel(int na, int nb, int ldx, int e0, int n, const double *__re
                 int i = ti + 16 * r, k = tj + 16 * c;
                 a[r][c] = (i < na && k < na) ? (float)D[i + (long)na * k] : 0.f;
             }
-        int sing = gauss_jordan<R, float>(na, a, colbuf, rowJ, rowP, perm);
+        int sing = gauss_jordan<R, float>(na, a, colbuf, rowbuf, pof, label);
         if (tid == 0 && info) info[e] = sing;
 #pragma unroll
         for (int c = 0; c < R; c++)
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 int i = ti + 16 * r, m = tj + 16 * c;
-                if (i < na && m < na) Ml[i + ldl * perm[m]] = a[r][c];
+                if (i < na && m < na) Ml[label[i] + ldl * pof[m]] = a[r][c];
             }
         __syncthreads();
         const double *F = Fx + e * (long)na * ldx;
