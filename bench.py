@@ -114,9 +114,13 @@ def cpu_baseline(prob, seconds=10.0):
     kind = "reference"
     try:
         ref = O.Reference(prob)
-        probe = max(ref.time(20, impl=1) / 20, 1e-5)
-        n = int(min(max(seconds / probe, 50), 50000))
-        t = ref.time(n, impl=1)
+        ref.time(20, impl=1)                              # warm-up
+        n, t = 50, 0.0
+        while True:                                       # grow the sample to ~`seconds` of work
+            t = ref.time(n, impl=1)
+            if t >= seconds or n >= 50000:
+                break
+            n = int(min(50000, max(2 * n, n * seconds / max(t, 1e-3) * 1.1)))
     except Exception:
         kind = "port"
         orc = O.Oracle(prob)
