@@ -341,89 +341,94 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                 }
                 __syncthreads();
                 for (int rt = wave; rt < nrt; rt += nwave) {
-                    // this lane's A row and its accumulator rows
+                    // this lane's A row and its accumulator rows.  Operands are read at
+                    // clamped indices and zeroed by selects, never under a branch, so that
+                    // all loads of a k-step issue together; only zeros along the k index
+                    // matter (rows and columns past the end are never stored).
                     const int ra = rt * 16 + lr;
                     const bool va = ra < ncu * npv;
                     const int kca = va ? ra / npv : 0, ia = va ? ra % npv : 0;
+                    const double tdj = (ph.tdep && kca == j) ? ph.fc_u : 0.0;
+                    int ccl[TD], lcl[TF];
+#pragma unroll
+                    for (int t = 0; t < TD; t++) ccl[t] = min(t * 16 + lr, npv - 1);
+#pragma unroll
+                    for (int t = 0; t < TF; t++) lcl[t] = min(t * 16 + lr, ndf - 1);
                     f64x4 acc[NT];
 #pragma unroll
                     for (int t = 0; t < NT; t++) acc[t] = (f64x4){0.0, 0.0, 0.0, 0.0};
-                    for (int xq = 0; xq < nd1; xq++) {
+                    // volume points, u columns: B = shapvt
+                    for (int g0 = 0; g0 < ngv; g0 += 4) {
+                        const int g = min(g0 + lk, ngv - 1);
+                        double sa = 0.0;
+#pragma unroll
+                        for (int x = 0; x < ND; x++) sa += fuj[g + ngv * (x + ND * kca)] * G[ia + npv * (g + ngv * x)];
+                        double a = SVG(ia, g, 0) * tdj * jac[g] - sa;
+                        a = g0 + lk < ngv ? a : 0.0;
+                        double b[TD];
+#pragma unroll
+                        for (int t = 0; t < TD; t++) b[t] = SVT(g, ccl[t], 0);
+#pragma unroll
+                        for (int t = 0; t < TD; t++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[t], acc[t], 0, 0, 0);
+                    }
+                    // volume points, q columns: B = M^{-1}C and M^{-1}E traces
+                    for (int xq = 1; xq < nd1; xq++) {
+                        const int c = xq * ncu + kca;
                         for (int g0 = 0; g0 < ngv; g0 += 4) {
-                            const int g = g0 + lk;
-                            double a = 0.0;
-                            if (va && g < ngv) {
-                                const int c = xq * ncu + kca;
-                                double sa = 0.0;
+                            const int g = min(g0 + lk, ngv - 1);
+                            double sa = 0.0;
 #pragma unroll
-                                for (int x = 0; x < ND; x++) sa += fuj[g + ngv * (x + ND * c)] * G[ia + npv * (g + ngv * x)];
-                                a = -sa;
-                                if (xq == 0 && ph.tdep && kca == j) a += SVG(ia, g, 0) * ph.fc_u * jac[g];
-                            }
+                            for (int x = 0; x < ND; x++) sa += fuj[g + ngv * (x + ND * c)] * G[ia + npv * (g + ngv * x)];
+                            const double a = g0 + lk < ngv ? -sa : 0.0;
+                            double b[NT];
 #pragma unroll
-                            for (int t = 0; t < TD; t++) {
-                                const int cc = t * 16 + lr;
-                                double b = 0.0;
-                                if (cc < npv && g < ngv)
-                                    b = xq == 0 ? SVT(g, cc, 0) : shapMiC[cc + npv * (g + ngv * (xq - 1))];
-                                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
-                            }
-                            if (xq > 0) {
+                            for (int t = 0; t < TD; t++) b[t] = shapMiC[ccl[t] + npv * (g + ngv * (xq - 1))];
 #pragma unroll
-                                for (int t = TD; t < NT; t++) {
-                                    const int l = (t - TD) * 16 + lr;
-                                    const double b = (l < ndf && g < ngv) ? shapMiE[l + ndf * (g + ngv * (xq - 1))] : 0.0;
-                                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
-                                }
-                            }
+                            for (int t = 0; t < TF; t++) b[TD + t] = shapMiE[lcl[t] + ndf * (g + ngv * (xq - 1))];
+#pragma unroll
+                            for (int t = 0; t < NT; t++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[t], acc[t], 0, 0, 0);
                         }
                     }
                     // face points
                     for (int p0 = 0; p0 < np; p0 += 4) {
-                        const int pp = p0 + lk;
-                        const bool vp = pp < np;
-                        const int a = vp ? pp / ngf : 0, gf = vp ? pp % ngf : 0;
-                        double af = 0.0;
-                        if (va && vp) {
-                            const int k = INVP(ia, a);
-                            if (k >= 0) af = SFG(k, gf, 0) * jacf[pp];
+                        const int pp = min(p0 + lk, np - 1);
+                        const int a = pp / ngf, gf = pp % ngf;
+                        const int ki = INVP(ia, a);
+                        double af = SFG(max(ki, 0), gf, 0) * jacf[pp];
+                        af = (p0 + lk < np && ki >= 0) ? af : 0.0;
+                        double bu[NT];
+#pragma unroll
+                        for (int t = 0; t < TD; t++) {
+                            const int k = INVP(ccl[t], a);
+                            const double v = SFT(gf, max(k, 0), 0);
+                            bu[t] = k >= 0 ? v : 0.0;
+                        }
+#pragma unroll
+                        for (int t = 0; t < TF; t++) {
+                            const double v = SFT(gf, lcl[t] % npf, 0);
+                            bu[TD + t] = lcl[t] / npf == a ? v : 0.0;
+                        }
+                        double bq[ND][NT];
+#pragma unroll
+                        for (int x = 0; x < ND; x++) {
+#pragma unroll
+                            for (int t = 0; t < TD; t++) bq[x][t] = shapMiCf[ccl[t] + npv * (pp + np * x)];
+#pragma unroll
+                            for (int t = 0; t < TF; t++) bq[x][TD + t] = -shapMiEf[lcl[t] + ndf * (pp + np * x)];
                         }
                         // u part: D with fhu, F with fhuh
-                        {
-                            const double ad = af * fhuj[(vp ? pp : 0) + np * kca];
-                            const double ah = af * fhuhj[(vp ? pp : 0) + np * kca];
+                        const double ad = af * fhuj[pp + np * kca];
+                        const double ah = af * fhuhj[pp + np * kca];
 #pragma unroll
-                            for (int t = 0; t < TD; t++) {
-                                const int cc = t * 16 + lr;
-                                double b = 0.0;
-                                if (cc < npv && vp) {
-                                    const int k = INVP(cc, a);
-                                    if (k >= 0) b = SFT(gf, k, 0);
-                                }
-                                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, b, acc[t], 0, 0, 0);
-                            }
+                        for (int t = 0; t < TD; t++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, bu[t], acc[t], 0, 0, 0);
 #pragma unroll
-                            for (int t = TD; t < NT; t++) {
-                                const int l = (t - TD) * 16 + lr;
-                                const double b = (l < ndf && vp && l / npf == a) ? SFT(gf, l % npf, 0) : 0.0;
-                                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ah, b, acc[t], 0, 0, 0);
-                            }
-                        }
+                        for (int t = TD; t < NT; t++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ah, bu[t], acc[t], 0, 0, 0);
                         // q parts
+#pragma unroll
                         for (int x = 0; x < ND; x++) {
-                            const double aq = af * fhuj[(vp ? pp : 0) + np * (ncu * (x + 1) + kca)];
+                            const double aq = af * fhuj[pp + np * (ncu * (x + 1) + kca)];
 #pragma unroll
-                            for (int t = 0; t < TD; t++) {
-                                const int cc = t * 16 + lr;
-                                const double b = (cc < npv && vp) ? shapMiCf[cc + npv * (pp + np * x)] : 0.0;
-                                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(aq, b, acc[t], 0, 0, 0);
-                            }
-#pragma unroll
-                            for (int t = TD; t < NT; t++) {
-                                const int l = (t - TD) * 16 + lr;
-                                const double b = (l < ndf && vp) ? -shapMiEf[l + ndf * (pp + np * x)] : 0.0;
-                                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(aq, b, acc[t], 0, 0, 0);
-                            }
+                            for (int t = 0; t < NT; t++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(aq, bq[x][t], acc[t], 0, 0, 0);
                         }
                     }
 #pragma unroll
@@ -476,49 +481,54 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                     const bool vb = colb < ncol;
                     const int kcb = vb ? colb / (npf * nch) : 0, rb = vb ? colb % (npf * nch) : 0;
                     const int kkb = rb / nch, jb = rb % nch;
+                    // A rows of this lane (clamped; rows past the end are never stored) and
+                    // their face-node positions on face a
+                    int ccl[TD], kcl[TD], lcl[TF];
+#pragma unroll
+                    for (int t = 0; t < TD; t++) {
+                        ccl[t] = min(t * 16 + lr, npv - 1);
+                        kcl[t] = INVP(ccl[t], a);
+                    }
+#pragma unroll
+                    for (int t = 0; t < TF; t++) lcl[t] = min(t * 16 + lr, ndf - 1);
                     f64x4 acc[NT];
 #pragma unroll
                     for (int t = 0; t < NT; t++) acc[t] = (f64x4){0.0, 0.0, 0.0, 0.0};
                     for (int g0 = 0; g0 < ngf; g0 += 4) {
-                        const int g = g0 + lk, pp = g + ngf * a;
-                        const bool vg = g < ngf;
-                        const double wgt = (vb && vg) ? SFG(kkb, g, 0) * jacf[pp] : 0.0;
-                        const int gi = vg ? g : 0;
-                        // u part: K with xu, H with xuh
-                        {
-                            const double bk = wgt * xa[gi + ngf * (jb + nch * kcb)];
-                            const double bh = wgt * xha[gi + ngf * (jb + nch * kcb)];
+                        const int g = min(g0 + lk, ngf - 1), pp = g + ngf * a;
+                        double wgt = SFG(kkb, g, 0) * jacf[pp];
+                        wgt = g0 + lk < ngf ? wgt : 0.0;
+                        double au[NT];
 #pragma unroll
-                            for (int t = 0; t < TD; t++) {
-                                const int cc = t * 16 + lr;
-                                double av = 0.0;
-                                if (cc < npv && vg) {
-                                    const int k = INVP(cc, a);
-                                    if (k >= 0) av = SFT(g, k, 0);
-                                }
-                                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bk, acc[t], 0, 0, 0);
-                            }
-#pragma unroll
-                            for (int t = TD; t < NT; t++) {
-                                const int l = (t - TD) * 16 + lr;
-                                const double av = (l < ndf && vg && l / npf == a) ? SFT(g, l % npf, 0) : 0.0;
-                                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bh, acc[t], 0, 0, 0);
-                            }
+                        for (int t = 0; t < TD; t++) {
+                            const double v = SFT(g, max(kcl[t], 0), 0);
+                            au[t] = kcl[t] >= 0 ? v : 0.0;
                         }
+#pragma unroll
+                        for (int t = 0; t < TF; t++) {
+                            const double v = SFT(g, lcl[t] % npf, 0);
+                            au[TD + t] = lcl[t] / npf == a ? v : 0.0;
+                        }
+                        double aq[ND][NT];
+#pragma unroll
                         for (int x = 0; x < ND; x++) {
-                            const double bq = wgt * xa[gi + ngf * (jb + nch * (ncu * (x + 1) + kcb))];
 #pragma unroll
-                            for (int t = 0; t < TD; t++) {
-                                const int cc = t * 16 + lr;
-                                const double av = (cc < npv && vg) ? shapMiCf[cc + npv * (pp + np * x)] : 0.0;
-                                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bq, acc[t], 0, 0, 0);
-                            }
+                            for (int t = 0; t < TD; t++) aq[x][t] = shapMiCf[ccl[t] + npv * (pp + np * x)];
 #pragma unroll
-                            for (int t = TD; t < NT; t++) {
-                                const int l = (t - TD) * 16 + lr;
-                                const double av = (l < ndf && vg) ? -shapMiEf[l + ndf * (pp + np * x)] : 0.0;
-                                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bq, acc[t], 0, 0, 0);
-                            }
+                            for (int t = 0; t < TF; t++) aq[x][TD + t] = -shapMiEf[lcl[t] + ndf * (pp + np * x)];
+                        }
+                        // u part: K with xu, H with xuh
+                        const double bk = wgt * xa[g + ngf * (jb + nch * kcb)];
+                        const double bh = wgt * xha[g + ngf * (jb + nch * kcb)];
+#pragma unroll
+                        for (int t = 0; t < TD; t++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(au[t], bk, acc[t], 0, 0, 0);
+#pragma unroll
+                        for (int t = TD; t < NT; t++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(au[t], bh, acc[t], 0, 0, 0);
+#pragma unroll
+                        for (int x = 0; x < ND; x++) {
+                            const double bq = wgt * xa[g + ngf * (jb + nch * (ncu * (x + 1) + kcb))];
+#pragma unroll
+                            for (int t = 0; t < NT; t++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(aq[x][t], bq, acc[t], 0, 0, 0);
                         }
                     }
                     // result: lane holds C(row = l/16 + 4v, col = l%16) of each tile
