@@ -69,10 +69,8 @@ extern "C" int hfem_create(hfem_ctx **out, const hfem_dims *hd, int device)
     ALLOC(ru2, ne);
     ALLOC(info, ne);
     ALLOC(partial, 1024);
-    c->sstride = scratch_layout(d).total;
     c->grid_asm = (int)std::min<long>(ne, 512);
     c->grid_cond = (int)std::min<long>(ne, 1024);
-    ALLOC(scratch, (long)c->grid_asm * c->sstride);
     c->fstride = flux_layout(d).total;
     c->chunk = (int)std::min<long>(ne, 4096);
     ALLOC(fluxbuf, (long)c->chunk * c->fstride);
@@ -118,7 +116,7 @@ extern "C" int hfem_destroy(hfem_ctx *c)
     if (c->stream) hipStreamSynchronize(c->stream);
     void *ptrs[] = {c->shapvt, c->shapvg, c->shapft, c->shapfg, c->perm, c->invperm, c->dgnodes, c->elcon, c->bf,
                     c->rowstart, c->cols, c->einv, c->f2e, c->blkptr, c->blklist, c->UDG, c->UH, c->SH, c->D, c->Fx,
-                    c->X, c->K, c->Hx, c->Ru, c->Rh, c->ru2, c->info, c->scratch, c->dinv, c->rwork, c->fluxbuf, c->Hg, c->Rg,
+                    c->X, c->K, c->Hx, c->Ru, c->Rh, c->ru2, c->info, c->dinv, c->rwork, c->fluxbuf, c->Hg, c->Rg,
                     c->r, c->partial, c->xv, c->yv, c->sv.d_o2u, c->sv.d_u2o, c->sv.d_lrows, c->sv.d_urows,
                     c->sv.Mg, c->sv.Dinv, c->sv.Cw, c->sv.d_info, c->sv.x, c->sv.V, c->sv.b, c->sv.t, c->sv.dots,
                     c->sv.coef, c->sv.UDGi};
@@ -299,9 +297,9 @@ extern "C" int hfem_assemble(hfem_ctx *c, int e0, int n)
     Master m{c->shapvt, c->shapvg, c->shapft, c->shapfg, c->perm, c->invperm};
     const int grid = std::min(n, c->grid_asm);
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
-    HIPCHK(launch_assemble(c->d, m, c->ph, c->dgnodes, c->UDG, c->UH, c->SH, c->elcon, c->bf, e0, n, c->scratch,
-                           c->sstride, grid, c->ldx, c->fluxbuf, c->fstride, c->chunk, c->Ru, c->D, c->Fx, c->K, c->Hx,
-                           c->Rh, c->ru2, c->stream));
+    HIPCHK(launch_assemble(c->d, m, c->ph, c->dgnodes, c->UDG, c->UH, c->SH, c->elcon, c->bf, e0, n, grid, c->ldx,
+                           c->fluxbuf, c->fstride, c->chunk, c->Ru, c->D, c->Fx, c->K, c->Hx, c->Rh, c->ru2,
+                           c->stream));
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
     return 0;
 }

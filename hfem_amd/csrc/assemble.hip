@@ -5,23 +5,21 @@
 // and the faceint/elementint outputs).  D and K use the reference's column-major
 // layouts; F is written row-major into Fx (na x ldx) with Ru as column nb, and H
 // column-major into Hx (nb x (nb+1)) with Rh as column nb -- the layouts the
-// condensation kernel streams through its MFMA operands.
+// condensation GEMMs stream.  Everything an element needs lives in LDS; global
+// memory holds only the inputs, the pointwise fluxes and the outputs.
 //
 // Phases (barriers between them; inside a phase threads own disjoint outputs):
-//   P0 element data and the master tables in LDS  P5 traces of M^{-1}C, M^{-1}E;
-//   P1 volume/face geometry                          solution at volume points
-//   P2 q operators M, C, E (qint)                  P7 wrl = -[s_u jac ; f_u . Xx]
-//   P3 M^{-1} (Gauss-Jordan)
-//   P4 Q = [shapvt; face traces] M^{-1}, so that P5 forms the traces of M^{-1}C
-//      and M^{-1}E as Q C and Q E without solving per column
-//   (the pointwise fluxes and their Jacobians come from flux_kernel, launched
-//   before this kernel on the same element chunk)
-//   P8 element rows: one thread per column of D or F, all NPV rows in registers;
-//      the q-folded integrand (elementint :510-549, faceint :962-1072) is formed
-//      on the fly and contracted with the test functions, volume then faces
-//      (faces containing the row node: the gather form of faceint's scatter)
-//   P9 trace rows: one thread per (face, column) of K or H, npf rows in registers;
-//      Ru and Rh entries.
+//   P0 element data (the master tables are loaded once per workgroup)
+//   P1 volume/face geometry
+//   P2 q operators M, C, E (qint) and the solution at volume points
+//   P3 M^{-1} by Gauss-Jordan
+//   P4 Q = [shapvt; face traces] M^{-1}: the point values of M^{-1}C and M^{-1}E
+//      are Q C and Q E, never formed as tables
+//   P8 element rows of D and F on the FP64 MFMA, one row block j at a time
+//   P9 Ru and Rh
+//   P9k trace rows of K and H on the FP64 MFMA, one face at a time
+// (the pointwise fluxes and their Jacobians come from flux_kernel, launched before
+// this kernel on the same element chunk)
 #include "hfem_device.h"
 
 #include <algorithm>
@@ -29,76 +27,108 @@
 namespace hfem {
 
 constexpr int ASM_THREADS = 512;
+constexpr int ASM_LDS_MAX = 160 * 1024;   // LDS per workgroup on gfx950
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-// LDS copies of the master tables, transposed so that the thread-varying index is
-// fastest: svt[n + npv*(g + ngv*dd)], svg[i + npv*(g + ngv*dd)] (as shapvg),
+// LDS layout of the assembly kernel in doubles (the int tables follow).  Master
+// tables are transposed so that the thread-varying index is fastest:
+// svt[n + npv*(g + ngv*dd)], svg[i + npv*(g + ngv*dd)] (as shapvg),
 // sft[k + npf*(g + ngf*dd)], sfg[k + npf*(g + ngf*dd)] (as shapfg).
-struct LdsTables {
-    double *svt, *svg, *sft, *sfg;
-    int *perm, *invperm;
+// The shapvt derivatives are dead after P1; their region then holds G (P2) and
+// fu.Xx of one row block (P8).  The face operands of P9k (flux Jacobians of one
+// face and its point values of M^{-1}C, M^{-1}E) get a region of their own when
+// it fits, else they overlay the shapvt derivatives and shapvg, which P0 then
+// reloads for the next element.
+struct AsmLayout {
+    int svt, svg, sft, sfg, pn, udg, uh, sh, jac, Xx, jacf, nl, M, Qm, rowk, colk, C, E, udgg, fj, fhj, red;
+    int xa, xha, cf, ef;
+    int total, nints, overlay;
 };
+
+__host__ __device__ inline AsmLayout asm_layout(const Dims &d)
+{
+    const int npv = d.npv, ngv = d.ngv, nd = d.nd, nc = d.nc, ncu = d.ncu, nch = d.nch, np = d.np;
+    const int npf = d.npf, ngf = d.ngf, ndf = d.ndf, nd1 = d.nd + 1;
+    AsmLayout L;
+    int o = 0;
+    const int deriv = (npv > nc ? npv : nc) * ngv * nd;
+    L.svt = o; o += npv * ngv + deriv;
+    L.svg = o; o += npv * ngv * nd1;
+    L.sft = o; o += npf * ngf * nd;
+    L.sfg = o; o += npf * ngf * nd;
+    L.pn = o; o += npv * nd;
+    L.udg = o; o += npv * nc;
+    L.uh = o; o += ndf * nch;
+    L.sh = o; o += npv * ncu;
+    L.jac = o; o += ngv;
+    L.Xx = o; o += ngv * nd * nd;
+    L.jacf = o; o += np;
+    L.nl = o; o += np * nd;
+    L.M = o; o += npv * npv;
+    L.Qm = o; o += (ngv + np) * npv;
+    L.rowk = o; o += npv;
+    L.colk = o; o += npv;
+    L.C = o; o += nd * npv * npv;
+    L.E = o; o += nd * npv * ndf;
+    L.udgg = o; o += ngv * nc;
+    // fhu of one row block (P8); the Ru/Rh tables (P9)
+    const int fjn = np * nc, tn = ngv * ncu * nd1 + np * nch;
+    L.fj = o; o += fjn > tn ? fjn : tn;
+    L.fhj = o; o += np * nch;
+    L.red = o; o += 64;
+    L.nints = npf * d.nfe + npv * d.nfe + d.nfe + 8;
+    const int kh = ngf * nch * (nc + nch) + (npv + ndf) * ngf * nd;
+    int base = o;
+    L.overlay = 0;
+    if ((long)(o + kh) * 8 + (long)L.nints * 4 > ASM_LDS_MAX && kh <= deriv + npv * ngv * nd1) {
+        L.overlay = 1;
+        base = L.svt + npv * ngv;
+    } else {
+        o += kh;
+    }
+    L.xa = base;
+    L.xha = L.xa + ngf * nch * nc;
+    L.cf = L.xha + ngf * nch * nch;
+    L.ef = L.cf + npv * ngf * nd;
+    L.total = o;
+    return L;
+}
 
 template <int ND, int NPV, int NPF>
 __global__ void __launch_bounds__(ASM_THREADS)
 assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, const double *__restrict__ UDG,
                 const double *__restrict__ UH, const double *__restrict__ SH, const int *__restrict__ elcon,
-                const int *__restrict__ bf, int e0, int n, double *__restrict__ scratch, long sstride, int ldx,
-                const double *__restrict__ fluxbuf, long fstride, double *__restrict__ Ru,
-                double *__restrict__ Dm, double *__restrict__ Fx, double *__restrict__ Km, double *__restrict__ Hx,
-                double *__restrict__ Rh, double *__restrict__ ru2)
+                const int *__restrict__ bf, int e0, int n, int ldx, const double *__restrict__ fluxbuf, long fstride,
+                double *__restrict__ Ru, double *__restrict__ Dm, double *__restrict__ Fx, double *__restrict__ Km,
+                double *__restrict__ Hx, double *__restrict__ Rh, double *__restrict__ ru2)
 {
     extern __shared__ double lds[];
     const int tid = threadIdx.x, nth = blockDim.x;
+    const int lane = tid & 63, wave = tid >> 6, nwave = nth >> 6, lr = lane & 15, lk = lane >> 4;
     const FluxLayout FL = flux_layout(d);
+    const AsmLayout LY = asm_layout(d);
     constexpr int NCU = ND + 2, NC = NCU * (ND + 1), ND1 = ND + 1;
-    const int npv = NPV, ngv = d.ngv, npf = NPF, ngf = d.ngf, nfe = d.nfe, ndf = d.ndf;
+    // faces per element from the face size (tets: triangular faces, hexes: square faces)
+    constexpr int NFE = ND == 2 ? 3 : ((NPF == 4 || NPF == 9 || NPF == 16) ? 6 : 4);
+    constexpr int NDF = NPF * NFE;
+    // 16-wide MFMA tiles: TD over the npv columns of D and K, TF over the ndf columns
+    // of F and H, KT over the npv nodes contracted in the W products of P8
+    constexpr int TD = (NPV + 15) / 16, TF = (NDF + 15) / 16, NT = TD + TF, KT = (NPV + 15) / 16;
+    const int npv = NPV, ngv = d.ngv, npf = NPF, ngf = d.ngf, nfe = NFE, ndf = NDF;
     const int ncu = NCU, nch = NCU, nc = NC, na = d.na, nb = d.nb, nd1 = ND1, np = d.np;
 
-    // ---- LDS layout ----
-    double *p = lds;
-    // shapvt; its derivative part (dead after P1) later holds fu of one row block
-    double *svt = p; p += npv * ngv + (npv * ND > ND * NC ? npv : NC) * ngv * ND;
-    double *svg = p; p += npv * ngv * nd1;
-    double *sft = p; p += npf * ngf * ND;
-    double *sfg = p; p += npf * ngf * ND;
-    double *pn = p; p += npv * ND;
-    double *udg = p; p += npv * nc;
-    double *uh = p; p += ndf * nch;            // uh[k*ndf + j]
-    double *sh = p; p += npv * ncu;
-    double *jac = p; p += ngv;
-    double *Xx = p; p += ngv * ND * ND;        // Xx(g,x,r) = jac d xi_r / d x_x
-    double *jacf = p; p += np;
-    double *nl = p; p += np * ND;
-    // mass matrix / its inverse, Q and the Gauss-Jordan rows; later G (P8), then the
-    // flux Jacobians of one face (P9k)
-    double *M = p;
-    double *Qm = M + npv * npv;                // [k + npv*r]: rows of shapvt / face traces times M^{-1}
-    double *rowk = Qm + (ngv + np) * npv;
-    double *colk = rowk + npv;
-    double *xa = M, *xha = M + ngf * nch * nc;
-    {
-        int mr = (npv + ngv + np + 2) * npv;
-        if (npv * ngv * ND > mr) mr = npv * ngv * ND;
-        if (ngf * nch * (nc + nch) > mr) mr = ngf * nch * (nc + nch);
-        p += mr;
-    }
-    double *shapMiC = p; p += npv * ngv * ND;  // [n + npv*(g + ngv*x)]
-    double *shapMiE = p; p += ndf * ngv * ND;  // [l + ndf*(g + ngv*x)], includes the minus sign
-    double *udgg = p; p += ngv * nc;           // [g + ngv*c]
-    double *fhuj = p; p += np * nc;            // fhu of one row block [pp + np*c]
-    double *fhuhj = p; p += np * nch;          // fhuh of one row block [pp + np*kc]
-    double *red = p; p += ASM_THREADS;
-    int *perm = (int *)p;
+    double *svt = lds + LY.svt, *svg = lds + LY.svg, *sft = lds + LY.sft, *sfg = lds + LY.sfg;
+    double *pn = lds + LY.pn, *udg = lds + LY.udg, *uh = lds + LY.uh, *sh = lds + LY.sh;
+    double *jac = lds + LY.jac, *Xx = lds + LY.Xx, *jacf = lds + LY.jacf, *nl = lds + LY.nl;
+    double *M = lds + LY.M, *Qm = lds + LY.Qm, *rowk = lds + LY.rowk, *colk = lds + LY.colk;
+    double *C = lds + LY.C, *E = lds + LY.E, *udgg = lds + LY.udgg, *fj = lds + LY.fj, *fhj = lds + LY.fhj;
+    double *red = lds + LY.red;
+    double *xa = lds + LY.xa, *cf = lds + LY.cf, *ef = lds + LY.ef;
+    double *G = svt + npv * ngv, *fuX = G;       // the shapvt-derivative region, reused
+    int *perm = (int *)(lds + LY.total);
     int *invperm = perm + npf * nfe;
     int *bnd = invperm + npv * nfe;
-
-    // ---- per-workgroup scratch (global) ----
-    double *S = scratch + (long)blockIdx.x * sstride;
-    const Scratch L = scratch_layout(d);
-    double *C = S + L.C, *E = S + L.E, *shapMiCf = S + L.shapMiCf, *shapMiEf = S + L.shapMiEf;
-    const double tau = ph.param[5];
 
     // master tables once per workgroup
     for (int i = tid; i < npv * ngv * nd1; i += nth) {
@@ -126,12 +156,16 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
         const double *f = FB + FL.f, *fu = FB + FL.fu, *fh = FB + FL.fh, *fhu = FB + FL.fhu, *fhuh = FB + FL.fhuh;
         const double *fb = FB + FL.fb, *fbu = FB + FL.fbu, *fbuh = FB + FL.fbuh;
         // ---------------- P0: gather ----------------
-        // the derivative part of shapvt held the previous element's fu in P8
-        if (t != (int)blockIdx.x)
+        // the previous element overwrote the shapvt derivatives (and, with the
+        // overlay, shapvg)
+        if (t != (int)blockIdx.x) {
             for (int i = npv * ngv + tid; i < npv * ngv * nd1; i += nth) {
                 const int nn = i % npv, r = i / npv, g = r % ngv, dd = r / ngv;
                 svt[i] = m.shapvt[g + ngv * (nn + npv * dd)];
             }
+            if (LY.overlay)
+                for (int i = tid; i < npv * ngv * nd1; i += nth) svg[i] = m.shapvg[i];
+        }
         for (int i = tid; i < npv * ND; i += nth) pn[i] = dgnodes[e * npv * ND + i];
         for (int i = tid; i < npv * nc; i += nth) udg[i] = UDG[e * npv * nc + i];
         for (int i = tid; i < ndf * nch; i += nth) {
@@ -194,31 +228,47 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
         }
         __syncthreads();
 
-        // ---------------- P2: q operators (qint) ----------------
+        // ---------------- P2: q operators (qint), solution at volume points ----------------
+        // G(i,g,x) = sum_r shapvg(i,g,r+1) Xx(g,x,r), the jac-weighted test-function
+        // gradients, so that C_x(i,k) = sum_g G(i,g,x) shapvt(g,k)
+        for (int i = tid; i < npv * ngv * ND; i += nth) {
+            const int nn = i % npv, r = i / npv, g = r % ngv, x = r / ngv;
+            double sg = 0;
+#pragma unroll
+            for (int q = 0; q < ND; q++) sg += SVG(nn, g, q + 1) * Xx[g + ngv * (x + ND * q)];
+            G[i] = sg;
+        }
+        __syncthreads();
         {
-            const int nM = npv * npv, nC = ND * npv * npv, nE = ND * npv * ndf;
-            for (int i = tid; i < nM + nC + nE; i += nth) {
+            const int nM = npv * npv, nC = ND * npv * npv, nE = ND * npv * ndf, nU = ngv * nc;
+            for (int i = tid; i < nM + nC + nE + nU; i += nth) {
                 if (i < nM) {
-                    int jj = i % npv, ii = i / npv;
+                    const int jj = i % npv, ii = i / npv;
                     double s = 0;
                     for (int g = 0; g < ngv; g++) s += (SVG(jj, g, 0) * SVT(g, ii, 0)) * jac[g];
                     M[i] = ph.fc_q * s;
                 } else if (i < nM + nC) {
-                    int r = i - nM, x = r / (npv * npv), rr = r % (npv * npv), jj = rr % npv, ii = rr / npv;
+                    // C_x(jj, ii) at C[x*npv*npv + ii*npv + jj]
+                    const int r = i - nM, x = r / (npv * npv), rr = r % (npv * npv), jj = rr % npv, ii = rr / npv;
                     double s = 0;
-                    for (int q = 0; q < ND; q++)
-                        for (int g = 0; g < ngv; g++) s += (SVG(jj, g, q + 1) * SVT(g, ii, 0)) * Xx[g + ngv * (x + ND * q)];
+                    for (int g = 0; g < ngv; g++) s += G[jj + npv * (g + ngv * x)] * SVT(g, ii, 0);
                     C[r] = s;
-                } else {
-                    int r = i - nM - nC, x = r / (npv * ndf), rr = r % (npv * ndf), nn = rr % npv, l = rr / npv;
-                    int a = l / npf, j = l % npf, k = INVP(nn, a);
+                } else if (i < nM + nC + nE) {
+                    // E_x(nn, l) at E[x*npv*ndf + l*npv + nn]
+                    const int r = i - nM - nC, x = r / (npv * ndf), rr = r % (npv * ndf), nn = rr % npv, l = rr / npv;
+                    const int a = l / npf, jf = l % npf, k = INVP(nn, a);
                     double s = 0;
                     if (k >= 0)
                         for (int g = 0; g < ngf; g++) {
-                            int pp = g + ngf * a;
-                            s += (SFG(k, g, 0) * SFT(g, j, 0)) * nl[pp + np * x] * jacf[pp];
+                            const int pp = g + ngf * a;
+                            s += (SFG(k, g, 0) * SFT(g, jf, 0)) * nl[pp + np * x] * jacf[pp];
                         }
                     E[r] = s;
+                } else {
+                    const int r = i - nM - nC - nE, g = r % ngv, c = r / ngv;
+                    double s = 0;
+                    for (int q = 0; q < npv; q++) s += SVT(g, q, 0) * udg[c * npv + q];
+                    udgg[r] = s;
                 }
             }
         }
@@ -257,146 +307,158 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
         }
         __syncthreads();
 
-        // ---------------- P5: traces of M^{-1}C, M^{-1}E; solution at points ----------------
-        {
-            const int n1 = ngv * npv * ND, n2 = ngv * ndf * ND, n3 = np * npv * ND, n4 = np * ndf * ND;
-            const int n5 = ngv * nc;
-            const int tot = n1 + n2 + n3 + n4 + n5;
-            for (int i = tid; i < tot; i += nth) {
-                if (i < n1) {
-                    int nn = i % npv, r = i / npv, g = r % ngv, x = r / ngv;
-                    const double *q = Qm + npv * g, *c = C + x * npv * npv + nn * npv;
-                    double s = 0;
-                    for (int l = 0; l < npv; l++) s += q[l] * c[l];
-                    shapMiC[i] = s;
-                } else if (i < n1 + n2) {
-                    int r0 = i - n1, l = r0 % ndf, r = r0 / ndf, g = r % ngv, x = r / ngv;
-                    const double *q = Qm + npv * g, *c = E + (long)x * npv * ndf + l * npv;
-                    double s = 0;
-                    for (int k = 0; k < npv; k++) s += q[k] * c[k];
-                    shapMiE[r0] = -s;
-                } else if (i < n1 + n2 + n3) {
-                    int r0 = i - n1 - n2, nn = r0 % npv, r = r0 / npv, pp = r % np, x = r / np;
-                    const double *q = Qm + npv * (ngv + pp), *c = C + x * npv * npv + nn * npv;
-                    double s = 0;
-                    for (int k = 0; k < npv; k++) s += q[k] * c[k];
-                    shapMiCf[r0] = s;
-                } else if (i < n1 + n2 + n3 + n4) {
-                    int r0 = i - n1 - n2 - n3, l = r0 % ndf, r = r0 / ndf, pp = r % np, x = r / np;
-                    const double *q = Qm + npv * (ngv + pp), *c = E + (long)x * npv * ndf + l * npv;
-                    double s = 0;
-                    for (int k = 0; k < npv; k++) s += q[k] * c[k];
-                    shapMiEf[r0] = s;
-                } else {
-                    int r0 = i - n1 - n2 - n3 - n4, g = r0 % ngv, c = r0 / ngv;
-                    double s = 0;
-                    for (int q = 0; q < npv; q++) s += SVT(g, q, 0) * udg[c * npv + q];
-                    udgg[r0] = s;
-                }
-            }
-        }
-        __syncthreads();
-
         // ---------------- P8: element rows of D and F on the FP64 MFMA ----------------
-        // Row block j of D and F as one product over volume and face points.  With
-        // G(i,g,x) = sum_r shapvg(i,g,r+1) Xx(g,x,r) (jac-weighted test-function
-        // gradients), the q-folded volume integrand of elementint (:418-549) is
-        //   D_j[(kc,i), cc] += sum_{x',g} P[(kc,i),(x',g)] Bd[(x',g), cc]
-        //   F_j[(kc,i), l]  += sum_{x>=1,g} P[(kc,i),(x,g)] Be[(x,g), l]
-        //   P = -sum_x fu(g,j,x, c) G(i,g,x) (+ fc_u jac shapvg(i,g,0) for c = j, tdep),
-        //   c = x' ncu + kc;  Bd = [shapvt(g,cc); M^{-1}C traces], Be = M^{-1}E traces;
+        // Row block j of D and F as one product over volume and face points.  The
+        // q-folded volume integrand of elementint (:418-549), with
+        // P_x'[(kc,i), g] = -sum_r shapvg(i,g,r+1) fuX(g, r, x' ncu + kc)
+        // (+ fc_u jac shapvg(i,g,0) for x' = 0, kc = j, tdep) and
+        // fuX(g,r,c) = sum_x fu(g,j,x,c) Xx(g,x,r), contributes
+        //   D_j[(kc,i), cc] += sum_g P_0 shapvt(g,cc) + sum_{x,g} P_{x+1} (M^{-1}C_x)(g,cc)
+        //   F_j[(kc,i), l]  += -sum_{x,g} P_{x+1} (M^{-1}E_x)(g,l);
         // the face integrand of faceint (:862-1072), the gather over the faces that
-        // contain node i, is the same product over face points pp = (a, g):
-        //   A = [i on a] shapfg(k(i,a), g) jacf(pp) fhu(pp, j, c),
-        //   D: B = [cc on a] shapft(g, k(cc,a)) (c = kc), M^{-1}C face traces (c = q);
-        //   F: B = [l on a] shapft(g, l) with fhuh in A (u part), -M^{-1}E traces (q).
-        // One wave per (j, 16-row tile of (kc,i)), all 16-column tiles of D (cc) and
-        // F (l) accumulated together; A formed on the fly from LDS operands.
+        // contain node i, is the same product over face points pp = (a, g) with
+        //   A_c[(kc,i), pp] = [i on a] shapfg(k(i,a), g) jacf(pp) fhu(pp, j, c):
+        //   D += A_kc [cc on a] shapft(g, k(cc,a)) + sum_x A_{(x+1)ncu+kc} (M^{-1}C_x)(pp,cc)
+        //   F += A^h_kc [l on a] shapft(g, l) (fhuh in A^h) - sum_x A_{(x+1)ncu+kc} (M^{-1}E_x)(pp,l).
+        // The point values of M^{-1}C_x are Q C_x (Q from P4), so the q columns are
+        // contracted through the nodes: W_x = sum_g P_{x+1} Q(g,:) + sum_pp A_x Q(pp,:)
+        // is accumulated as W_x^T (whose MFMA result layout is the A operand layout
+        // of the next product), then D += sum_x W_x C_x, F -= sum_x W_x E_x.
+        // One wave per (j, 16-row tile of (kc,i)), all column tiles of D (cc) and F (l)
+        // accumulated together.  Operands are read at clamped indices and zeroed by
+        // selects, never under a branch, so that the loads of a k-step issue together;
+        // only zeros along the contracted index matter (rows and columns past the end
+        // are never stored).
         {
-            double *G = M;                                   // M, Qm dead after P5
-            double *fuj = svt + npv * ngv;                   // shapvt derivatives dead after P1
-            for (int i = tid; i < npv * ngv * ND; i += nth) {
-                const int nn = i % npv, r = i / npv, g = r % ngv, x = r / ngv;
-                double sg = 0;
-#pragma unroll
-                for (int q = 0; q < ND; q++) sg += SVG(nn, g, q + 1) * Xx[g + ngv * (x + ND * q)];
-                G[i] = sg;
-            }
-            const int lane = tid & 63, wave = tid >> 6, nwave = nth >> 6, lr = lane & 15, lk = lane >> 4;
             const int nrt = (ncu * npv + 15) / 16;
-            // column tiles: TD of D (cc < npv), TF of F (l < ndf); faces per element from
-            // the face size (tets: npf triangular, hexes: npf square)
-            constexpr int NFE = ND == 2 ? 3 : ((NPF == 4 || NPF == 9 || NPF == 16) ? 6 : 4);
-            constexpr int TD = (NPV + 15) / 16, TF = (NPF * NFE + 15) / 16, NT = TD + TF;
             for (int j = 0; j < ncu; j++) {
                 __syncthreads();
-                for (int i = tid; i < ngv * ND * NC; i += nth) {
-                    const int g = i % ngv, r = i / ngv, x = r % ND, c = r / ND;
-                    fuj[i] = fu[g + ngv * (j + ncu * (x + ND * c))];
-                }
-                for (int i = tid; i < np * NC; i += nth) {
-                    const int pp = i % np, c = i / np;
-                    fhuj[i] = fhu[pp + np * (j + nch * c)];
-                    if (c < nch) fhuhj[i] = fhuh[pp + np * (j + nch * c)];
+                // row block j of the flux Jacobians:
+                //   fuX[g + ngv*(r + ND*c)], fj[pp + np*c] = fhu(pp,j,c), fhj[pp + np*kc] = fhuh(pp,j,kc)
+                {
+                    // the thread index is made opaque here so that the staging index
+                    // arithmetic is recomputed per row block instead of being hoisted
+                    // (and held in registers through the MFMA tasks)
+                    int ti = tid;
+                    asm volatile("" : "+v"(ti));
+                    const int N1 = np * NC, N2 = N1 + np * nch, NU = ngv * NC;
+                    double vh[3], vu[2][ND];
+#pragma unroll
+                    for (int u = 0; u < 3; u++) {
+                        const int i = min(ti + u * nth, N2 - 1);
+                        const double *src = i < N1 ? fhu + (i % np) + np * (j + nch * (i / np))
+                                                   : fhuh + ((i - N1) % np) + np * (j + nch * ((i - N1) / np));
+                        vh[u] = *src;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 2; u++) {
+                        const int i = min(ti + u * nth, NU - 1), g = i % ngv, c = i / ngv;
+#pragma unroll
+                        for (int x = 0; x < ND; x++) vu[u][x] = fu[g + ngv * (j + ncu * (x + ND * c))];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 3; u++) {
+                        const int i = ti + u * nth;
+                        if (i < N1) fj[i] = vh[u];
+                        else if (i < N2) fhj[i - N1] = vh[u];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 2; u++) {
+                        const int i = ti + u * nth;
+                        if (i < NU) {
+                            const int g = i % ngv, c = i / ngv;
+#pragma unroll
+                            for (int r = 0; r < ND; r++) {
+                                double s = 0.0;
+#pragma unroll
+                                for (int x = 0; x < ND; x++) s += vu[u][x] * Xx[g + ngv * (x + ND * r)];
+                                fuX[g + ngv * (r + ND * c)] = s;
+                            }
+                        }
+                    }
+                    // sizes beyond the unrolled part
+                    for (int i = ti + 3 * nth; i < N2; i += nth) {
+                        if (i < N1) fj[i] = fhu[(i % np) + np * (j + nch * (i / np))];
+                        else fhj[i - N1] = fhuh[((i - N1) % np) + np * (j + nch * ((i - N1) / np))];
+                    }
+                    for (int i = ti + 2 * nth; i < NU; i += nth) {
+                        const int g = i % ngv, c = i / ngv;
+                        for (int r = 0; r < ND; r++) {
+                            double s = 0.0;
+                            for (int x = 0; x < ND; x++)
+                                s += fu[g + ngv * (j + ncu * (x + ND * c))] * Xx[g + ngv * (x + ND * r)];
+                            fuX[g + ngv * (r + ND * c)] = s;
+                        }
+                    }
                 }
                 __syncthreads();
                 for (int rt = wave; rt < nrt; rt += nwave) {
-                    // this lane's A row and its accumulator rows.  Operands are read at
-                    // clamped indices and zeroed by selects, never under a branch, so that
-                    // all loads of a k-step issue together; only zeros along the k index
-                    // matter (rows and columns past the end are never stored).
+                    // this lane's A row (kc, i) and its accumulator rows
                     const int ra = rt * 16 + lr;
                     const bool va = ra < ncu * npv;
                     const int kca = va ? ra / npv : 0, ia = va ? ra % npv : 0;
                     const double tdj = (ph.tdep && kca == j) ? ph.fc_u : 0.0;
-                    int ccl[TD], lcl[TF];
+                    int ccl[TD], lcl[TF], kq[KT];
+                    bool kqv[KT];
 #pragma unroll
                     for (int t = 0; t < TD; t++) ccl[t] = min(t * 16 + lr, npv - 1);
 #pragma unroll
                     for (int t = 0; t < TF; t++) lcl[t] = min(t * 16 + lr, ndf - 1);
-                    f64x4 acc[NT];
+#pragma unroll
+                    for (int kt = 0; kt < KT; kt++) {
+                        kq[kt] = min(kt * 16 + lr, npv - 1);
+                        kqv[kt] = kt * 16 + lr < npv;
+                    }
+                    f64x4 acc[NT], w[ND][KT];
 #pragma unroll
                     for (int t = 0; t < NT; t++) acc[t] = (f64x4){0.0, 0.0, 0.0, 0.0};
-                    // volume points, u columns: B = shapvt
+#pragma unroll
+                    for (int x = 0; x < ND; x++)
+#pragma unroll
+                        for (int kt = 0; kt < KT; kt++) w[x][kt] = (f64x4){0.0, 0.0, 0.0, 0.0};
+                    // volume points
                     for (int g0 = 0; g0 < ngv; g0 += 4) {
                         const int g = min(g0 + lk, ngv - 1);
-                        double sa = 0.0;
+                        const bool vg = g0 + lk < ngv;
+                        double s[ND1];
 #pragma unroll
-                        for (int x = 0; x < ND; x++) sa += fuj[g + ngv * (x + ND * kca)] * G[ia + npv * (g + ngv * x)];
-                        double a = SVG(ia, g, 0) * tdj * jac[g] - sa;
-                        a = g0 + lk < ngv ? a : 0.0;
-                        double b[TD];
+                        for (int r = 0; r < ND1; r++) s[r] = SVG(ia, g, r);
+                        double a0 = tdj * jac[g] * s[0];
+#pragma unroll
+                        for (int r = 0; r < ND; r++) a0 -= fuX[g + ngv * (r + ND * kca)] * s[r + 1];
+                        a0 = vg ? a0 : 0.0;
+                        double ax[ND];
+#pragma unroll
+                        for (int x = 0; x < ND; x++) {
+                            double v = 0.0;
+#pragma unroll
+                            for (int r = 0; r < ND; r++) v += fuX[g + ngv * (r + ND * ((x + 1) * ncu + kca))] * s[r + 1];
+                            ax[x] = vg ? -v : 0.0;
+                        }
+                        double b[TD], q[KT];
 #pragma unroll
                         for (int t = 0; t < TD; t++) b[t] = SVT(g, ccl[t], 0);
 #pragma unroll
-                        for (int t = 0; t < TD; t++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[t], acc[t], 0, 0, 0);
-                    }
-                    // volume points, q columns: B = M^{-1}C and M^{-1}E traces
-                    for (int xq = 1; xq < nd1; xq++) {
-                        const int c = xq * ncu + kca;
-                        for (int g0 = 0; g0 < ngv; g0 += 4) {
-                            const int g = min(g0 + lk, ngv - 1);
-                            double sa = 0.0;
-#pragma unroll
-                            for (int x = 0; x < ND; x++) sa += fuj[g + ngv * (x + ND * c)] * G[ia + npv * (g + ngv * x)];
-                            const double a = g0 + lk < ngv ? -sa : 0.0;
-                            double b[NT];
-#pragma unroll
-                            for (int t = 0; t < TD; t++) b[t] = shapMiC[ccl[t] + npv * (g + ngv * (xq - 1))];
-#pragma unroll
-                            for (int t = 0; t < TF; t++) b[TD + t] = shapMiE[lcl[t] + ndf * (g + ngv * (xq - 1))];
-#pragma unroll
-                            for (int t = 0; t < NT; t++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[t], acc[t], 0, 0, 0);
+                        for (int kt = 0; kt < KT; kt++) {
+                            const double qv = Qm[kq[kt] + npv * g];
+                            q[kt] = kqv[kt] ? qv : 0.0;
                         }
+#pragma unroll
+                        for (int t = 0; t < TD; t++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b[t], acc[t], 0, 0, 0);
+#pragma unroll
+                        for (int x = 0; x < ND; x++)
+#pragma unroll
+                            for (int kt = 0; kt < KT; kt++)
+                                w[x][kt] = __builtin_amdgcn_mfma_f64_16x16x4f64(q[kt], ax[x], w[x][kt], 0, 0, 0);
                     }
                     // face points
                     for (int p0 = 0; p0 < np; p0 += 4) {
                         const int pp = min(p0 + lk, np - 1);
                         const int a = pp / ngf, gf = pp % ngf;
                         const int ki = INVP(ia, a);
-                        double af = SFG(max(ki, 0), gf, 0) * jacf[pp];
-                        af = (p0 + lk < np && ki >= 0) ? af : 0.0;
-                        double bu[NT];
+                        const double sv = SFG(max(ki, 0), gf, 0) * jacf[pp];
+                        const double af = (p0 + lk < np && ki >= 0) ? sv : 0.0;
+                        double bu[NT], q[KT];
 #pragma unroll
                         for (int t = 0; t < TD; t++) {
                             const int k = INVP(ccl[t], a);
@@ -408,29 +470,47 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                             const double v = SFT(gf, lcl[t] % npf, 0);
                             bu[TD + t] = lcl[t] / npf == a ? v : 0.0;
                         }
-                        double bq[ND][NT];
 #pragma unroll
-                        for (int x = 0; x < ND; x++) {
-#pragma unroll
-                            for (int t = 0; t < TD; t++) bq[x][t] = shapMiCf[ccl[t] + npv * (pp + np * x)];
-#pragma unroll
-                            for (int t = 0; t < TF; t++) bq[x][TD + t] = -shapMiEf[lcl[t] + ndf * (pp + np * x)];
+                        for (int kt = 0; kt < KT; kt++) {
+                            const double qv = Qm[kq[kt] + npv * (ngv + pp)];
+                            q[kt] = kqv[kt] ? qv : 0.0;
                         }
-                        // u part: D with fhu, F with fhuh
-                        const double ad = af * fhuj[pp + np * kca];
-                        const double ah = af * fhuhj[pp + np * kca];
+                        const double ad = af * fj[pp + np * kca];
+                        const double ah = af * fhj[pp + np * kca];
 #pragma unroll
                         for (int t = 0; t < TD; t++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, bu[t], acc[t], 0, 0, 0);
 #pragma unroll
                         for (int t = TD; t < NT; t++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ah, bu[t], acc[t], 0, 0, 0);
-                        // q parts
 #pragma unroll
                         for (int x = 0; x < ND; x++) {
-                            const double aq = af * fhuj[pp + np * (ncu * (x + 1) + kca)];
+                            const double aq = af * fj[pp + np * ((x + 1) * ncu + kca)];
 #pragma unroll
-                            for (int t = 0; t < NT; t++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(aq, bq[x][t], acc[t], 0, 0, 0);
+                            for (int kt = 0; kt < KT; kt++)
+                                w[x][kt] = __builtin_amdgcn_mfma_f64_16x16x4f64(q[kt], aq, w[x][kt], 0, 0, 0);
                         }
                     }
+                    // q columns through the nodes: element v of W_x^T tile kt is the A
+                    // operand of the k-step over nodes kt*16 + 4v .. +3
+#pragma unroll
+                    for (int x = 0; x < ND; x++)
+#pragma unroll
+                        for (int kt = 0; kt < KT; kt++)
+#pragma unroll
+                            for (int v = 0; v < 4; v++) {
+                                if (kt * 16 + 4 * v >= NPV) continue;
+                                const int k = min(kt * 16 + 4 * v + lk, npv - 1);
+                                double b[NT];
+#pragma unroll
+                                for (int t = 0; t < TD; t++) b[t] = C[x * npv * npv + ccl[t] * npv + k];
+#pragma unroll
+                                for (int t = 0; t < TF; t++) b[TD + t] = -E[x * npv * ndf + lcl[t] * npv + k];
+#pragma unroll
+                                for (int t = 0; t < NT; t++)
+                                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(w[x][kt][v], b[t], acc[t], 0, 0, 0);
+                                // keep the scheduler from hoisting every k-step's operand loads
+                                // ahead of the products (they would not fit in registers)
+                                __builtin_amdgcn_sched_barrier(0);
+                            }
 #pragma unroll
                     for (int t = 0; t < NT; t++)
 #pragma unroll
@@ -451,29 +531,137 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
         }
         __syncthreads();
 
+        // ---------------- P9: Ru, Rh ----------------
+        // elementint's residual rows (:418-549) and faceint's (:862-1072, boundary
+        // faces with fbou): tables T[g + ngv*(j + ncu*r)] (r = 0 the time term, r > 0
+        // f . Xx) and the face fluxes times jacf (interior fh for Ru; fh or fb for Rh)
+        {
+            double *T = fj, *Fi = fj + ngv * ncu * nd1, *Fo = fhj;
+            const int NT0 = ngv * ncu * nd1, NT1 = NT0 + np * nch, NT2 = NT1 + np * nch;
+            for (int i = tid; i < NT2; i += nth) {
+                if (i < NT0) {
+                    const int g = i % ngv, r2 = i / ngv, jj = r2 % ncu, r = r2 / ncu;
+                    double s = 0.0;
+                    if (r == 0) {
+                        if (ph.tdep) {
+                            double sv = 0.0;
+                            for (int q = 0; q < npv; q++) sv += SVT(g, q, 0) * sh[jj * npv + q];
+                            s = (sv - udgg[g + ngv * jj] * ph.fc_u) * jac[g];
+                        }
+                    } else {
+#pragma unroll
+                        for (int x = 0; x < ND; x++) s += f[g + ngv * (jj + ncu * x)] * Xx[g + ngv * (x + ND * (r - 1))];
+                    }
+                    T[i] = s;
+                } else if (i < NT1) {
+                    const int k = i - NT0, pp = k % np, jj = k / np;
+                    Fi[k] = fh[pp + np * jj] * jacf[pp];
+                } else {
+                    const int k = i - NT1, pp = k % np, jj = k / np;
+                    const double *fx = bnd[pp / ngf] ? fb : fh;
+                    Fo[k] = fx[pp + np * jj] * jacf[pp];
+                }
+            }
+            __syncthreads();
+            const long nH = (long)nb * nb;
+            double ru2loc = 0.0;
+            for (int col = tid; col < na + nb; col += nth) {
+                if (col < na) {
+                    const int row = col, jj = row / npv, ii = row % npv;
+                    double s = 0.0;
+                    for (int g = 0; g < ngv; g++)
+#pragma unroll
+                        for (int r = 0; r < ND1; r++) s += SVG(ii, g, r) * T[g + ngv * (jj + ncu * r)];
+                    for (int a = 0; a < nfe; a++) {
+                        const int kk = INVP(ii, a);
+                        if (kk < 0) continue;
+                        for (int g = 0; g < ngf; g++) s -= SFG(kk, g, 0) * Fi[g + ngf * a + np * jj];
+                    }
+                    Ru[e * na + row] = s;
+                    Fx[e * (long)na * ldx + (long)row * ldx + nb] = s;
+                    ru2loc += s * s;
+                } else {
+                    const int row = col - na, jj = row % nch, tr = row / nch, a = tr / npf, kk = tr % npf;
+                    double s = 0.0;
+                    for (int g = 0; g < ngf; g++) s += SFG(kk, g, 0) * Fo[g + ngf * a + np * jj];
+                    Rh[e * nb + row] = -s;
+                    Hx[e * (nH + nb) + nH + row] = -s;
+                }
+            }
+            // |Ru_e|^2 for the residual norm (assembly/globalassembly.cpp:501-502)
+            for (int o = 32; o > 0; o >>= 1) ru2loc += __shfl_xor(ru2loc, o);
+            if (lane == 0) red[wave] = ru2loc;
+            __syncthreads();
+            if (tid == 0 && ru2) {
+                double s = 0.0;
+                for (int w = 0; w < nwave; w++) s += red[w];
+                ru2[e] = s;
+            }
+        }
+
         // ---------------- P9k: trace rows of K and H on the FP64 MFMA ----------------
         // faceint's trace rows (:1254-1318; boundary faces with the far-field flux,
         // :1074-1246): for face a, rows (a, kk, j) and columns (kc, cc) of K, (kc, l) of H,
         //   K^T[cc, (kc,kk,j)] = sum_{x',g} Bk[cc, (x',g)] Pk[(x',g), (kc,kk,j)],
         //   Pk = shapfg(kk,g) jacf(pp) xu(pp, j, x' ncu + kc)  (xuh for the u part of H),
-        // with Bk / Bh the same face-point operands as the face part of D / F and xu the
+        // with Bk / Bh the face-point operands of the face part of D / F and xu the
         // interior (fhu) or boundary (fbu) flux Jacobian of the face.  Computed
         // transposed so that a tile column runs along contiguous (kk, j) rows of K, H.
+        // The face's point values of M^{-1}C_x and M^{-1}E_x (Q C_x, Q E_x on its
+        // ngf points) are formed first, on the MFMA, next to the flux Jacobians.
         {
-            const int lane = tid & 63, wave = tid >> 6, nwave = nth >> 6, lr = lane & 15, lk = lane >> 4;
-            constexpr int NFE = ND == 2 ? 3 : ((NPF == 4 || NPF == 9 || NPF == 16) ? 6 : 4);
-            constexpr int TD = (NPV + 15) / 16, TF = (NPF * NFE + 15) / 16, NT = TD + TF;
             const int ncol = ncu * npf * nch, nct = (ncol + 15) / 16;
+            const int nrg = (ngf + 15) / 16;
             const long nH = (long)nb * nb;
             for (int a = 0; a < nfe; a++) {
                 const bool bd = bnd[a];
                 const double *xu = bd ? fbu : fhu, *xuh = bd ? fbuh : fhuh;
                 __syncthreads();
-                // this face's flux Jacobians: xa[g + ngf*(j + nch*c)], xha[g + ngf*(j + nch*kc)]
-                for (int i = tid; i < ngf * nch * (nc + nch); i += nth) {
-                    const int g = i % ngf, r = i / ngf, jj = r % nch, c = r / nch, pp = g + ngf * a;
-                    if (c < nc) xa[i] = xu[pp + np * (jj + nch * c)];
-                    else xha[g + ngf * (jj + nch * (c - nc))] = xuh[pp + np * (jj + nch * (c - nc))];
+                // this face's flux Jacobians: xa[g + ngf*(j + nch*c)] (c < nc), then
+                // xha[g + ngf*(j + nch*kc)] right after them
+                {
+                    const int N = ngf * nch * (nc + nch);
+                    for (int i0 = tid; i0 < N; i0 += 3 * nth) {
+                        double v[3];
+#pragma unroll
+                        for (int u = 0; u < 3; u++) {
+                            const int i = min(i0 + u * nth, N - 1);
+                            const int g = i % ngf, r = i / ngf, jj = r % nch, c = r / nch, pp = g + ngf * a;
+                            const double *src = c < nc ? xu + pp + np * (jj + nch * c) : xuh + pp + np * (jj + nch * (c - nc));
+                            v[u] = *src;
+                        }
+#pragma unroll
+                        for (int u = 0; u < 3; u++)
+                            if (i0 + u * nth < N) xa[i0 + u * nth] = v[u];
+                    }
+                }
+                // cf[cc + npv*(g + ngf*x)] = sum_k Q(pp,k) C_x(k,cc),
+                // ef[l + ndf*(g + ngf*x)] = sum_k Q(pp,k) E_x(k,l),   pp = g + ngf*a
+                for (int task = wave; task < nrg * ND * NT; task += nwave) {
+                    const int t = task % NT, r2 = task / NT, x = r2 % ND, rg = r2 / ND;
+                    const int pp = min(rg * 16 + lr, ngf - 1) + ngf * a;
+                    const bool isD = t < TD;
+                    const int col = isD ? min(t * 16 + lr, npv - 1) : min((t - TD) * 16 + lr, ndf - 1);
+                    const double *bsrc = isD ? C + x * npv * npv + col * npv : E + x * npv * ndf + col * npv;
+                    f64x4 acc = (f64x4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int k0 = 0; k0 < NPV; k0 += 4) {
+                        const int k = min(k0 + lk, npv - 1);
+                        const double qv = Qm[k + npv * (ngv + pp)];
+                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(k0 + lk < npv ? qv : 0.0, bsrc[k], acc, 0, 0, 0);
+                    }
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        const int gr = rg * 16 + lk + 4 * v;
+                        if (gr >= ngf) continue;
+                        if (isD) {
+                            const int cc = t * 16 + lr;
+                            if (cc < npv) cf[cc + npv * (gr + ngf * x)] = acc[v];
+                        } else {
+                            const int l = (t - TD) * 16 + lr;
+                            if (l < ndf) ef[l + ndf * (gr + ngf * x)] = acc[v];
+                        }
+                    }
                 }
                 __syncthreads();
                 for (int ct = wave; ct < nct; ct += nwave) {
@@ -513,13 +701,13 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
 #pragma unroll
                         for (int x = 0; x < ND; x++) {
 #pragma unroll
-                            for (int t = 0; t < TD; t++) aq[x][t] = shapMiCf[ccl[t] + npv * (pp + np * x)];
+                            for (int t = 0; t < TD; t++) aq[x][t] = cf[ccl[t] + npv * (g + ngf * x)];
 #pragma unroll
-                            for (int t = 0; t < TF; t++) aq[x][TD + t] = -shapMiEf[lcl[t] + ndf * (pp + np * x)];
+                            for (int t = 0; t < TF; t++) aq[x][TD + t] = -ef[lcl[t] + ndf * (g + ngf * x)];
                         }
                         // u part: K with xu, H with xuh
                         const double bk = wgt * xa[g + ngf * (jb + nch * kcb)];
-                        const double bh = wgt * xha[g + ngf * (jb + nch * kcb)];
+                        const double bh = wgt * xa[ngf * nch * nc + g + ngf * (jb + nch * kcb)];
 #pragma unroll
                         for (int t = 0; t < TD; t++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(au[t], bk, acc[t], 0, 0, 0);
 #pragma unroll
@@ -550,60 +738,6 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                     }
                 }
             }
-        }
-        __syncthreads();
-
-        // ---------------- P9: Ru, Rh ----------------
-        {
-            const long nH = (long)nb * nb;
-            double ru2loc = 0.0;
-            for (int col = tid; col < na + nb; col += nth) {
-                if (col < na) {
-                    int row = col, j = row / npv, ii = row % npv;
-                    double s = 0;
-                    for (int g = 0; g < ngv; g++) {
-                        if (ph.tdep) {
-                            double sv = 0;
-                            for (int q = 0; q < npv; q++) sv += SVT(g, q, 0) * sh[j * npv + q];
-                            s += SVG(ii, g, 0) * (sv - udgg[g + ngv * j] * ph.fc_u) * jac[g];
-                        }
-                        for (int mq = 0; mq < ND; mq++) {
-                            double tx = 0;
-                            for (int x = 0; x < ND; x++) tx += f[g + ngv * (j + ncu * x)] * Xx[g + ngv * (x + ND * mq)];
-                            s += SVG(ii, g, mq + 1) * tx;
-                        }
-                    }
-                    for (int a = 0; a < nfe; a++) {
-                        int kk = INVP(ii, a);
-                        if (kk < 0) continue;
-                        for (int g = 0; g < ngf; g++) {
-                            int pp = g + ngf * a;
-                            s -= SFG(kk, g, 0) * fh[pp + np * j] * jacf[pp];
-                        }
-                    }
-                    Ru[e * na + row] = s;
-                    Fx[e * (long)na * ldx + (long)row * ldx + nb] = s;
-                    ru2loc += s * s;
-                } else {
-                    int row = col - na, j = row % nch, tr = row / nch, a = tr / npf, kk = tr % npf;
-                    const double *fx = bnd[a] ? fb : fh;
-                    double s = 0;
-                    for (int g = 0; g < ngf; g++) {
-                        int pp = g + ngf * a;
-                        s += SFG(kk, g, 0) * fx[pp + np * j] * jacf[pp];
-                    }
-                    Rh[e * nb + row] = -s;
-                    Hx[e * (nH + nb) + nH + row] = -s;
-                }
-            }
-            // |Ru_e|^2 for the residual norm (assembly/globalassembly.cpp:501-502)
-            red[tid] = ru2loc;
-            __syncthreads();
-            for (int s = nth / 2; s > 0; s >>= 1) {
-                if (tid < s) red[tid] += red[tid + s];
-                __syncthreads();
-            }
-            if (tid == 0 && ru2) ru2[e] = red[0];
         }
         __syncthreads();
     }
@@ -965,15 +1099,8 @@ size_t flux_lds_bytes(const Dims &d)
 
 size_t assemble_lds_bytes(const Dims &d)
 {
-    const size_t nc = d.nc, npv = d.npv, ngv = d.ngv, np = d.np, nd = d.nd;
-    const size_t svt = npv * ngv + (npv > nc ? npv : nc) * ngv * nd;
-    size_t mreg = (npv + ngv + np + 2) * npv > npv * ngv * nd ? (npv + ngv + np + 2) * npv : npv * ngv * nd;
-    if ((size_t)d.ngf * d.nch * (nc + d.nch) > mreg) mreg = (size_t)d.ngf * d.nch * (nc + d.nch);
-    size_t n = svt + npv * ngv * d.nd1 + 2 * (size_t)d.npf * d.ngf * nd + npv * nd + npv * nc + (size_t)d.ndf * d.nch +
-               npv * d.ncu + ngv + ngv * nd * nd + np + np * nd + mreg + npv * ngv * nd + (size_t)d.ndf * ngv * nd +
-               ngv * nc + np * nc + np * d.nch + ASM_THREADS;
-    size_t ints = (size_t)d.npf * d.nfe + npv * d.nfe + d.nfe + 8;
-    return n * sizeof(double) + ints * sizeof(int);
+    const AsmLayout L = asm_layout(d);
+    return (size_t)L.total * sizeof(double) + (size_t)L.nints * sizeof(int);
 }
 
 // element families with compiled kernels: (nd, npv, npf)
@@ -991,7 +1118,7 @@ bool assemble_supported(int nd, int npv, int npf)
 
 hipError_t launch_assemble(const Dims &d, const Master &m, const Phys &ph, const double *dgnodes, const double *UDG,
                            const double *UH, const double *SH, const int *elcon, const int *bf, int e0, int n,
-                           double *scratch, long sstride, int grid, int ldx, double *fluxbuf, long fstride, int chunk,
+                           int grid, int ldx, double *fluxbuf, long fstride, int chunk,
                            double *Ru, double *D, double *Fx, double *K, double *Hx, double *Rh, double *ru2,
                            hipStream_t stream)
 {
@@ -1009,7 +1136,7 @@ hipError_t launch_assemble(const Dims &d, const Master &m, const Phys &ph, const
             hipFuncSetAttribute((const void *)assemble_kernel<a, b, c>, hipFuncAttributeMaxDynamicSharedMemorySize,\
                                 (int)lds);                                                                          \
             hipLaunchKernelGGL((assemble_kernel<a, b, c>), dim3(agrid), dim3(ASM_THREADS), lds, stream, d, m, ph,   \
-                               dgnodes, UDG, UH, SH, elcon, bf, c0, cn, scratch, sstride, ldx, fluxbuf, fstride, Ru, \
+                               dgnodes, UDG, UH, SH, elcon, bf, c0, cn, ldx, fluxbuf, fstride, Ru, \
                                D, Fx, K, Hx, Rh, ru2);                                                              \
             done = true;                                                                                            \
         }

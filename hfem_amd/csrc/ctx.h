@@ -12,7 +12,7 @@
 
 namespace hfem {
 hipError_t launch_assemble(const Dims &, const Master &, const Phys &, const double *, const double *, const double *,
-                           const double *, const int *, const int *, int, int, double *, long, int, int, double *, long,
+                           const double *, const int *, const int *, int, int, int, int, double *, long,
                            int, double *, double *, double *, double *, double *, double *, double *, hipStream_t);
 size_t flux_lds_bytes(const Dims &d);
 hipError_t launch_dinv(int na, int e0, int n, const double *D, double *Dinv, int *info, hipStream_t s);
@@ -122,11 +122,10 @@ struct hfem_ctx {
     int *info = nullptr;
     int ldx = 0;
     // work
-    double *scratch = nullptr, *dinv = nullptr, *rwork = nullptr, *fluxbuf = nullptr;   // dinv: ne x na x na
+    double *dinv = nullptr, *rwork = nullptr, *fluxbuf = nullptr;   // dinv: ne x na x na
     rocblas_handle blas = nullptr;
     long fstride = 0;
     int chunk = 0;
-    long sstride = 0;
     int grid_asm = 0, grid_cond = 0;
     int maxlds = 0;
     // trace system

@@ -150,24 +150,6 @@ __device__ __forceinline__ void ns_normal_jac(double gam, double eps, const dual
     }
 }
 
-// Per-workgroup scratch offsets (doubles) used by the assembly kernel.
-struct Scratch {
-    long C, E, shapMiCf, shapMiEf, total;
-};
-
-__host__ __device__ inline Scratch scratch_layout(const Dims &d)
-{
-    Scratch s;
-    long o = 0;
-    auto take = [&](long n) { long r = o; o += (n + 31) / 32 * 32; return r; };
-    s.C = take((long)d.nd * d.npv * d.npv);
-    s.E = take((long)d.nd * d.npv * d.ndf);
-    s.shapMiCf = take((long)d.np * d.npv * d.nd);
-    s.shapMiEf = take((long)d.np * d.ndf * d.nd);
-    s.total = o;
-    return s;
-}
-
 // Per-element pointwise flux data written by the flux kernel, read by assembly.
 struct FluxLayout {
     long f, fu, fh, fhu, fhuh, fb, fbu, fbuh, total;
