@@ -12,7 +12,7 @@
 //   P0 element data (the master tables are loaded once per workgroup)
 //   P1 volume/face geometry
 //   P2 q operators M, C, E (qint) and the solution at volume points
-//   P3 M^{-1} by Gauss-Jordan
+//   P3 M^{-1} by Gauss-Jordan (one wave, in registers) next to C
 //   P4 Q = [shapvt; face traces] M^{-1}: the point values of M^{-1}C and M^{-1}E
 //      are Q C and Q E, never formed as tables
 //   P8 element rows of D and F on the FP64 MFMA, one row block j at a time
@@ -41,7 +41,7 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 // it fits, else they overlay the shapvt derivatives and shapvg, which P0 then
 // reloads for the next element.
 struct AsmLayout {
-    int svt, svg, sft, sfg, pn, udg, uh, sh, jac, Xx, jacf, nl, M, Qm, rowk, colk, C, E, udgg, fj, fhj, red;
+    int svt, svg, sft, sfg, pn, udg, uh, sh, jac, Xx, jacf, nl, M, Qm, C, E, udgg, fj, fhj, red;
     int xa, xha, cf, ef;
     int total, nints, overlay;
 };
@@ -67,8 +67,6 @@ __host__ __device__ inline AsmLayout asm_layout(const Dims &d)
     L.nl = o; o += np * nd;
     L.M = o; o += npv * npv;
     L.Qm = o; o += (ngv + np) * npv;
-    L.rowk = o; o += npv;
-    L.colk = o; o += npv;
     L.C = o; o += nd * npv * npv;
     L.E = o; o += nd * npv * ndf;
     L.udgg = o; o += ngv * nc;
@@ -93,6 +91,43 @@ __host__ __device__ inline AsmLayout asm_layout(const Dims &d)
     L.ef = L.cf + npv * ngf * nd;
     L.total = o;
     return L;
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int l)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+
+// In-place inverse of the NPV x NPV matrix M (column-major, M(i,j) at M[i + NPV*j])
+// by Gauss-Jordan without pivoting, run by one wave: lane j holds column j in
+// registers and each pivot column is broadcast with readlane, so no barrier is
+// needed.  For symmetric positive definite M (the mass matrix).
+template <int NPV>
+__device__ __forceinline__ void gauss_jordan_wave(double *M, int lane)
+{
+    const int col = lane < NPV ? lane : 0;
+    double m[NPV];
+#pragma unroll
+    for (int i = 0; i < NPV; i++) m[i] = M[i + NPV * col];
+#pragma unroll
+    for (int k = 0; k < NPV; k++) {
+        double ck[NPV];                              // column k: M(i, k)
+#pragma unroll
+        for (int i = 0; i < NPV; i++) ck[i] = readlane_f64(m[i], k);
+        const double pinv = 1.0 / ck[k];
+        const double rk = m[k];                      // row k: M(k, col)
+        const bool isk = lane == k;
+#pragma unroll
+        for (int i = 0; i < NPV; i++) {
+            if (i == k) m[i] = isk ? pinv : rk * pinv;
+            else m[i] = isk ? -ck[i] * pinv : m[i] - ck[i] * rk * pinv;
+        }
+    }
+    if (lane < NPV)
+#pragma unroll
+        for (int i = 0; i < NPV; i++) M[i + NPV * lane] = m[i];
 }
 
 template <int ND, int NPV, int NPF>
@@ -121,7 +156,7 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
     double *svt = lds + LY.svt, *svg = lds + LY.svg, *sft = lds + LY.sft, *sfg = lds + LY.sfg;
     double *pn = lds + LY.pn, *udg = lds + LY.udg, *uh = lds + LY.uh, *sh = lds + LY.sh;
     double *jac = lds + LY.jac, *Xx = lds + LY.Xx, *jacf = lds + LY.jacf, *nl = lds + LY.nl;
-    double *M = lds + LY.M, *Qm = lds + LY.Qm, *rowk = lds + LY.rowk, *colk = lds + LY.colk;
+    double *M = lds + LY.M, *Qm = lds + LY.Qm;
     double *C = lds + LY.C, *E = lds + LY.E, *udgg = lds + LY.udgg, *fj = lds + LY.fj, *fhj = lds + LY.fhj;
     double *red = lds + LY.red;
     double *xa = lds + LY.xa, *cf = lds + LY.cf, *ef = lds + LY.ef;
@@ -229,33 +264,32 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
         __syncthreads();
 
         // ---------------- P2: q operators (qint), solution at volume points ----------------
-        // G(i,g,x) = sum_r shapvg(i,g,r+1) Xx(g,x,r), the jac-weighted test-function
-        // gradients, so that C_x(i,k) = sum_g G(i,g,x) shapvt(g,k)
-        for (int i = tid; i < npv * ngv * ND; i += nth) {
-            const int nn = i % npv, r = i / npv, g = r % ngv, x = r / ngv;
-            double sg = 0;
-#pragma unroll
-            for (int q = 0; q < ND; q++) sg += SVG(nn, g, q + 1) * Xx[g + ngv * (x + ND * q)];
-            G[i] = sg;
-        }
-        __syncthreads();
+        // M, E, udgg and G(i,g,x) = sum_r shapvg(i,g,r+1) Xx(g,x,r), the jac-weighted
+        // test-function gradients (C_x(i,k) = sum_g G(i,g,x) shapvt(g,k) follows in P3)
         {
-            const int nM = npv * npv, nC = ND * npv * npv, nE = ND * npv * ndf, nU = ngv * nc;
-            for (int i = tid; i < nM + nC + nE + nU; i += nth) {
-                if (i < nM) {
-                    const int jj = i % npv, ii = i / npv;
+            const int nG = npv * ngv * ND, nM = npv * npv, nU = ngv * nc, nE = ND * npv * ndf;
+            for (int i = tid; i < nG + nM + nU + nE; i += nth) {
+                if (i < nG) {
+                    const int nn = i % npv, r = i / npv, g = r % ngv, x = r / ngv;
+                    double sg = 0;
+#pragma unroll
+                    for (int q = 0; q < ND; q++) sg += SVG(nn, g, q + 1) * Xx[g + ngv * (x + ND * q)];
+                    G[i] = sg;
+                } else if (i < nG + nM) {
+                    // M(jj, ii) at M[jj + npv*ii]
+                    const int r = i - nG, jj = r % npv, ii = r / npv;
                     double s = 0;
                     for (int g = 0; g < ngv; g++) s += (SVG(jj, g, 0) * SVT(g, ii, 0)) * jac[g];
-                    M[i] = ph.fc_q * s;
-                } else if (i < nM + nC) {
-                    // C_x(jj, ii) at C[x*npv*npv + ii*npv + jj]
-                    const int r = i - nM, x = r / (npv * npv), rr = r % (npv * npv), jj = rr % npv, ii = rr / npv;
+                    M[r] = ph.fc_q * s;
+                } else if (i < nG + nM + nU) {
+                    const int r = i - nG - nM, g = r % ngv, c = r / ngv;
                     double s = 0;
-                    for (int g = 0; g < ngv; g++) s += G[jj + npv * (g + ngv * x)] * SVT(g, ii, 0);
-                    C[r] = s;
-                } else if (i < nM + nC + nE) {
+                    for (int q = 0; q < npv; q++) s += SVT(g, q, 0) * udg[c * npv + q];
+                    udgg[r] = s;
+                } else {
                     // E_x(nn, l) at E[x*npv*ndf + l*npv + nn]
-                    const int r = i - nM - nC, x = r / (npv * ndf), rr = r % (npv * ndf), nn = rr % npv, l = rr / npv;
+                    const int r = i - nG - nM - nU, x = r / (npv * ndf), rr = r % (npv * ndf), nn = rr % npv,
+                              l = rr / npv;
                     const int a = l / npf, jf = l % npf, k = INVP(nn, a);
                     double s = 0;
                     if (k >= 0)
@@ -264,34 +298,25 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                             s += (SFG(k, g, 0) * SFT(g, jf, 0)) * nl[pp + np * x] * jacf[pp];
                         }
                     E[r] = s;
-                } else {
-                    const int r = i - nM - nC - nE, g = r % ngv, c = r / ngv;
-                    double s = 0;
-                    for (int q = 0; q < npv; q++) s += SVT(g, q, 0) * udg[c * npv + q];
-                    udgg[r] = s;
                 }
             }
         }
         __syncthreads();
 
-        // ---------------- P3: M^{-1} by Gauss-Jordan in LDS (M is SPD: no pivoting) ----------------
-        for (int k = 0; k < npv; k++) {
-            for (int i = tid; i < 2 * npv; i += nth) {
-                if (i < npv) rowk[i] = M[k + npv * i];      // M(k, i)
-                else colk[i - npv] = M[(i - npv) + npv * k];  // M(i, k)
+        // ---------------- P3: M^{-1} by Gauss-Jordan (M is SPD: no pivoting), and C ----------------
+        // Wave 0 inverts M in registers, one column per lane; the other waves form
+        // C_x(jj, ii) at C[x*npv*npv + ii*npv + jj] meanwhile.
+        if (wave == 0) {
+            gauss_jordan_wave<NPV>(M, lane);
+        } else {
+            for (int r = tid - 64; r < ND * npv * npv; r += nth - 64) {
+                const int x = r / (npv * npv), rr = r % (npv * npv), jj = rr % npv, ii = rr / npv;
+                double s = 0;
+                for (int g = 0; g < ngv; g++) s += G[jj + npv * (g + ngv * x)] * SVT(g, ii, 0);
+                C[r] = s;
             }
-            __syncthreads();
-            const double pinv = 1.0 / rowk[k];
-            for (int idx = tid; idx < npv * npv; idx += nth) {
-                const int i = idx % npv, j = idx / npv;       // M(i, j) at i + npv*j
-                double v;
-                if (i == k) v = (j == k) ? pinv : rowk[j] * pinv;
-                else if (j == k) v = -colk[i] * pinv;
-                else v = M[idx] - colk[i] * rowk[j] * pinv;
-                M[idx] = v;
-            }
-            __syncthreads();
         }
+        __syncthreads();
 
         // ---------------- P4: Q = [shapvt ; face traces] M^{-1} ----------------
         for (int idx = tid; idx < (ngv + np) * npv; idx += nth) {
