@@ -119,7 +119,7 @@ extern "C" int hfem_destroy(hfem_ctx *c)
                     c->X, c->K, c->Hx, c->Ru, c->Rh, c->ru2, c->info, c->dinv, c->rwork, c->fluxbuf, c->Hg, c->Rg,
                     c->r, c->partial, c->xv, c->yv, c->sv.d_o2u, c->sv.d_u2o, c->sv.d_lrows, c->sv.d_urows,
                     c->sv.Mg, c->sv.Dinv, c->sv.Cw, c->sv.d_info, c->sv.x, c->sv.V, c->sv.b, c->sv.t, c->sv.dots,
-                    c->sv.coef, c->sv.UDGi};
+                    c->sv.coef, c->sv.UDGi, c->qn_dinv, c->qn_K};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (auto &ev : c->ev)
@@ -430,6 +430,83 @@ int hfem_reduce_sum(hfem_ctx *c, long n, const double *x, int square, double *ou
     return 0;
 }
 
+// updateQuasiNewtonStructures (assembly/globalassembly.cpp:327-389): keep inv(D) and K
+// of this Jacobian, in float (quasiNewtonAccuracy 0) or double (1).  The reference
+// keeps the LU factors of D and solves with them; inv(D) is what this build's
+// condensation forms, so it is kept instead.
+static int store_quasi_newton(hfem_ctx *c, int precision)
+{
+    const Dims &d = c->d;
+    const int ne = c->hd.ne;
+    const long nD = (long)ne * d.na * d.na, nK = (long)ne * d.nb * d.na;
+    // the mixed condensation does not form inv(D) in FP64
+    if (precision != HFEM_FP64) HIPCHK(launch_dinv(d.na, 0, ne, c->D, c->dinv, c->info, c->stream));
+    if (c->qn_acc == 1) {
+        HIPCHK(hipMemcpyAsync(c->qn_dinv, c->dinv, nD * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->qn_K, c->K, nK * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    } else {
+        HIPCHK(launch_to_f32(nD, c->dinv, (float *)c->qn_dinv, c->stream));
+        HIPCHK(launch_to_f32(nK, c->K, (float *)c->qn_K, c->stream));
+    }
+    c->qn_valid = true;
+    return 0;
+}
+
+extern "C" int hfem_quasi_newton_storage(hfem_ctx *c, int accuracy)
+{
+    if (!c) return hfem_fail(-1, "hfem_quasi_newton_storage: null context");
+    if (accuracy < -1 || accuracy > 1) return hfem_fail(-1, "hfem_quasi_newton_storage: accuracy %d", accuracy);
+    if (accuracy == c->qn_acc) return 0;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->qn_dinv) HIPCHK(hipFree(c->qn_dinv));
+    if (c->qn_K) HIPCHK(hipFree(c->qn_K));
+    c->qn_dinv = c->qn_K = nullptr;
+    c->qn_valid = false;
+    c->qn_acc = accuracy;
+    if (accuracy < 0) return 0;
+    const Dims &d = c->d;
+    const size_t elt = accuracy == 1 ? sizeof(double) : sizeof(float);
+    const size_t nD = (size_t)c->hd.ne * d.na * d.na, nK = (size_t)c->hd.ne * d.nb * d.na;
+    if (hipMalloc(&c->qn_dinv, nD * elt) != hipSuccess || hipMalloc(&c->qn_K, nK * elt) != hipSuccess) {
+        if (c->qn_dinv) hipFree(c->qn_dinv);
+        c->qn_dinv = c->qn_K = nullptr;
+        c->qn_acc = -1;
+        return hfem_fail(-2, "hfem_quasi_newton_storage: out of device memory (%zu bytes)", (nD + nK) * elt);
+    }
+    return 0;
+}
+
+// assembleRHS (assembly/globalassembly.cpp:559-616): element vectors at the current
+// state, condensed with the stored inv(D) and K (schurElementVector); Rg and r
+// gathered; inv(D) Ru replaces the Ru column of X, whose inv(D) F columns stay
+// those of the stored Jacobian (what updateSol then combines).
+extern "C" int hfem_assemble_rhs(hfem_ctx *c, double *rNorm)
+{
+    if (!c) return hfem_fail(-1, "hfem_assemble_rhs: null context");
+    if (c->qn_acc < 0 || !c->qn_valid)
+        return hfem_fail(-3, "hfem_assemble_rhs: no stored Jacobian (enable hfem_quasi_newton_storage, then "
+                             "hfem_assemble_linear_system)");
+    const int ne = c->hd.ne, nf = c->hd.nf;
+    const Dims &d = c->d;
+    int rc = hfem_assemble(c, 0, ne);
+    if (rc) return rc;
+    const int bsz = d.nch * d.npf;
+    HIPCHK(launch_face_gather(nf, bsz, d.nch, d.npf, d.nb, d.nb, c->f2e, c->einv, c->Rh, c->r, nullptr, nullptr,
+                              c->stream));
+    double s1 = 0, s2 = 0;
+    if ((rc = hfem_reduce_sum(c, ne, c->ru2, 0, &s1))) return rc;
+    if ((rc = hfem_reduce_sum(c, (long)nf * bsz, c->r, 1, &s2))) return rc;
+    if (rNorm) *rNorm = sqrt(s1 + s2);
+    HIPCHK(launch_schur_vector(d.na, d.nb, c->ldx, ne, c->qn_acc == 0, c->qn_dinv, c->qn_K, c->Ru, c->Rh, c->X,
+                               c->Hx, c->stream));
+    const long es = (long)d.nb * (d.nb + 1);
+    HIPCHK(launch_face_gather(nf, bsz, d.nch, d.npf, d.nb, es, c->f2e, c->einv, c->Hx + (long)d.nb * d.nb, c->Rg,
+                              nullptr, nullptr, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 extern "C" int hfem_assemble_linear_system(hfem_ctx *c, int precision, double *rNorm)
 {
     if (!c) return hfem_fail(-1, "hfem_assemble_linear_system: null context");
@@ -446,6 +523,7 @@ extern "C" int hfem_assemble_linear_system(hfem_ctx *c, int precision, double *r
     if ((rc = hfem_reduce_sum(c, (long)nf * bsz, c->r, 1, &s2))) return rc;
     if (rNorm) *rNorm = sqrt(s1 + s2);
     if ((rc = hfem_condense(c, 0, ne, precision))) return rc;
+    if (c->qn_acc >= 0 && (rc = store_quasi_newton(c, precision))) return rc;
     const long es = (long)d.nb * (d.nb + 1);
     HIPCHK(launch_bsr_gather(c->nblocks, bsz, d.nch, d.npf, d.nb, es, c->blkptr, c->blklist, c->einv, c->Hx, c->Hg,
                              c->stream));

@@ -23,6 +23,7 @@
 #include "hfem_device.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace hfem {
 
@@ -41,9 +42,10 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 // it fits, else they overlay the shapvt derivatives and shapvg, which P0 then
 // reloads for the next element.
 struct AsmLayout {
-    int svt, svg, sft, sfg, pn, udg, uh, sh, jac, Xx, jacf, nl, M, Qm, C, E, udgg, fj, fhj, red;
+    int svt, svg, sft, sfg, pn, udg, uh, sh, jac, Xx, jacf, nl, M, Qm, C, E, udgg, fj, red;
     int xa, xha, cf, ef;
-    int total, nints, overlay;
+    int fuX2, fj2;          // second staging buffer of P8
+    int total, nints, overlay, dbuf;
 };
 
 __host__ __device__ inline AsmLayout asm_layout(const Dims &d)
@@ -70,10 +72,10 @@ __host__ __device__ inline AsmLayout asm_layout(const Dims &d)
     L.C = o; o += nd * npv * npv;
     L.E = o; o += nd * npv * ndf;
     L.udgg = o; o += ngv * nc;
-    // fhu of one row block (P8); the Ru/Rh tables (P9)
-    const int fjn = np * nc, tn = ngv * ncu * nd1 + np * nch;
-    L.fj = o; o += fjn > tn ? fjn : tn;
-    L.fhj = o; o += np * nch;
+    // fhu, fhuh of one row block (P8); the Ru/Rh tables (P9)
+    const int fjn = np * (nc + nch), tn = ngv * ncu * nd1 + 2 * np * nch;
+    L.fj = o;
+    o += fjn > tn ? fjn : tn;
     L.red = o; o += 64;
     L.nints = npf * d.nfe + npv * d.nfe + d.nfe + 8;
     const int kh = ngf * nch * (nc + nch) + (npv + ndf) * ngf * nd;
@@ -89,8 +91,45 @@ __host__ __device__ inline AsmLayout asm_layout(const Dims &d)
     L.xha = L.xa + ngf * nch * nc;
     L.cf = L.xha + ngf * nch * nch;
     L.ef = L.cf + npv * ngf * nd;
+    const int sb = ngv * nd * nc + np * nc + np * nch;
+    L.dbuf = (long)(o + sb) * 8 + (long)L.nints * 4 <= ASM_LDS_MAX;
+    L.fuX2 = L.fj2 = 0;
+    if (L.dbuf) {
+        L.fuX2 = o;
+        L.fj2 = o + ngv * nd * nc;
+        o += sb;
+    }
     L.total = o;
     return L;
+}
+
+// Volume geometry at one point: J(x,r) = sum_k dshape(k, r) pn[x*npv + k] with
+// dshape(k, r) the derivative r of shape function k, and adj(J)(r,x), so that
+// Xx(x,r) = adj[r][x] = jac d xi_r / d x_x.  Returns jac = det J.  Shared by the
+// flux and assembly kernels so that both see the same Xx.
+template <int ND, class DS>
+__device__ __forceinline__ double volume_adjugate(DS dshape, const double *pn, int npv, double adj[ND][ND])
+{
+    double J[ND][ND];
+    for (int x = 0; x < ND; x++)
+        for (int r = 0; r < ND; r++) {
+            double s = 0;
+            for (int k = 0; k < npv; k++) s += dshape(k, r) * pn[x * npv + k];
+            J[x][r] = s;
+        }
+    double det;
+    if (ND == 2) {
+        det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        adj[0][0] = J[1][1]; adj[0][1] = -J[0][1]; adj[1][0] = -J[1][0]; adj[1][1] = J[0][0];
+    } else {
+        for (int r = 0; r < ND; r++)
+            for (int x = 0; x < ND; x++) {
+                const int x1 = (x + 1) % 3, x2 = (x + 2) % 3, r1 = (r + 1) % 3, r2 = (r + 2) % 3;
+                adj[r][x] = J[x1][r1] * J[x2][r2] - J[x1][r2] * J[x2][r1];
+            }
+        det = J[0][0] * adj[0][0] + J[1][0] * adj[0][1] + J[2][0] * adj[0][2];
+    }
+    return det;
 }
 
 __device__ __forceinline__ double readlane_f64(double v, int l)
@@ -157,7 +196,7 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
     double *pn = lds + LY.pn, *udg = lds + LY.udg, *uh = lds + LY.uh, *sh = lds + LY.sh;
     double *jac = lds + LY.jac, *Xx = lds + LY.Xx, *jacf = lds + LY.jacf, *nl = lds + LY.nl;
     double *M = lds + LY.M, *Qm = lds + LY.Qm;
-    double *C = lds + LY.C, *E = lds + LY.E, *udgg = lds + LY.udgg, *fj = lds + LY.fj, *fhj = lds + LY.fhj;
+    double *C = lds + LY.C, *E = lds + LY.E, *udgg = lds + LY.udgg, *fj = lds + LY.fj;
     double *red = lds + LY.red;
     double *xa = lds + LY.xa, *cf = lds + LY.cf, *ef = lds + LY.ef;
     double *G = svt + npv * ngv, *fuX = G;       // the shapvt-derivative region, reused
@@ -188,8 +227,9 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
     for (int t = blockIdx.x; t < n; t += gridDim.x) {
         const long e = e0 + t;
         const double *FB = fluxbuf + (long)t * fstride;
-        const double *f = FB + FL.f, *fu = FB + FL.fu, *fh = FB + FL.fh, *fhu = FB + FL.fhu, *fhuh = FB + FL.fhuh;
+        const double *fX = FB + FL.fX, *FJ = FB + FL.J, *fh = FB + FL.fh;
         const double *fb = FB + FL.fb, *fbu = FB + FL.fbu, *fbuh = FB + FL.fbuh;
+        const long ohu = (long)ngv * ND * NC, ohh = ohu + (long)np * NC;   // fhu, fhuh in a row block
         // ---------------- P0: gather ----------------
         // the previous element overwrote the shapvt derivatives (and, with the
         // overlay, shapvg)
@@ -216,25 +256,8 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
         for (int i = tid; i < ngv + np; i += nth) {
             if (i < ngv) {
                 const int g = i;
-                double J[ND][ND];
-                for (int x = 0; x < ND; x++)
-                    for (int r = 0; r < ND; r++) {
-                        double s = 0;
-                        for (int k = 0; k < npv; k++) s += SVT(g, k, 1 + r) * pn[x * npv + k];
-                        J[x][r] = s;
-                    }
-                double adj[ND][ND], det;
-                if (ND == 2) {
-                    det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-                    adj[0][0] = J[1][1]; adj[0][1] = -J[0][1]; adj[1][0] = -J[1][0]; adj[1][1] = J[0][0];
-                } else {
-                    for (int r = 0; r < ND; r++)
-                        for (int x = 0; x < ND; x++) {
-                            int x1 = (x + 1) % 3, x2 = (x + 2) % 3, r1 = (r + 1) % 3, r2 = (r + 2) % 3;
-                            adj[r][x] = J[x1][r1] * J[x2][r2] - J[x1][r2] * J[x2][r1];
-                        }
-                    det = J[0][0] * adj[0][0] + J[1][0] * adj[0][1] + J[2][0] * adj[0][2];
-                }
+                double adj[ND][ND];
+                const double det = volume_adjugate<ND>([&](int k, int r) { return SVT(g, k, 1 + r); }, pn, npv, adj);
                 jac[g] = det;
                 for (int x = 0; x < ND; x++)
                     for (int r = 0; r < ND; r++) Xx[g + ngv * (x + ND * r)] = adj[r][x];
@@ -356,68 +379,47 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
         // are never stored).
         {
             const int nrt = (ncu * npv + 15) / 16;
-            for (int j = 0; j < ncu; j++) {
-                __syncthreads();
-                // row block j of the flux Jacobians:
-                //   fuX[g + ngv*(r + ND*c)], fj[pp + np*c] = fhu(pp,j,c), fhj[pp + np*kc] = fhuh(pp,j,kc)
-                {
-                    // the thread index is made opaque here so that the staging index
-                    // arithmetic is recomputed per row block instead of being hoisted
-                    // (and held in registers through the MFMA tasks)
-                    int ti = tid;
-                    asm volatile("" : "+v"(ti));
-                    const int N1 = np * NC, N2 = N1 + np * nch, NU = ngv * NC;
-                    double vh[3], vu[2][ND];
+            // row block j of the flux Jacobians (one contiguous record of the flux kernel)
+            // into uX[g + ngv*(r + ND*c)] = sum_x fu(g,j,x,c) Xx(g,x,r) and
+            // hj[pp + np*c] = fhu(pp,j,c), then hj[np*NC + pp + np*kc] = fhuh(pp,j,kc), by
+            // threads ti = 0 .. nt-1, U loads in flight per thread before the stores
+            auto stage = [&](auto Ut, int j, int ti, int nt, double *uX, double *hj) {
+                constexpr int U = decltype(Ut)::value;
+                // opaque index: the index arithmetic is not hoisted into the callers' loops
+                asm volatile("" : "+v"(ti));
+                const double *src = FJ + (long)j * FL.sj;
+                const int N = (int)FL.sj, NU3 = ngv * ND * NC;
+                for (int i0 = ti; i0 < N; i0 += U * nt) {
+                    double v[U];
 #pragma unroll
-                    for (int u = 0; u < 3; u++) {
-                        const int i = min(ti + u * nth, N2 - 1);
-                        const double *src = i < N1 ? fhu + (i % np) + np * (j + nch * (i / np))
-                                                   : fhuh + ((i - N1) % np) + np * (j + nch * ((i - N1) / np));
-                        vh[u] = *src;
-                    }
+                    for (int u = 0; u < U; u++) v[u] = src[min(i0 + u * nt, N - 1)];
 #pragma unroll
-                    for (int u = 0; u < 2; u++) {
-                        const int i = min(ti + u * nth, NU - 1), g = i % ngv, c = i / ngv;
-#pragma unroll
-                        for (int x = 0; x < ND; x++) vu[u][x] = fu[g + ngv * (j + ncu * (x + ND * c))];
-                    }
-#pragma unroll
-                    for (int u = 0; u < 3; u++) {
-                        const int i = ti + u * nth;
-                        if (i < N1) fj[i] = vh[u];
-                        else if (i < N2) fhj[i - N1] = vh[u];
-                    }
-#pragma unroll
-                    for (int u = 0; u < 2; u++) {
-                        const int i = ti + u * nth;
-                        if (i < NU) {
-                            const int g = i % ngv, c = i / ngv;
-#pragma unroll
-                            for (int r = 0; r < ND; r++) {
-                                double s = 0.0;
-#pragma unroll
-                                for (int x = 0; x < ND; x++) s += vu[u][x] * Xx[g + ngv * (x + ND * r)];
-                                fuX[g + ngv * (r + ND * c)] = s;
-                            }
-                        }
-                    }
-                    // sizes beyond the unrolled part
-                    for (int i = ti + 3 * nth; i < N2; i += nth) {
-                        if (i < N1) fj[i] = fhu[(i % np) + np * (j + nch * (i / np))];
-                        else fhj[i - N1] = fhuh[((i - N1) % np) + np * (j + nch * ((i - N1) / np))];
-                    }
-                    for (int i = ti + 2 * nth; i < NU; i += nth) {
-                        const int g = i % ngv, c = i / ngv;
-                        for (int r = 0; r < ND; r++) {
-                            double s = 0.0;
-                            for (int x = 0; x < ND; x++)
-                                s += fu[g + ngv * (j + ncu * (x + ND * c))] * Xx[g + ngv * (x + ND * r)];
-                            fuX[g + ngv * (r + ND * c)] = s;
-                        }
+                    for (int u = 0; u < U; u++) {
+                        const int i = i0 + u * nt;
+                        if (i < NU3) uX[i] = v[u];
+                        else if (i < N) hj[i - NU3] = v[u];
                     }
                 }
+            };
+            // With the second buffer the last wave stages row block j+1 while the
+            // others contract row block j; without it all waves stage between blocks.
+            const bool dbuf = LY.dbuf;
+            const int ntw = dbuf ? nwave - 1 : nwave;
+            double *uXb[2] = {fuX, lds + LY.fuX2}, *hjb[2] = {fj, lds + LY.fj2};
+            stage(std::integral_constant<int, 8>(), 0, tid, nth, uXb[0], hjb[0]);
+            for (int j = 0; j < ncu; j++) {
                 __syncthreads();
-                for (int rt = wave; rt < nrt; rt += nwave) {
+                const int cb = dbuf ? (j & 1) : 0;
+                if (!dbuf && j > 0) {
+                    stage(std::integral_constant<int, 8>(), j, tid, nth, uXb[0], hjb[0]);
+                    __syncthreads();
+                }
+                if (dbuf && wave == nwave - 1) {
+                    if (j + 1 < ncu) stage(std::integral_constant<int, 24>(), j + 1, lane, 64, uXb[cb ^ 1], hjb[cb ^ 1]);
+                    continue;
+                }
+                const double *uX = uXb[cb], *hj = hjb[cb], *hhj = hj + np * NC;
+                for (int rt = wave; rt < nrt; rt += ntw) {
                     // this lane's A row (kc, i) and its accumulator rows
                     const int ra = rt * 16 + lr;
                     const bool va = ra < ncu * npv;
@@ -450,14 +452,14 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                         for (int r = 0; r < ND1; r++) s[r] = SVG(ia, g, r);
                         double a0 = tdj * jac[g] * s[0];
 #pragma unroll
-                        for (int r = 0; r < ND; r++) a0 -= fuX[g + ngv * (r + ND * kca)] * s[r + 1];
+                        for (int r = 0; r < ND; r++) a0 -= uX[g + ngv * (r + ND * kca)] * s[r + 1];
                         a0 = vg ? a0 : 0.0;
                         double ax[ND];
 #pragma unroll
                         for (int x = 0; x < ND; x++) {
                             double v = 0.0;
 #pragma unroll
-                            for (int r = 0; r < ND; r++) v += fuX[g + ngv * (r + ND * ((x + 1) * ncu + kca))] * s[r + 1];
+                            for (int r = 0; r < ND; r++) v += uX[g + ngv * (r + ND * ((x + 1) * ncu + kca))] * s[r + 1];
                             ax[x] = vg ? -v : 0.0;
                         }
                         double b[TD], q[KT];
@@ -500,15 +502,15 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                             const double qv = Qm[kq[kt] + npv * (ngv + pp)];
                             q[kt] = kqv[kt] ? qv : 0.0;
                         }
-                        const double ad = af * fj[pp + np * kca];
-                        const double ah = af * fhj[pp + np * kca];
+                        const double ad = af * hj[pp + np * kca];
+                        const double ah = af * hhj[pp + np * kca];
 #pragma unroll
                         for (int t = 0; t < TD; t++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, bu[t], acc[t], 0, 0, 0);
 #pragma unroll
                         for (int t = TD; t < NT; t++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ah, bu[t], acc[t], 0, 0, 0);
 #pragma unroll
                         for (int x = 0; x < ND; x++) {
-                            const double aq = af * fj[pp + np * ((x + 1) * ncu + kca)];
+                            const double aq = af * hj[pp + np * ((x + 1) * ncu + kca)];
 #pragma unroll
                             for (int kt = 0; kt < KT; kt++)
                                 w[x][kt] = __builtin_amdgcn_mfma_f64_16x16x4f64(q[kt], aq, w[x][kt], 0, 0, 0);
@@ -559,9 +561,9 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
         // ---------------- P9: Ru, Rh ----------------
         // elementint's residual rows (:418-549) and faceint's (:862-1072, boundary
         // faces with fbou): tables T[g + ngv*(j + ncu*r)] (r = 0 the time term, r > 0
-        // f . Xx) and the face fluxes times jacf (interior fh for Ru; fh or fb for Rh)
+        // f . Xx from the flux kernel) and the face fluxes times jacf (interior fh for Ru; fh or fb for Rh)
         {
-            double *T = fj, *Fi = fj + ngv * ncu * nd1, *Fo = fhj;
+            double *T = fj, *Fi = fj + ngv * ncu * nd1, *Fo = Fi + np * nch;
             const int NT0 = ngv * ncu * nd1, NT1 = NT0 + np * nch, NT2 = NT1 + np * nch;
             for (int i = tid; i < NT2; i += nth) {
                 if (i < NT0) {
@@ -574,8 +576,7 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                             s = (sv - udgg[g + ngv * jj] * ph.fc_u) * jac[g];
                         }
                     } else {
-#pragma unroll
-                        for (int x = 0; x < ND; x++) s += f[g + ngv * (jj + ncu * x)] * Xx[g + ngv * (x + ND * (r - 1))];
+                        s = fX[g + ngv * (jj + ncu * (r - 1))];
                     }
                     T[i] = s;
                 } else if (i < NT1) {
@@ -640,7 +641,8 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
             const long nH = (long)nb * nb;
             for (int a = 0; a < nfe; a++) {
                 const bool bd = bnd[a];
-                const double *xu = bd ? fbu : fhu, *xuh = bd ? fbuh : fhuh;
+                const double *xu = bd ? fbu : FJ + ohu, *xuh = bd ? fbuh : FJ + ohh;
+                const long sju = bd ? (long)np * NC : FL.sj, sjh = bd ? (long)np * NCU : FL.sj;
                 __syncthreads();
                 // this face's flux Jacobians: xa[g + ngf*(j + nch*c)] (c < nc), then
                 // xha[g + ngf*(j + nch*kc)] right after them
@@ -652,7 +654,7 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                         for (int u = 0; u < 3; u++) {
                             const int i = min(i0 + u * nth, N - 1);
                             const int g = i % ngf, r = i / ngf, jj = r % nch, c = r / nch, pp = g + ngf * a;
-                            const double *src = c < nc ? xu + pp + np * (jj + nch * c) : xuh + pp + np * (jj + nch * (c - nc));
+                            const double *src = c < nc ? xu + pp + np * c + sju * jj : xuh + pp + np * (c - nc) + sjh * jj;
                             v[u] = *src;
                         }
 #pragma unroll
@@ -779,7 +781,8 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
 // fb = (A+|A|)(u-uh) - (A-|A|)(ui-uh) = A(uh)(u-ui) + |A|(uh)(u+ui-2uh), with A the
 // normal flux Jacobian at uh (getan).  Column k of d fb / d uh goes to fbuh; the
 // direction-0 call also writes fb and d fb / d u (zero in the q columns).  Point
-// arrays are strided by np: uh[np*c], u[np*c], nl[np*x], outputs [np*(j + NCU*col)].
+// arrays are strided by np: uh[np*c], u[np*c], nl[np*x]; outputs fb[np*j],
+// fbu[np*(col + NC*j)], fbuh[np*(col + NCU*j)].
 template <int ND>
 __device__ __forceinline__ void boundary_point(const Phys &ph, const double *uh, const double *u, const double *nl,
                                             int np, int k, double *fb, double *fbu, double *fbuh)
@@ -800,15 +803,15 @@ __device__ __forceinline__ void boundary_point(const Phys &ph, const double *uh,
         if (k == 0)
             for (int q = 0; q < NCU; q++)
                 for (int j = 0; j < NCU; j++) {
-                    double *o = &fbu[np * (j + NCU * q)];
+                    double *o = &fbu[np * (q + NC * j)];
                     *o = absflag ? *o + Am[q * NCU + j].v : Am[q * NCU + j].v;
                 }
     }
-    for (int j = 0; j < NCU; j++) fbuh[np * (j + NCU * k)] = acc[j].d;
+    for (int j = 0; j < NCU; j++) fbuh[np * (k + NCU * j)] = acc[j].d;
     if (k == 0)
         for (int j = 0; j < NCU; j++) {
             fb[np * j] = acc[j].v;
-            for (int q = NCU; q < NC; q++) fbu[np * (j + NCU * q)] = 0.0;
+            for (int q = NCU; q < NC; q++) fbu[np * (q + NC * j)] = 0.0;
         }
 }
 
@@ -835,6 +838,7 @@ flux_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, const
     double *udgg = p; p += ngv * nc;
     double *ufg = p; p += np * nc;
     double *uhgl = p; p += np * nch;
+    double *Xx = p; p += ngv * ND * ND;    // Xx(g,x,r) at [g + ngv*(x + ND*r)]
     int *bnd = (int *)p;
     const FluxLayout FL = flux_layout(d);
     const double tau = ph.param[5];
@@ -844,8 +848,9 @@ flux_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, const
     for (int t = blockIdx.x; t < n; t += gridDim.x) {
         const long e = e0 + t;
         double *F = fluxbuf + (long)t * fstride;
-        double *f = F + FL.f, *fu = F + FL.fu, *fh = F + FL.fh, *fhu = F + FL.fhu, *fhuh = F + FL.fhuh;
+        double *fX = F + FL.fX, *FJ = F + FL.J, *fh = F + FL.fh;
         double *fb = F + FL.fb, *fbu = F + FL.fbu, *fbuh = F + FL.fbuh;
+        const long sj = FL.sj, ohu = (long)ngv * ND * NC, ohh = ohu + (long)np * NC;
         for (int i = tid; i < npv * ND; i += nth) pn[i] = dgnodes[e * npv * ND + i];
         for (int i = tid; i < npv * nc; i += nth) udg[i] = UDG[e * npv * nc + i];
         for (int i = tid; i < ndf * nch; i += nth) {
@@ -855,8 +860,14 @@ flux_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, const
         for (int a = tid; a < nfe; a += nth) bnd[a] = bf[e * nfe + a] < 0;
         __syncthreads();
         // unit normals at face points, solution at volume and face points
-        for (int i = tid; i < np + ngv * nc + np * nc + np * nch; i += nth) {
-            if (i < np) {
+        for (int i = tid; i < np + ngv * nc + np * nc + np * nch + ngv; i += nth) {
+            if (i >= np + ngv * nc + np * nc + np * nch) {
+                const int g = i - (np + ngv * nc + np * nc + np * nch);
+                double adj[ND][ND];
+                volume_adjugate<ND>([&](int k, int r) { return SVT(g, k, 1 + r); }, pn, npv, adj);
+                for (int x = 0; x < ND; x++)
+                    for (int r = 0; r < ND; r++) Xx[g + ngv * (x + ND * r)] = adj[r][x];
+            } else if (i < np) {
                 const int pp = i, g = pp % ngf, a = pp / ngf;
                 double tg[2][3];
                 for (int r = 0; r < ND - 1; r++)
@@ -901,10 +912,18 @@ flux_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, const
                 dual w[NC], fl[NCU * ND];
                 for (int c = 0; c < NC; c++) w[c] = mk(udgg[g + ngv * c], c == k ? 1.0 : 0.0);
                 ns_flux<ND>(ph.param, w, fl);
-                for (int jx = 0; jx < NCU * ND; jx++) {
-                    fu[g + ngv * (jx + NCU * ND * k)] = fl[jx].d;
-                    if (k == 0) f[g + ngv * jx] = fl[jx].v;
-                }
+                // folded with Xx: row block j, column k of d f / d udg, and f itself
+                for (int j = 0; j < NCU; j++)
+                    for (int r = 0; r < ND; r++) {
+                        double sd = 0.0, sv = 0.0;
+                        for (int x = 0; x < ND; x++) {
+                            const double xx = Xx[g + ngv * (x + ND * r)];
+                            sd += fl[j + NCU * x].d * xx;
+                            sv += fl[j + NCU * x].v * xx;
+                        }
+                        FJ[j * sj + g + ngv * (r + ND * k)] = sd;
+                        if (k == 0) fX[g + ngv * (j + NCU * r)] = sv;
+                    }
             }
             for (int r = tid; r < nfac; r += nth) {
                 const int pp = r % np, k = r / np;
@@ -915,8 +934,8 @@ flux_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, const
                 for (int j = 0; j < NCU; j++) {
                     dual fn = mk(0.0);
                     for (int x = 0; x < ND; x++) fn = fn + nl[pp + np * x] * fl[j + NCU * x];
-                    fhu[pp + np * (j + NCU * k)] = (k < NCU) ? (k == j ? tau : 0.0) : fn.d;
-                    if (k < NCU) fhuh[pp + np * (j + NCU * k)] = fn.d - (k == j ? tau : 0.0);
+                    FJ[j * sj + ohu + pp + np * k] = (k < NCU) ? (k == j ? tau : 0.0) : fn.d;
+                    if (k < NCU) FJ[j * sj + ohh + pp + np * k] = fn.d - (k == j ? tau : 0.0);
                     if (k == 0) fh[pp + np * j] = fn.v + tau * (ufg[pp + np * j] - w[j].v);
                 }
             }
@@ -984,25 +1003,8 @@ getq_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, doubl
         for (int i = tid; i < ngv + np; i += nth) {
             if (i < ngv) {
                 const int g = i;
-                double J[ND][ND];
-                for (int x = 0; x < ND; x++)
-                    for (int r = 0; r < ND; r++) {
-                        double s = 0;
-                        for (int k = 0; k < npv; k++) s += SVT(g, k, 1 + r) * pn[x * npv + k];
-                        J[x][r] = s;
-                    }
-                double adj[ND][ND], det;
-                if (ND == 2) {
-                    det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-                    adj[0][0] = J[1][1]; adj[0][1] = -J[0][1]; adj[1][0] = -J[1][0]; adj[1][1] = J[0][0];
-                } else {
-                    for (int r = 0; r < ND; r++)
-                        for (int x = 0; x < ND; x++) {
-                            int x1 = (x + 1) % 3, x2 = (x + 2) % 3, r1 = (r + 1) % 3, r2 = (r + 2) % 3;
-                            adj[r][x] = J[x1][r1] * J[x2][r2] - J[x1][r2] * J[x2][r1];
-                        }
-                    det = J[0][0] * adj[0][0] + J[1][0] * adj[0][1] + J[2][0] * adj[0][2];
-                }
+                double adj[ND][ND];
+                const double det = volume_adjugate<ND>([&](int k, int r) { return SVT(g, k, 1 + r); }, pn, npv, adj);
                 jac[g] = det;
                 for (int x = 0; x < ND; x++)
                     for (int r = 0; r < ND; r++) Xx[g + ngv * (x + ND * r)] = adj[r][x];
@@ -1118,7 +1120,7 @@ size_t getq_lds_bytes(const Dims &d)
 size_t flux_lds_bytes(const Dims &d)
 {
     size_t n = (size_t)d.npv * d.nd + (size_t)d.npv * d.nc + (size_t)d.ndf * d.nch + (size_t)d.np * d.nd +
-               (size_t)d.ngv * d.nc + (size_t)d.np * d.nc + (size_t)d.np * d.nch;
+               (size_t)d.ngv * d.nc + (size_t)d.np * d.nc + (size_t)d.np * d.nch + (size_t)d.ngv * d.nd * d.nd;
     return n * sizeof(double) + 16 * sizeof(int);
 }
 

@@ -399,4 +399,71 @@ hipError_t launch_condense_mixed(int na, int nb, int ldx, int e0, int n, const d
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Right-hand-side condensation with stored operands (schurElementVector,
+// assembly/elementschur.cpp:345-458, schurImplementation 0): per element
+//   y = inv(D) Ru,   FH = Rh - K y
+// with inv(D) (column-major na x na) and K (column-major nb x na) kept from the
+// last Jacobian in T = double or float (quasiNewtonAccuracy 1 / 0; the float
+// copies are widened to double before use, as the reference does).  y goes to
+// column nb of X (inv(D) Ru, row-major X), FH to column nb of Hx.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256)
+schur_vector_kernel(int na, int nb, int ldx, int ne, const T *__restrict__ Dinv, const T *__restrict__ K,
+                    const double *__restrict__ Ru, const double *__restrict__ Rh, double *__restrict__ X,
+                    double *__restrict__ Hx)
+{
+    extern __shared__ double sv[];
+    double *ru = sv, *y = sv + na;
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const long es = (long)nb * (nb + 1);
+    for (int e = blockIdx.x; e < ne; e += gridDim.x) {
+        for (int i = tid; i < na; i += nth) ru[i] = Ru[(long)e * na + i];
+        __syncthreads();
+        const T *De = Dinv + (long)e * na * na;
+        for (int i = tid; i < na; i += nth) {
+            double s = 0.0;
+            for (int k = 0; k < na; k++) s += (double)De[i + (long)na * k] * ru[k];
+            y[i] = s;
+            X[((long)e * na + i) * ldx + nb] = s;
+        }
+        __syncthreads();
+        const T *Ke = K + (long)e * nb * na;
+        for (int r = tid; r < nb; r += nth) {
+            double s = Rh[(long)e * nb + r];
+            for (int k = 0; k < na; k++) s -= (double)Ke[r + (long)nb * k] * y[k];
+            Hx[e * es + (long)nb * nb + r] = s;
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_schur_vector(int na, int nb, int ldx, int ne, int fp32, const void *Dinv, const void *K,
+                               const double *Ru, const double *Rh, double *X, double *Hx, hipStream_t s)
+{
+    const int grid = std::min(ne, 4096);
+    const size_t lds = 2 * (size_t)na * sizeof(double);
+    if (fp32)
+        hipLaunchKernelGGL(schur_vector_kernel<float>, dim3(grid), dim3(256), lds, s, na, nb, ldx, ne,
+                           (const float *)Dinv, (const float *)K, Ru, Rh, X, Hx);
+    else
+        hipLaunchKernelGGL(schur_vector_kernel<double>, dim3(grid), dim3(256), lds, s, na, nb, ldx, ne,
+                           (const double *)Dinv, (const double *)K, Ru, Rh, X, Hx);
+    return hipGetLastError();
+}
+
+__global__ void to_f32_kernel(long n, const double *__restrict__ x, float *__restrict__ y)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        y[i] = (float)x[i];
+}
+
+hipError_t launch_to_f32(long n, const double *x, float *y, hipStream_t s)
+{
+    const long grid = std::min<long>((n + 255) / 256, 8192);
+    if (grid > 0) hipLaunchKernelGGL(to_f32_kernel, dim3(grid), dim3(256), 0, s, n, x, y);
+    return hipGetLastError();
+}
+
 }  // namespace hfem

@@ -20,6 +20,9 @@ hipError_t launch_condense_mixed(int na, int nb, int ldx, int e0, int n, const d
                                  double *Rw, const double *K, double *Hx, int *info, int grid, int sweeps,
                                  hipStream_t s);
 size_t condense_lds_bytes(int na, bool dinv_in_lds, int elt);
+hipError_t launch_schur_vector(int na, int nb, int ldx, int ne, int fp32, const void *Dinv, const void *K,
+                               const double *Ru, const double *Rh, double *X, double *Hx, hipStream_t s);
+hipError_t launch_to_f32(long n, const double *x, float *y, hipStream_t s);
 size_t assemble_lds_bytes(const Dims &d);
 bool assemble_supported(int nd, int npv, int npf);
 hipError_t launch_bsr_gather(int, int, int, int, int, long, const int *, const int *, const int *, const double *,
@@ -131,6 +134,10 @@ struct hfem_ctx {
     // trace system
     double *Hg = nullptr, *Rg = nullptr, *r = nullptr, *partial = nullptr, *xv = nullptr, *yv = nullptr;
     float ms_asm = 0, ms_cond = 0;
+    // quasi-Newton: inv(D) and K of the last Jacobian (qn_acc -1 off, 0 float, 1 double)
+    int qn_acc = -1;
+    bool qn_valid = false;
+    void *qn_dinv = nullptr, *qn_K = nullptr;
     // trace-system solver (solve.hip)
     SolverState sv;
 };

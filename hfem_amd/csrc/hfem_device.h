@@ -151,8 +151,16 @@ __device__ __forceinline__ void ns_normal_jac(double gam, double eps, const dual
 }
 
 // Per-element pointwise flux data written by the flux kernel, read by assembly.
+// Volume quantities come folded with the Jacobian adjugate Xx(g,x,r) = jac d xi_r/d x_x:
+//   fX[g + ngv*(j + ncu*r)]                  sum_x f(g,j,x) Xx(g,x,r)
+//   row block j at J + j*sj (one contiguous record per row of the flux Jacobians):
+//     [g + ngv*(r + nd*c)]                   sum_x d f(g,j,x)/d udg_c Xx(g,x,r)
+//     [ngv*nd*nc + pp + np*c]                d fhat(pp,j)/d udg_c            (fhu)
+//     [ngv*nd*nc + np*nc + pp + np*k]        d fhat(pp,j)/d uh_k             (fhuh)
+//   fh[pp + np*j], fb[pp + np*j]            interior / far-field face flux
+//   fbu[pp + np*(c + nc*j)], fbuh[pp + np*(k + nch*j)]   far-field Jacobians
 struct FluxLayout {
-    long f, fu, fh, fhu, fhuh, fb, fbu, fbuh, total;
+    long fX, J, sj, fh, fb, fbu, fbuh, total;
 };
 
 __host__ __device__ inline FluxLayout flux_layout(const Dims &d)
@@ -160,11 +168,10 @@ __host__ __device__ inline FluxLayout flux_layout(const Dims &d)
     FluxLayout s;
     long o = 0;
     auto take = [&](long n) { long r = o; o += (n + 31) / 32 * 32; return r; };
-    s.f = take((long)d.ngv * d.ncu * d.nd);
-    s.fu = take((long)d.ngv * d.ncu * d.nd * d.nc);
+    s.fX = take((long)d.ngv * d.ncu * d.nd);
+    s.sj = (long)d.ngv * d.nd * d.nc + (long)d.np * d.nc + (long)d.np * d.nch;
+    s.J = take((long)d.ncu * s.sj);
     s.fh = take((long)d.np * d.nch);
-    s.fhu = take((long)d.np * d.nch * d.nc);
-    s.fhuh = take((long)d.np * d.nch * d.nch);
     s.fb = take((long)d.np * d.nch);
     s.fbu = take((long)d.np * d.nch * d.nc);
     s.fbuh = take((long)d.np * d.nch * d.nch);

@@ -7,6 +7,7 @@
 // (solvers/dgsolvers.cpp:142-205, :221-444) with status codes in place of exit().
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <set>
 #include <utility>
@@ -517,6 +518,8 @@ static int check_params(const hfem_solver_params *p)
     if (p->ordering != 0 && p->ordering != 1) return hfem_fail(-1, "solver parameters: ordering %d", p->ordering);
     if (p->orthog != 0 && p->orthog != 1) return hfem_fail(-1, "solver parameters: orthog %d", p->orthog);
     if (p->precision != HFEM_FP64 && p->precision != HFEM_MIXED) return hfem_fail(-1, "solver parameters: precision");
+    if (p->quasi_newton && (p->qn_accuracy < 0 || p->qn_accuracy > 1 || !(p->qn_reuse_ratio >= 0)))
+        return hfem_fail(-1, "solver parameters: quasi-Newton accuracy %d, reuse ratio", p->qn_accuracy);
     return 0;
 }
 
@@ -525,22 +528,51 @@ extern "C" int hfem_solve_steady(hfem_ctx *c, const hfem_solver_params *p, int *
 {
     RC(require_system(c, "hfem_solve_steady"));
     RC(check_params(p));
+    const bool qn = p->quasi_newton != 0;
+    if (qn) RC(hfem_quasi_newton_storage(c, p->qn_accuracy));
     const double alpha_min = 1.0e-4;
+    const int true_max = p->true_newton_maxiter > 0 ? p->true_newton_maxiter : INT_MAX;
     double rNorm = p->newton_tol + 1.0;
-    int it = 0;
-    while (rNorm > p->newton_tol && it < p->newton_maxiter) {
+    int it = 0, true_it = 0;
+    // quasi-Newton bookkeeping (sys.linearSolvesClocks, sys.lastAssemblyAndPrecClocks)
+    bool reuse = false;
+    double lin_time = 0.0, jac_time = 0.0;
+    auto now = [] { return std::chrono::steady_clock::now(); };
+    auto secs = [](std::chrono::steady_clock::time_point t0) {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    };
+    auto recompute_jacobian = [&] { reuse = false; lin_time = 0.0; };
+    while (rNorm > p->newton_tol && it < p->newton_maxiter && true_it < true_max) {
         it++;
+        const bool reused = reuse;
+        if (!reused) true_it++;
         double oldNorm;
-        RC(hfem_assemble_linear_system(c, p->precision, &oldNorm));
+        auto t0 = now();
+        if (!reused) {
+            RC(hfem_assemble_linear_system(c, p->precision, &oldNorm));
+            jac_time = secs(t0);
+        } else {
+            RC(hfem_assemble_rhs(c, &oldNorm));
+        }
         if (!std::isfinite(oldNorm)) return hfem_fail(-5, "hfem_solve_steady: residual norm is NaN or Inf");
         if (c->sv.order_method < 0 || (p->reuse_ordering == 0) || c->sv.order_method != p->ordering)
             RC(hfem_compute_ordering(c, p->ordering));
-        RC(hfem_precond_factor(c, p->precond));
+        if (!reused) {
+            auto t1 = now();
+            RC(hfem_precond_factor(c, p->precond));
+            jac_time += secs(t1);
+        }
         int gi = 0, gf = 0;
+        auto t2 = now();
         RC(hfem_gmres(c, p->restart, p->gmres_tol, p->gmres_maxit, p->orthog, &gi, nullptr, &gf));
+        lin_time += secs(t2);
         if (history) {
             history[4 * (it - 1)] = oldNorm;
-            history[4 * (it - 1) + 2] = gi;
+            history[4 * (it - 1) + 2] = reused ? -gi : gi;
+        }
+        if (qn) {
+            if (lin_time < p->qn_reuse_ratio * jac_time) reuse = true;
+            else recompute_jacobian();
         }
         if (it == 1 && oldNorm < p->newton_tol) {
             rNorm = oldNorm;
@@ -555,6 +587,7 @@ extern "C" int hfem_solve_steady(hfem_ctx *c, const hfem_solver_params *p, int *
             if (!std::isfinite(rNorm) || (rNorm > oldNorm && rNorm > p->newton_tol)) {
                 const double a = std::fabs(alpha / 2);
                 alpha = -a;
+                if (qn) recompute_jacobian();
                 if (a < alpha_min) {
                     // back to the previous iterate (robust mode, dgsolvers.cpp:176-180)
                     alpha = -2.0 * std::fabs(alpha);

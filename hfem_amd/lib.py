@@ -52,6 +52,8 @@ EXPORTS = {
     "hfem_read_elements": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
     "hfem_read_condensed": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _ip]),
     "hfem_assemble_linear_system": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "hfem_quasi_newton_storage": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "hfem_assemble_rhs": (ctypes.c_int, [_vp, _dp]),
     "hfem_read_system": (ctypes.c_int, [_vp, _dp, _dp]),
     "hfem_bsr_matvec": (ctypes.c_int, [_vp, _dp, _dp]),
     "hfem_compute_ordering": (ctypes.c_int, [_vp, ctypes.c_int]),
@@ -79,13 +81,15 @@ class hfem_solver_params(ctypes.Structure):
     _fields_ = [("newton_tol", ctypes.c_double), ("newton_maxiter", ctypes.c_int), ("precond", ctypes.c_int),
                 ("ordering", ctypes.c_int), ("reuse_ordering", ctypes.c_int), ("restart", ctypes.c_int),
                 ("gmres_tol", ctypes.c_double), ("gmres_maxit", ctypes.c_int), ("orthog", ctypes.c_int),
-                ("precision", ctypes.c_int)]
+                ("precision", ctypes.c_int), ("quasi_newton", ctypes.c_int), ("qn_accuracy", ctypes.c_int),
+                ("true_newton_maxiter", ctypes.c_int), ("qn_reuse_ratio", ctypes.c_double)]
 
     @classmethod
     def make(cls, newton_tol=1e-7, newton_maxiter=10, precond=0, ordering=1, reuse_ordering=1, restart=30,
-             gmres_tol=1e-6, gmres_maxit=1000, orthog=1, precision=0):
+             gmres_tol=1e-6, gmres_maxit=1000, orthog=1, precision=0, quasi_newton=0, qn_accuracy=1,
+             true_newton_maxiter=0, qn_reuse_ratio=3.0):
         return cls(newton_tol, newton_maxiter, precond, ordering, reuse_ordering, restart, gmres_tol, gmres_maxit,
-                   orthog, precision)
+                   orthog, precision, quasi_newton, qn_accuracy, true_newton_maxiter, qn_reuse_ratio)
 
 
 def load():
@@ -218,6 +222,16 @@ class HDGContext:
         _check(load().hfem_assemble_linear_system(self.h, precision, ctypes.byref(r)))
         return r.value
 
+    def quasi_newton_storage(self, accuracy=1):
+        """Keep inv(D) and K of every Jacobian: accuracy 0 float, 1 double, -1 off."""
+        _check(load().hfem_quasi_newton_storage(self.h, accuracy))
+
+    def assemble_rhs(self):
+        """assembleRHS with the stored Jacobian; returns rNorm."""
+        r = ctypes.c_double()
+        _check(load().hfem_assemble_rhs(self.h, ctypes.byref(r)))
+        return r.value
+
     def read_system(self):
         msh = self.prob.mesh
         bsz = self.d["nch"] * self.d["npf"]
@@ -286,7 +300,8 @@ class HDGContext:
 
     # -- drivers (solveNonlinearProblem, solveUnsteadyProblem) -------------------
     def solve_steady(self, params=None, **kw):
-        """Newton with line search; returns (converged, iterations, history[it, 4])."""
+        """Newton with line search; returns (converged, iterations, history[it, 4]);
+        history[:, 2] is negative for iterations that reused the Jacobian."""
         p = params or hfem_solver_params.make(**kw)
         conv = ctypes.c_int()
         nit = ctypes.c_int()

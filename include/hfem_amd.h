@@ -96,8 +96,19 @@ int hfem_read_condensed(hfem_ctx *ctx, int e0, int n, double *DinvRu, double *Di
 
 /* assembleLinearSystem: assemble + condense all elements, scatter AH/FH into the
  * BSR matrix Hg and right-hand side Rg (device), and return
- * rNorm = sqrt(sum_e |Ru_e|^2 + |r|^2) with r the assembled uncondensed Rh. */
+ * rNorm = sqrt(sum_e |Ru_e|^2 + |r|^2) with r the assembled uncondensed Rh.
+ * With quasi-Newton storage enabled it also keeps inv(D) and K of every element
+ * (updateQuasiNewtonStructures, assembly/globalassembly.cpp:327-389). */
 int hfem_assemble_linear_system(hfem_ctx *ctx, int precision, double *rNorm);
+/* Quasi-Newton storage (app.quasiNewton, sys.quasiNewtonAccuracy,
+ * preprocessing/initializestructs.cpp:193,223-230): accuracy -1 off, 0 inv(D) and K
+ * kept in float (DinvFloat, Kfloat), 1 kept in double. */
+int hfem_quasi_newton_storage(hfem_ctx *ctx, int accuracy);
+/* assembleRHS (assembly/globalassembly.cpp:559-616): Rg, r and rNorm at the current
+ * state with the stored inv(D) and K of the last hfem_assemble_linear_system
+ * (schurElementVector, assembly/elementschur.cpp:345-458); Hg is left as it is.
+ * inv(D) Ru is updated for hfem_update_solution; inv(D) F stays the stored one. */
+int hfem_assemble_rhs(hfem_ctx *ctx, double *rNorm);
 /* Copy Hg (nblocks*bsz*bsz, column-major blocks) and Rg (nf*bsz) to the host. */
 int hfem_read_system(hfem_ctx *ctx, double *Hg, double *Rg);
 /* y = Hg x with host vectors of length nf*bsz. */
@@ -150,11 +161,20 @@ typedef struct {
     int gmres_maxit;
     int orthog;          /* 0 MGS, 1 ICGS */
     int precision;       /* condensation precision, HFEM_FP64 or HFEM_MIXED */
+    int quasi_newton;    /* 1: reuse the Jacobian and its preconditioner (app.quasiNewton) */
+    int qn_accuracy;     /* stored inv(D) and K: 0 float, 1 double (quasiNewtonAccuracy) */
+    int true_newton_maxiter; /* limit on iterations that assemble the Jacobian (trueNewtonMaxiter); 0 none */
+    double qn_reuse_ratio;   /* reuse while the GMRES time since the last Jacobian is below this
+                                multiple of that Jacobian's assembly + preconditioner time (3 in
+                                solvers/dgsolvers.cpp:137-141) */
 } hfem_solver_params;
 
 /* solveNonlinearProblem (solvers/dgsolvers.cpp:142-205): Newton with the
- * reference's halving line search.  history (4 doubles per iteration, may be
- * NULL): residual before, residual after, GMRES iterations, accumulated step. */
+ * reference's halving line search and, with quasi_newton, its Jacobian reuse
+ * (solveLinearProblem :40-105; a damped step or a reuse test that fails
+ * recomputes the Jacobian).  history (4 doubles per iteration, may be NULL):
+ * residual before, residual after, GMRES iterations (negative when the
+ * iteration reused the Jacobian), accumulated step. */
 int hfem_solve_steady(hfem_ctx *ctx, const hfem_solver_params *p, int *converged, int *niter, double *history);
 /* One time step of solveUnsteadyProblem (solvers/dgsolvers.cpp:221-444) with
  * DIRK(nstage, order) of solvers/DIRKcoeff.cpp; each stage is a steady solve

@@ -13,6 +13,13 @@ preconditioner and GMRES exactly as the reference's drivers call them:
 * ``residual_norm``: computeResidualNorm, assembly/globalassembly.cpp:618-678.
 * ``dirk_coefficients`` / ``dirk_step``: DIRKcoeff (solvers/DIRKcoeff.cpp:6-118)
   and the stage loop of solveUnsteadyProblem (solvers/dgsolvers.cpp:221-444).
+* ``trace_system_qn`` / ``assemble_rhs`` / ``newton_solve_qn``: the quasi-Newton
+  path -- updateQuasiNewtonStructures (assembly/globalassembly.cpp:327-389),
+  assembleRHS (:559-616) with schurElementVector (assembly/elementschur.cpp:345-458)
+  and the Jacobian reuse of solveLinearProblem / solveNonlinearProblem
+  (solvers/dgsolvers.cpp:40-205).  The stored operand is inv(D) (the explicit
+  inverse this build's condensation forms) rather than the reference's LU factors;
+  in exact arithmetic both give the same inv(D) Ru.
 """
 import dataclasses
 
@@ -153,3 +160,106 @@ def dirk_step(prob, UDG, UH, SH, dt, nstage=3, order=3, **newton):
             raise RuntimeError(f"DIRK stage {s + 1} did not converge")
         hist.append(h)
     return UDG, UH, SH, hist
+
+
+def quasi_newton_store(blocks, accuracy=1):
+    """inv(D) and K of every element from uncondensed blocks (element-major flat,
+    column-major); rounded to float and widened back for accuracy 0 (DinvFloat,
+    Kfloat), as schurElementVector reads them."""
+    n, na = blocks["Ru"].shape
+    Dinv = np.empty_like(blocks["D"])
+    for e in range(n):
+        A, info = O.dense_inverse(blocks["D"][e].reshape(na, na, order="F"))
+        if info:
+            raise FloatingPointError(f"singular D in element {e}")
+        Dinv[e] = A.reshape(-1, order="F")
+    K = blocks["K"].copy()
+    if accuracy == 0:
+        Dinv = Dinv.astype(np.float32).astype(np.float64)
+        K = K.astype(np.float32).astype(np.float64)
+    return dict(Dinv=Dinv, K=K)
+
+
+def trace_system_qn(prob, UDG, UH, SH, accuracy=1):
+    """assembleLinearSystem with quasiNewton = 1: trace_system's outputs plus the
+    stored inv(D) and K of this Jacobian."""
+    orc = O.Oracle(prob)
+    raw = orc.assemble(0, prob.ne, UDG=UDG, UH=UH, SH=SH)
+    stored = quasi_newton_store(raw, accuracy)
+    M, Rg, rnorm, blocks = O.trace_system(prob, UDG=UDG, UH=UH, SH=SH)
+    return M, Rg, rnorm, blocks, stored
+
+
+def assemble_rhs(prob, UDG, UH, SH, stored):
+    """assembleRHS: element vectors at (UDG, UH, SH) condensed with the stored
+    operands, y = inv(D) Ru and FH = Rh - K y, scattered into Rg; rNorm from the
+    uncondensed Ru and the assembled uncondensed Rh.  Returns (Rg, rNorm, y)."""
+    d = prob.dims
+    na, nb, ne = d["na"], d["nb"], prob.ne
+    b = O.Oracle(prob).assemble(0, ne, UDG=UDG, UH=UH, SH=SH)
+    zeros = np.zeros((ne, nb * nb))
+    _, r = O.scatter(prob, zeros, b["Rh"])
+    rnorm = float(np.sqrt((b["Ru"] ** 2).sum() + (r ** 2).sum()))
+    y = np.empty((ne, na))
+    FH = np.empty((ne, nb))
+    for e in range(ne):
+        Dinv = stored["Dinv"][e].reshape(na, na, order="F")
+        K = stored["K"][e].reshape(nb, na, order="F")
+        y[e] = Dinv @ b["Ru"][e]
+        FH[e] = b["Rh"][e] - K @ y[e]
+    _, Rg = O.scatter(prob, zeros, FH)
+    return Rg, rnorm, y
+
+
+def newton_solve_qn(prob, UDG, UH, SH, newton_tol=1e-7, maxiter=10, alpha_min=1e-4, accuracy=1, order=None,
+                    **lin):
+    """solveNonlinearProblem with quasiNewton = 1 where the time test of
+    solvers/dgsolvers.cpp:137-141 always passes: after an iteration the Jacobian
+    (Hg, its preconditioner, the stored inv(D), K and inv(D) F) is reused until a
+    damped step recomputes it (:171).  history records ``reused`` per iteration."""
+    order = {} if order is None else order
+    UDG = np.array(UDG, dtype=float)
+    UH = np.array(UH, dtype=float)
+    rnorm = newton_tol + 1.0
+    history = []
+    it = 0
+    reuse = False
+    M = stored = DinvF = None
+    while rnorm > newton_tol and it < maxiter:
+        it += 1
+        reused = reuse
+        if not reused:
+            M, Rg, old, blocks, stored = trace_system_qn(prob, UDG, UH, SH, accuracy)
+            DinvF = blocks["F"]
+            upd = blocks
+        else:
+            Rg, old, y = assemble_rhs(prob, UDG, UH, SH, stored)
+            upd = dict(F=DinvF, Ru=y)
+        if not np.isfinite(old):
+            raise FloatingPointError("residual norm is NaN or Inf")
+        x, flag, iters, rev = linear_solve(prob, M, Rg, order=order, **lin)
+        rec = dict(old=old, gmres_iters=iters, gmres_flag=flag, alphas=[], reused=reused)
+        history.append(rec)
+        reuse = True
+        if it == 1 and old < newton_tol:
+            rnorm = old
+            rec["new"] = rnorm
+            break
+        alpha = 1.0
+        while True:
+            UDG, UH = update_solution(prob, UDG, UH, SH, upd, x, alpha)
+            rec["alphas"].append(alpha)
+            rnorm = residual_norm(prob, UDG, UH, SH)
+            if not np.isfinite(rnorm) or (rnorm > old and rnorm > newton_tol):
+                a = abs(alpha / 2)
+                alpha = -a
+                reuse = False
+                if a < alpha_min:
+                    alpha = -2.0 * abs(alpha)
+                    UDG, UH = update_solution(prob, UDG, UH, SH, upd, x, alpha)
+                    rec["alphas"].append(alpha)
+                    break
+            else:
+                break
+        rec["new"] = rnorm
+    return UDG, UH, rnorm < newton_tol, history
