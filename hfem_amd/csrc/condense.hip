@@ -353,6 +353,239 @@ condense_mixed_kernel(int na, int nb, int ldx, int e0, int n, const double *__re
     }
 }
 
+// Maximum of a 64-bit unsigned value over the wave, returned to every lane.  DPP
+// steps (quad exchanges, row shifts by 4 and 8, row broadcasts 15 and 31) leave
+// the maximum in lane 63; lanes a step does not write keep their own value.
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
+{
+#define HFEM_DPP_MAX(ctrl, rm, bm)                                                                           \
+    {                                                                                                        \
+        const int hi = (int)(v >> 32), lo = (int)(unsigned)v;                                                \
+        const int h2 = __builtin_amdgcn_update_dpp(hi, hi, ctrl, rm, bm, false);                             \
+        const int l2 = __builtin_amdgcn_update_dpp(lo, lo, ctrl, rm, bm, false);                             \
+        const unsigned long long o = ((unsigned long long)(unsigned)h2 << 32) | (unsigned)l2;                \
+        v = o > v ? o : v;                                                                                   \
+    }
+    HFEM_DPP_MAX(0xb1, 0xf, 0xf)    // quad_perm [1,0,3,2]
+    HFEM_DPP_MAX(0x4e, 0xf, 0xf)    // quad_perm [2,3,0,1]
+    HFEM_DPP_MAX(0x114, 0xf, 0xe)   // row_shr:4
+    HFEM_DPP_MAX(0x118, 0xf, 0xc)   // row_shr:8
+    HFEM_DPP_MAX(0x142, 0xa, 0xf)   // row_bcast:15
+    HFEM_DPP_MAX(0x143, 0xc, 0xf)   // row_bcast:31
+#undef HFEM_DPP_MAX
+    const int hi = __builtin_amdgcn_readlane((int)(v >> 32), 63), lo = __builtin_amdgcn_readlane((int)(unsigned)v, 63);
+    return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+
+// ---------------------------------------------------------------------------
+// inv(D) by blocked Gauss-Jordan on the FP64 MFMA.  Same pivoting and implicit
+// row exchanges as gauss_jordan above (partial pivoting, lowest index on ties,
+// rows never moved), so inv(D)[label[i], pof[c]] = G[i][c] holds on exit.
+// The na x na matrix (padded to NN = 16 NT) lives in registers as 16 x 16 MFMA
+// result tiles spread over the W waves (tile t = s W + wave in slot s; lane l,
+// element v holds row l/16 + 4v, column l%16 of its tile).  Per 16-column panel:
+//   1. wave 0 runs the 16 unblocked steps on the panel alone, in registers
+//      (two or three rows per lane, no barrier); the panel then holds Gp, whose column k is T e_{P[k]} for the product T of
+//      the panel's elementary steps and its pivot rows P;
+//   2. every other column becomes T a = Abar + Gp a(P, :), with Abar = a with
+//      the rows P zeroed: a rank-16 update done as four MFMAs per tile straight
+//      into the register tiles (A operand Gp, B operand the old pivot rows,
+//      both from LDS).
+// The result is written column by column of inv(D) through an LDS staging tile
+// so that the permutation (output column pof[c], row label^-1) is applied with
+// coalesced stores.
+// ---------------------------------------------------------------------------
+template <int NT, int W, int MW>
+__global__ void __launch_bounds__(64 * W, MW)
+dinv_mfma_kernel(int na, int e0, int n, const double *__restrict__ D, double *__restrict__ Dinv, int *__restrict__ info)
+{
+    constexpr int NN = 16 * NT, NTILE = NT * NT, TPW = (NTILE + W - 1) / W, RPL = (NN + 63) / 64;
+    constexpr int PS = 17;                // panel row stride: odd, so a column walk is bank-conflict free
+    __shared__ double pan[NN * PS];       // panel / staging of 16 columns, row-major: pan[i*PS + k]
+    __shared__ double Z[16 * NN];         // pivot rows of the panel: Z[k*NN + c]
+    __shared__ int label[NN], pof[NN], rowk[NN];
+    __shared__ int sing_s;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 15, lk = lane >> 4;
+    for (int t = blockIdx.x; t < n; t += gridDim.x) {
+        const long e = e0 + t;
+        const double *De = D + e * (long)na * na;
+        // D into the register tiles, one 16-column tile at a time through LDS so that
+        // the global reads run down D's contiguous columns
+        f64x4 a[TPW];
+        for (int q = 0; q < NT; q++) {
+            for (int idx = tid; idx < NN * 16; idx += 64 * W) {
+                const int i = idx % NN, cc = idx / NN, c = q * 16 + cc;
+                pan[i * PS + cc] = (i < na && c < na) ? De[i + (long)na * c] : 0.0;
+            }
+            __syncthreads();
+            int wv = __builtin_amdgcn_readfirstlane(wave);
+            asm volatile("" : "+s"(wv));
+#pragma unroll
+            for (int s = 0; s < TPW; s++) {
+                const int T = s * W + wv, r = T / NT, qq = T % NT;
+                if (T < NTILE && qq == q)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) a[s][v] = pan[(r * 16 + lk + 4 * v) * PS + lr];
+            }
+            __syncthreads();
+        }
+        for (int i = tid; i < NN; i += 64 * W) label[i] = i < na ? -1 : NN;
+        if (tid == 0) sing_s = 0;
+        __syncthreads();
+        for (int p = 0; p < NT; p++) {
+            // 1. the panel to LDS.  (wv: the wave index made opaque at each step, so
+            //    that per-tile addresses are formed where used, not held in registers
+            //    across the panel loop)
+            int wv = __builtin_amdgcn_readfirstlane(wave);
+            asm volatile("" : "+s"(wv));
+#pragma unroll
+            for (int s = 0; s < TPW; s++) {
+                const int T = s * W + wv, r = T / NT, q = T % NT;
+                if (T < NTILE && q == p)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) pan[(r * 16 + lk + 4 * v) * PS + lr] = a[s][v];
+            }
+            __syncthreads();
+            // 2. unblocked steps on the panel (wave 0) in registers: lane l holds rows
+            //    l, l+64, ...; the pivot row is broadcast by readlane, and the pivot is
+            //    found from an exact wave maximum of the |value| bit patterns (DPP) with
+            //    the lowest row among equal keys taken from a ballot
+            if (wave == 0) {
+                double x[RPL][16];
+                bool used[RPL];
+                int pk[RPL];
+#pragma unroll
+                for (int h = 0; h < RPL; h++) {
+                    const int i = lane + 64 * h;
+                    used[h] = i >= NN || label[i] >= 0;
+                    pk[h] = -1;
+#pragma unroll
+                    for (int k = 0; k < 16; k++) x[h][k] = i < NN ? pan[i * PS + k] : 0.0;
+                }
+                int sing = 0;
+#pragma unroll
+                for (int jj = 0; jj < 16; jj++) {
+                    // fully unrolled so that x[h][jj] is a register (no early exit)
+                    const int j = p * 16 + jj;
+                    if (j < na) {
+                        // key 0 for used rows, 1 + |value| bits otherwise (NaN ranks highest)
+                        unsigned long long key[RPL], m = 0;
+#pragma unroll
+                        for (int h = 0; h < RPL; h++) {
+                            key[h] = used[h] ? 0ull
+                                             : ((unsigned long long)__double_as_longlong(x[h][jj]) & 0x7fffffffffffffffull) + 1;
+                            m = key[h] > m ? key[h] : m;
+                        }
+                        m = wave_max_u64(m);
+                        int pr = NN;
+#pragma unroll
+                        for (int h = RPL - 1; h >= 0; h--) {
+                            const unsigned long long b = __ballot(key[h] == m);
+                            if (b) pr = 64 * h + __ffsll((long long)b) - 1;
+                        }
+                        const int owner = pr & 63, hp = pr >> 6;
+                        double prow[16];
+#pragma unroll
+                        for (int k = 0; k < 16; k++) {
+                            double v = x[0][k];
+#pragma unroll
+                            for (int h = 1; h < RPL; h++) v = hp == h ? x[h][k] : v;
+                            const int lo = __builtin_amdgcn_readlane(__double2loint(v), owner);
+                            const int hi = __builtin_amdgcn_readlane(__double2hiint(v), owner);
+                            prow[k] = __hiloint2double(hi, lo);
+                        }
+                        const double pv = prow[jj];
+                        if (pv == 0.0 && !sing) sing = j + 1;
+                        const double pinv = pv == 0.0 ? 0.0 : 1.0 / pv;
+#pragma unroll
+                        for (int h = 0; h < RPL; h++) {
+                            const int i = lane + 64 * h;
+                            const double f = x[h][jj];
+                            if (i == pr) {
+#pragma unroll
+                                for (int k = 0; k < 16; k++) x[h][k] = k == jj ? pinv : prow[k] * pinv;
+                                used[h] = true;
+                                pk[h] = jj;
+                            } else {
+#pragma unroll
+                                for (int k = 0; k < 16; k++) x[h][k] = k == jj ? -f * pinv : x[h][k] - f * prow[k] * pinv;
+                            }
+                        }
+                        if (lane == 0) pof[j] = pr;
+                    }
+                }
+#pragma unroll
+                for (int h = 0; h < RPL; h++) {
+                    const int i = lane + 64 * h;
+                    if (i < NN) {
+#pragma unroll
+                        for (int k = 0; k < 16; k++) pan[i * PS + k] = x[h][k];
+                        rowk[i] = pk[h];
+                        if (pk[h] >= 0) label[i] = p * 16 + pk[h];
+                    }
+                }
+                if (lane == 0 && sing && !sing_s) sing_s = sing;
+            }
+            __syncthreads();
+            // 3. the old pivot rows of the other columns
+            wv = __builtin_amdgcn_readfirstlane(wave);
+            asm volatile("" : "+s"(wv));
+#pragma unroll
+            for (int s = 0; s < TPW; s++) {
+                const int T = s * W + wv, r = T / NT, q = T % NT;
+                if (T < NTILE && q != p)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        const int k = rowk[r * 16 + lk + 4 * v];
+                        if (k >= 0) Z[k * NN + q * 16 + lr] = a[s][v];
+                    }
+            }
+            __syncthreads();
+            // 4. the rank-16 update (the panel columns take Gp)
+            wv = __builtin_amdgcn_readfirstlane(wave);
+            asm volatile("" : "+s"(wv));
+#pragma unroll
+            for (int s = 0; s < TPW; s++) {
+                const int T = s * W + wv, r = T / NT, q = T % NT;
+                if (T >= NTILE) continue;
+                if (q == p) {
+#pragma unroll
+                    for (int v = 0; v < 4; v++) a[s][v] = pan[(r * 16 + lk + 4 * v) * PS + lr];
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        if (rowk[r * 16 + lk + 4 * v] >= 0) a[s][v] = 0.0;
+#pragma unroll
+                    for (int ks = 0; ks < 4; ks++)
+                        a[s] = mfma64(pan[(r * 16 + lr) * PS + ks * 4 + lk], Z[(ks * 4 + lk) * NN + q * 16 + lr], a[s]);
+                }
+            }
+            __syncthreads();
+        }
+        // inv(D)(a, pof[c]) = G(pof[a], c), one 16-column tile of G at a time
+        double *Oe = Dinv + e * (long)na * na;
+        for (int q = 0; q < NT; q++) {
+            int wv = __builtin_amdgcn_readfirstlane(wave);
+            asm volatile("" : "+s"(wv));
+#pragma unroll
+            for (int s = 0; s < TPW; s++) {
+                const int T = s * W + wv, r = T / NT, qq = T % NT;
+                if (T < NTILE && qq == q)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) pan[(r * 16 + lk + 4 * v) * PS + lr] = a[s][v];
+            }
+            __syncthreads();
+            const int ncols = min(16, na - q * 16);
+            for (int idx = tid; idx < ncols * na; idx += 64 * W) {
+                const int cc = idx / na, ar = idx % na, c = q * 16 + cc;
+                Oe[ar + (long)na * pof[c]] = pan[pof[ar] * PS + cc];
+            }
+            __syncthreads();
+        }
+        if (tid == 0) info[e] = sing_s;
+    }
+}
+
 int condense_dinv_ld(int na) { return dinv_ld(na); }
 
 int condense_rows_per_thread(int na) { return na <= 32 ? 2 : (na + 15) / 16; }
@@ -361,15 +594,18 @@ int condense_rows_per_thread(int na) { return na <= 32 ? 2 : (na + 15) / 16; }
 
 hipError_t launch_dinv(int na, int e0, int n, const double *D, double *Dinv, int *info, hipStream_t s)
 {
-    const int R = condense_rows_per_thread(na);
-    const size_t lds = condense_lds_bytes(na, false, sizeof(double));
+    const int NT = (na + 15) / 16;
     const int grid = std::min(n, 32768);
-    switch (R) {
-#define CASE(RR)                                                                                                   \
-    case RR:                                                                                                       \
-        hipLaunchKernelGGL(dinv_kernel<RR>, dim3(grid), dim3(256), lds, s, na, e0, n, D, Dinv, info);             \
+    switch (NT) {
+// (tiles per side, waves, minimum waves per SIMD): two 8-wave workgroups per CU
+// where the register tiles fit in 128 VGPRs
+#define CASE(T, W, MW)                                                                                             \
+    case T:                                                                                                        \
+        hipLaunchKernelGGL((dinv_mfma_kernel<T, W, MW>), dim3(grid), dim3(64 * W), 0, s, na, e0, n, D, Dinv,      \
+                           info);                                                                                  \
         break;
-        HFEM_R_CASES(CASE)
+        CASE(1, 4, 2) CASE(2, 4, 2) CASE(3, 4, 2) CASE(4, 4, 2) CASE(5, 4, 2) CASE(6, 4, 2) CASE(7, 4, 2)
+        CASE(8, 4, 2) CASE(9, 8, 2) CASE(10, 8, 2) CASE(11, 8, 2)
 #undef CASE
     default:
         return hipErrorInvalidValue;
