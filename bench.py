@@ -202,7 +202,7 @@ def main():
     step_s = r["wall"] / args.steps
     # dominant kernel and its roofline (achieved = algorithmic flops / average launch time)
     if r["cond_ms"] >= r["asm_ms"]:
-        kname = "condense_mixed_kernel" if head == "mixed" else "dinv_kernel+dgemm_strided_batched"
+        kname = "condense_mixed_kernel" if head == "mixed" else "dinv_mfma_kernel+dgemm_strided_batched"
         kms, kflops = r["cond_ms"], f_cond
     else:
         kname, kms, kflops = "assemble_kernel+flux_kernel", r["asm_ms"], f_asm

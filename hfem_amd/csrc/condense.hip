@@ -402,32 +402,29 @@ dinv_mfma_kernel(int na, int e0, int n, const double *__restrict__ D, double *__
     constexpr int NN = 16 * NT, NTILE = NT * NT, TPW = (NTILE + W - 1) / W, RPL = (NN + 63) / 64;
     constexpr int PS = 17;                // panel row stride: odd, so a column walk is bank-conflict free
     __shared__ double pan[NN * PS];       // panel / staging of 16 columns, row-major: pan[i*PS + k]
-    __shared__ double Z[16 * NN];         // pivot rows of the panel: Z[k*NN + c]
+    __shared__ double Z[NN * PS];         // pivot rows of the panel, Z[k*NN + c]; output staging
     __shared__ int label[NN], pof[NN], rowk[NN];
     __shared__ int sing_s;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 15, lk = lane >> 4;
     for (int t = blockIdx.x; t < n; t += gridDim.x) {
         const long e = e0 + t;
         const double *De = D + e * (long)na * na;
-        // D into the register tiles, one 16-column tile at a time through LDS so that
-        // the global reads run down D's contiguous columns
+        // D into the register tiles: all loads of a wave issued together
         f64x4 a[TPW];
-        for (int q = 0; q < NT; q++) {
-            for (int idx = tid; idx < NN * 16; idx += 64 * W) {
-                const int i = idx % NN, cc = idx / NN, c = q * 16 + cc;
-                pan[i * PS + cc] = (i < na && c < na) ? De[i + (long)na * c] : 0.0;
-            }
-            __syncthreads();
+        {
             int wv = __builtin_amdgcn_readfirstlane(wave);
             asm volatile("" : "+s"(wv));
 #pragma unroll
             for (int s = 0; s < TPW; s++) {
-                const int T = s * W + wv, r = T / NT, qq = T % NT;
-                if (T < NTILE && qq == q)
+                const int T = s * W + wv, r = T / NT, q = T % NT;
 #pragma unroll
-                    for (int v = 0; v < 4; v++) a[s][v] = pan[(r * 16 + lk + 4 * v) * PS + lr];
+                for (int v = 0; v < 4; v++) {
+                    const int i = r * 16 + lk + 4 * v, c = q * 16 + lr;
+                    const bool ok = T < NTILE && i < na && c < na;
+                    const double d = De[ok ? i + (long)na * c : 0];
+                    a[s][v] = ok ? d : 0.0;
+                }
             }
-            __syncthreads();
         }
         for (int i = tid; i < NN; i += 64 * W) label[i] = i < na ? -1 : NN;
         if (tid == 0) sing_s = 0;
@@ -562,23 +559,27 @@ dinv_mfma_kernel(int na, int e0, int n, const double *__restrict__ D, double *__
             }
             __syncthreads();
         }
-        // inv(D)(a, pof[c]) = G(pof[a], c), one 16-column tile of G at a time
+        // inv(D)(a, pof[c]) = G(pof[a], c), two 16-column tiles of G at a time
+        // (staged in pan and Z)
         double *Oe = Dinv + e * (long)na * na;
-        for (int q = 0; q < NT; q++) {
+        for (int q0 = 0; q0 < NT; q0 += 2) {
             int wv = __builtin_amdgcn_readfirstlane(wave);
             asm volatile("" : "+s"(wv));
 #pragma unroll
             for (int s = 0; s < TPW; s++) {
-                const int T = s * W + wv, r = T / NT, qq = T % NT;
-                if (T < NTILE && qq == q)
+                const int T = s * W + wv, r = T / NT, q = T % NT;
+                if (T < NTILE && (q == q0 || q == q0 + 1)) {
+                    double *st = q == q0 ? pan : Z;
 #pragma unroll
-                    for (int v = 0; v < 4; v++) pan[(r * 16 + lk + 4 * v) * PS + lr] = a[s][v];
+                    for (int v = 0; v < 4; v++) st[(r * 16 + lk + 4 * v) * PS + lr] = a[s][v];
+                }
             }
             __syncthreads();
-            const int ncols = min(16, na - q * 16);
+            const int ncols = min(32, na - q0 * 16);
             for (int idx = tid; idx < ncols * na; idx += 64 * W) {
-                const int cc = idx / na, ar = idx % na, c = q * 16 + cc;
-                Oe[ar + (long)na * pof[c]] = pan[pof[ar] * PS + cc];
+                const int cc = idx / na, ar = idx % na, c = q0 * 16 + cc;
+                const double *st = cc < 16 ? pan : Z;
+                Oe[ar + (long)na * pof[c]] = st[pof[ar] * PS + (cc & 15)];
             }
             __syncthreads();
         }
