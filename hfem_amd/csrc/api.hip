@@ -72,7 +72,9 @@ extern "C" int hfem_create(hfem_ctx **out, const hfem_dims *hd, int device)
     c->grid_asm = (int)std::min<long>(ne, 512);
     c->grid_cond = (int)std::min<long>(ne, 1024);
     c->fstride = flux_layout(d).total;
-    c->chunk = (int)std::min<long>(ne, 4096);
+    // one chunk for meshes up to 32768 elements (the flux buffer holds about 160 KB per
+    // element for p=3 tets): no chunk boundaries to drain
+    c->chunk = (int)std::min<long>(ne, 32768);
     ALLOC(fluxbuf, (long)c->chunk * c->fstride);
     {
         int maxlds = 65536;

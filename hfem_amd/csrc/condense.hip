@@ -494,18 +494,22 @@ dinv_mfma_kernel(int na, int e0, int n, const double *__restrict__ D, double *__
                         const double pv = prow[jj];
                         if (pv == 0.0 && !sing) sing = j + 1;
                         const double pinv = pv == 0.0 ? 0.0 : 1.0 / pv;
+                        // scaled pivot row s_k = a(p, k) / a(p, j); every row takes one FMA per
+                        // column, then the pivot row (one lane) is overwritten
+                        double sk[16];
+#pragma unroll
+                        for (int k = 0; k < 16; k++) sk[k] = prow[k] * pinv;
 #pragma unroll
                         for (int h = 0; h < RPL; h++) {
                             const int i = lane + 64 * h;
                             const double f = x[h][jj];
+#pragma unroll
+                            for (int k = 0; k < 16; k++) x[h][k] = k == jj ? -f * pinv : fma(-f, sk[k], x[h][k]);
                             if (i == pr) {
 #pragma unroll
-                                for (int k = 0; k < 16; k++) x[h][k] = k == jj ? pinv : prow[k] * pinv;
+                                for (int k = 0; k < 16; k++) x[h][k] = k == jj ? pinv : sk[k];
                                 used[h] = true;
                                 pk[h] = jj;
-                            } else {
-#pragma unroll
-                                for (int k = 0; k < 16; k++) x[h][k] = k == jj ? -f * pinv : x[h][k] - f * prow[k] * pinv;
                             }
                         }
                         if (lane == 0) pof[j] = pr;
