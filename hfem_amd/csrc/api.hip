@@ -86,7 +86,7 @@ extern "C" int hfem_create(hfem_ctx **out, const hfem_dims *hd, int device)
             return hfem_fail(-1, "hfem_create: element family needs %zu bytes of LDS, device offers %d",
                         assemble_lds_bytes(d), maxlds);
         }
-        if (!assemble_supported(d.nd, d.npv, d.npf)) {
+        if (!assemble_supported(d)) {
             delete c;
             return hfem_fail(-1, "hfem_create: no assembly kernel compiled for nd=%d npv=%d npf=%d", d.nd, d.npv, d.npf);
         }
@@ -96,6 +96,7 @@ extern "C" int hfem_create(hfem_ctx **out, const hfem_dims *hd, int device)
         }
     }
     ALLOC(dinv, ne * d.na * d.na);
+    if (assemble_spill_doubles(d) > 0) ALLOC(spill, (long)c->grid_asm * assemble_spill_doubles(d));
     ALLOC(rwork, (long)c->grid_cond * d.na * c->ldx);
 #undef ALLOC
     if (e == hipSuccess) e = hipStreamCreate(&c->stream);
@@ -118,7 +119,7 @@ extern "C" int hfem_destroy(hfem_ctx *c)
     if (c->stream) hipStreamSynchronize(c->stream);
     void *ptrs[] = {c->shapvt, c->shapvg, c->shapft, c->shapfg, c->perm, c->invperm, c->dgnodes, c->elcon, c->bf,
                     c->rowstart, c->cols, c->einv, c->f2e, c->blkptr, c->blklist, c->UDG, c->UH, c->SH, c->D, c->Fx,
-                    c->X, c->K, c->Hx, c->Ru, c->Rh, c->ru2, c->info, c->dinv, c->rwork, c->fluxbuf, c->Hg, c->Rg,
+                    c->X, c->K, c->Hx, c->Ru, c->Rh, c->ru2, c->info, c->dinv, c->spill, c->rwork, c->fluxbuf, c->Hg, c->Rg,
                     c->r, c->partial, c->xv, c->yv, c->sv.d_o2u, c->sv.d_u2o, c->sv.d_lrows, c->sv.d_urows,
                     c->sv.Mg, c->sv.Dinv, c->sv.Cw, c->sv.d_info, c->sv.x, c->sv.V, c->sv.b, c->sv.t, c->sv.dots,
                     c->sv.coef, c->sv.UDGi, c->qn_dinv, c->qn_K};
@@ -301,7 +302,7 @@ extern "C" int hfem_assemble(hfem_ctx *c, int e0, int n)
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
     HIPCHK(launch_assemble(c->d, m, c->ph, c->dgnodes, c->UDG, c->UH, c->SH, c->elcon, c->bf, e0, n, grid, c->ldx,
                            c->fluxbuf, c->fstride, c->chunk, c->Ru, c->D, c->Fx, c->K, c->Hx, c->Rh, c->ru2,
-                           c->stream));
+                           c->spill, c->stream));
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
     return 0;
 }

@@ -46,17 +46,28 @@ struct AsmLayout {
     int xa, xha, cf, ef;
     int fuX2, fj2;          // second staging buffer of P8
     int total, nints, overlay, dbuf;
+    int spill, stotal;      // spill: svt, svg, Qm, C, E in a per-workgroup global region of stotal doubles
 };
 
-__host__ __device__ inline AsmLayout asm_layout(const Dims &d)
+// Offsets for one placement: spill = 0 puts everything in LDS, spill = 1 moves the
+// large read-mostly tables (shapvt with its reused derivative region, shapvg, Q,
+// C, E) to a per-workgroup region in global memory, for families whose element
+// data exceed the LDS (p=4 tets).
+__host__ __device__ inline AsmLayout asm_layout(const Dims &d, int spill)
 {
     const int npv = d.npv, ngv = d.ngv, nd = d.nd, nc = d.nc, ncu = d.ncu, nch = d.nch, np = d.np;
     const int npf = d.npf, ngf = d.ngf, ndf = d.ndf, nd1 = d.nd + 1;
     AsmLayout L;
-    int o = 0;
+    int o = 0, s = 0;
+    auto big = [&](int n) {
+        int r;
+        if (spill) { r = s; s += n; } else { r = o; o += n; }
+        return r;
+    };
     const int deriv = (npv > nc ? npv : nc) * ngv * nd;
-    L.svt = o; o += npv * ngv + deriv;
-    L.svg = o; o += npv * ngv * nd1;
+    L.spill = spill;
+    L.svt = big(npv * ngv + deriv);
+    L.svg = big(npv * ngv * nd1);
     L.sft = o; o += npf * ngf * nd;
     L.sfg = o; o += npf * ngf * nd;
     L.pn = o; o += npv * nd;
@@ -68,9 +79,9 @@ __host__ __device__ inline AsmLayout asm_layout(const Dims &d)
     L.jacf = o; o += np;
     L.nl = o; o += np * nd;
     L.M = o; o += npv * npv;
-    L.Qm = o; o += (ngv + np) * npv;
-    L.C = o; o += nd * npv * npv;
-    L.E = o; o += nd * npv * ndf;
+    L.Qm = big((ngv + np) * npv);
+    L.C = big(nd * npv * npv);
+    L.E = big(nd * npv * ndf);
     L.udgg = o; o += ngv * nc;
     // fhu, fhuh of one row block (P8); the Ru/Rh tables (P9)
     const int fjn = np * (nc + nch), tn = ngv * ncu * nd1 + 2 * np * nch;
@@ -81,7 +92,7 @@ __host__ __device__ inline AsmLayout asm_layout(const Dims &d)
     const int kh = ngf * nch * (nc + nch) + (npv + ndf) * ngf * nd;
     int base = o;
     L.overlay = 0;
-    if ((long)(o + kh) * 8 + (long)L.nints * 4 > ASM_LDS_MAX && kh <= deriv + npv * ngv * nd1) {
+    if (!spill && (long)(o + kh) * 8 + (long)L.nints * 4 > ASM_LDS_MAX && kh <= deriv + npv * ngv * nd1) {
         L.overlay = 1;
         base = L.svt + npv * ngv;
     } else {
@@ -100,7 +111,16 @@ __host__ __device__ inline AsmLayout asm_layout(const Dims &d)
         o += sb;
     }
     L.total = o;
+    L.stotal = s;
     return L;
+}
+
+// The placement used: everything in LDS when it fits, else the spill placement.
+__host__ __device__ inline AsmLayout asm_layout(const Dims &d)
+{
+    const AsmLayout L = asm_layout(d, 0);
+    if ((long)L.total * 8 + (long)L.nints * 4 <= ASM_LDS_MAX) return L;
+    return asm_layout(d, 1);
 }
 
 // Volume geometry at one point: J(x,r) = sum_k dshape(k, r) pn[x*npv + k] with
@@ -169,19 +189,19 @@ __device__ __forceinline__ void gauss_jordan_wave(double *M, int lane)
         for (int i = 0; i < NPV; i++) M[i + NPV * lane] = m[i];
 }
 
-template <int ND, int NPV, int NPF>
+template <int ND, int NPV, int NPF, bool SPILL>
 __global__ void __launch_bounds__(ASM_THREADS)
 assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, const double *__restrict__ UDG,
                 const double *__restrict__ UH, const double *__restrict__ SH, const int *__restrict__ elcon,
                 const int *__restrict__ bf, int e0, int n, int ldx, const double *__restrict__ fluxbuf, long fstride,
                 double *__restrict__ Ru, double *__restrict__ Dm, double *__restrict__ Fx, double *__restrict__ Km,
-                double *__restrict__ Hx, double *__restrict__ Rh, double *__restrict__ ru2)
+                double *__restrict__ Hx, double *__restrict__ Rh, double *__restrict__ ru2, double *__restrict__ spillbuf)
 {
     extern __shared__ double lds[];
     const int tid = threadIdx.x, nth = blockDim.x;
     const int lane = tid & 63, wave = tid >> 6, nwave = nth >> 6, lr = lane & 15, lk = lane >> 4;
     const FluxLayout FL = flux_layout(d);
-    const AsmLayout LY = asm_layout(d);
+    const AsmLayout LY = asm_layout(d, SPILL);
     constexpr int NCU = ND + 2, NC = NCU * (ND + 1), ND1 = ND + 1;
     // faces per element from the face size (tets: triangular faces, hexes: square faces)
     constexpr int NFE = ND == 2 ? 3 : ((NPF == 4 || NPF == 9 || NPF == 16) ? 6 : 4);
@@ -192,11 +212,13 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
     const int npv = NPV, ngv = d.ngv, npf = NPF, ngf = d.ngf, nfe = NFE, ndf = NDF;
     const int ncu = NCU, nch = NCU, nc = NC, na = d.na, nb = d.nb, nd1 = ND1, np = d.np;
 
-    double *svt = lds + LY.svt, *svg = lds + LY.svg, *sft = lds + LY.sft, *sfg = lds + LY.sfg;
+    // the large tables: LDS, or this workgroup's global region in the spill placement
+    double *big = SPILL ? spillbuf + (long)blockIdx.x * LY.stotal : lds;
+    double *svt = big + LY.svt, *svg = big + LY.svg, *sft = lds + LY.sft, *sfg = lds + LY.sfg;
     double *pn = lds + LY.pn, *udg = lds + LY.udg, *uh = lds + LY.uh, *sh = lds + LY.sh;
     double *jac = lds + LY.jac, *Xx = lds + LY.Xx, *jacf = lds + LY.jacf, *nl = lds + LY.nl;
-    double *M = lds + LY.M, *Qm = lds + LY.Qm;
-    double *C = lds + LY.C, *E = lds + LY.E, *udgg = lds + LY.udgg, *fj = lds + LY.fj;
+    double *M = lds + LY.M, *Qm = big + LY.Qm;
+    double *C = big + LY.C, *E = big + LY.E, *udgg = lds + LY.udgg, *fj = lds + LY.fj;
     double *red = lds + LY.red;
     double *xa = lds + LY.xa, *cf = lds + LY.cf, *ef = lds + LY.ef;
     double *G = svt + npv * ngv, *fuX = G;       // the shapvt-derivative region, reused
@@ -1135,39 +1157,69 @@ size_t assemble_lds_bytes(const Dims &d)
     X(2, 3, 2) X(2, 6, 3) X(2, 10, 4) X(2, 15, 5) X(2, 21, 6)                       \
     X(3, 4, 3) X(3, 10, 6) X(3, 20, 10) X(3, 35, 15) X(3, 8, 4) X(3, 27, 9)
 
-bool assemble_supported(int nd, int npv, int npf)
+// families whose element data exceed the LDS: a spill-placement instantiation as well
+#define HFEM_SPILL_FAMILIES(X) X(3, 35, 15)
+
+// true when a kernel is compiled for the family in the placement its sizes need
+bool assemble_supported(const Dims &d)
 {
-#define CHECK(a, b, c) if (nd == a && npv == b && npf == c) return true;
+    const bool spill = asm_layout(d).spill != 0;
+#define CHECK(a, b, c) if (!spill && d.nd == a && d.npv == b && d.npf == c) return true;
     HFEM_FAMILIES(CHECK)
 #undef CHECK
+#define CHECK(a, b, c) if (spill && d.nd == a && d.npv == b && d.npf == c) return true;
+    HFEM_SPILL_FAMILIES(CHECK)
+#undef CHECK
     return false;
+}
+
+// doubles per workgroup of the spill placement's global region (0 if none)
+long assemble_spill_doubles(const Dims &d)
+{
+    const AsmLayout L = asm_layout(d);
+    return L.spill ? L.stotal : 0;
 }
 
 hipError_t launch_assemble(const Dims &d, const Master &m, const Phys &ph, const double *dgnodes, const double *UDG,
                            const double *UH, const double *SH, const int *elcon, const int *bf, int e0, int n,
                            int grid, int ldx, double *fluxbuf, long fstride, int chunk,
                            double *Ru, double *D, double *Fx, double *K, double *Hx, double *Rh, double *ru2,
-                           hipStream_t stream)
+                           double *spill, hipStream_t stream)
 {
     const size_t lds = assemble_lds_bytes(d), flds = flux_lds_bytes(d);
+    const bool sp = asm_layout(d).spill != 0;
+    if (sp && !spill) return hipErrorInvalidValue;
     for (int c0 = e0; c0 < e0 + n; c0 += chunk) {
         const int cn = std::min(chunk, e0 + n - c0);
         const int fgrid = std::min(cn, 2048), agrid = std::min(cn, grid);
         bool done = false;
-#define LAUNCH(a, b, c)                                                                                             \
+#define FLUX(a, b, c)                                                                                               \
         if (!done && d.nd == a && d.npv == b && d.npf == c) {                                                       \
             hipFuncSetAttribute((const void *)flux_kernel<a, b, c>, hipFuncAttributeMaxDynamicSharedMemorySize,    \
                                 (int)flds);                                                                         \
             hipLaunchKernelGGL((flux_kernel<a, b, c>), dim3(fgrid), dim3(256), flds, stream, d, m, ph, dgnodes,     \
                                UDG, UH, elcon, bf, c0, cn, fluxbuf, fstride);                                       \
-            hipFuncSetAttribute((const void *)assemble_kernel<a, b, c>, hipFuncAttributeMaxDynamicSharedMemorySize,\
-                                (int)lds);                                                                          \
-            hipLaunchKernelGGL((assemble_kernel<a, b, c>), dim3(agrid), dim3(ASM_THREADS), lds, stream, d, m, ph,   \
-                               dgnodes, UDG, UH, SH, elcon, bf, c0, cn, ldx, fluxbuf, fstride, Ru, \
-                               D, Fx, K, Hx, Rh, ru2);                                                              \
             done = true;                                                                                            \
         }
-        HFEM_FAMILIES(LAUNCH)
+        HFEM_FAMILIES(FLUX)
+#undef FLUX
+        if (!done) return hipErrorInvalidValue;
+        done = false;
+#define LAUNCH(a, b, c, S)                                                                                          \
+        if (!done && sp == S && d.nd == a && d.npv == b && d.npf == c) {                                            \
+            hipFuncSetAttribute((const void *)assemble_kernel<a, b, c, S>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                (int)lds);                                                                          \
+            hipLaunchKernelGGL((assemble_kernel<a, b, c, S>), dim3(agrid), dim3(ASM_THREADS), lds, stream, d, m, ph,\
+                               dgnodes, UDG, UH, SH, elcon, bf, c0, cn, ldx, fluxbuf, fstride, Ru, D, Fx, K, Hx,   \
+                               Rh, ru2, spill);                                                                     \
+            done = true;                                                                                            \
+        }
+#define LAUNCH_LDS(a, b, c) LAUNCH(a, b, c, false)
+#define LAUNCH_SPILL(a, b, c) LAUNCH(a, b, c, true)
+        HFEM_FAMILIES(LAUNCH_LDS)
+        HFEM_SPILL_FAMILIES(LAUNCH_SPILL)
+#undef LAUNCH_SPILL
+#undef LAUNCH_LDS
 #undef LAUNCH
         if (!done) return hipErrorInvalidValue;
         hipError_t err = hipGetLastError();

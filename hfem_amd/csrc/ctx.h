@@ -13,7 +13,8 @@
 namespace hfem {
 hipError_t launch_assemble(const Dims &, const Master &, const Phys &, const double *, const double *, const double *,
                            const double *, const int *, const int *, int, int, int, int, double *, long,
-                           int, double *, double *, double *, double *, double *, double *, double *, hipStream_t);
+                           int, double *, double *, double *, double *, double *, double *, double *, double *,
+                           hipStream_t);
 size_t flux_lds_bytes(const Dims &d);
 hipError_t launch_dinv(int na, int e0, int n, const double *D, double *Dinv, int *info, hipStream_t s);
 hipError_t launch_condense_mixed(int na, int nb, int ldx, int e0, int n, const double *D, const double *Fx, double *X,
@@ -24,7 +25,8 @@ hipError_t launch_schur_vector(int na, int nb, int ldx, int ne, int fp32, const 
                                const double *Ru, const double *Rh, double *X, double *Hx, hipStream_t s);
 hipError_t launch_to_f32(long n, const double *x, float *y, hipStream_t s);
 size_t assemble_lds_bytes(const Dims &d);
-bool assemble_supported(int nd, int npv, int npf);
+bool assemble_supported(const Dims &d);
+long assemble_spill_doubles(const Dims &d);
 hipError_t launch_bsr_gather(int, int, int, int, int, long, const int *, const int *, const int *, const double *,
                              double *, hipStream_t);
 hipError_t launch_face_gather(int, int, int, int, int, long, const int *, const int *, const double *, double *,
@@ -126,6 +128,7 @@ struct hfem_ctx {
     int ldx = 0;
     // work
     double *dinv = nullptr, *rwork = nullptr, *fluxbuf = nullptr;   // dinv: ne x na x na
+    double *spill = nullptr;   // per-workgroup element tables of families beyond the LDS
     rocblas_handle blas = nullptr;
     long fstride = 0;
     int chunk = 0;

@@ -7,7 +7,7 @@ inputs and the element blocks before (Ru, D, F, K, H, Rh) and after
 condensation (DinvRu, DinvF, AH, FH) for both schurImplementation 0 and 1.
 Run from the repository root in the build container:
 
-    make -f oracle/ref.mk && python tests/golden/make_golden.py
+    make -f oracle/ref.mk && python tests/golden/make_golden.py [case ...]
 """
 import os
 import sys
@@ -31,12 +31,15 @@ CASES = {
     "tet1_p2": ("tet", 1, 2, [0, 4], True, True),
     "tet1_p3": ("tet", 1, 3, [0], True, True),
     "hex3_p2": ("hex", 3, 2, [13], False, False),
+    "tet1_p4": ("tet", 1, 4, [3], False, True),
 }
 
 
-def main():
+def main(names=None):
     out_dir = os.path.dirname(os.path.abspath(__file__))
     for name, (kind, ncell, porder, keep, impl1, dinvf) in CASES.items():
+        if names and name not in names:
+            continue
         pr = make_problem(kind, ncell, porder, seed=1)
         ref = O.Reference(pr)
         res = {}
@@ -55,4 +58,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -32,7 +32,7 @@ def say(*a):
     print(*a, flush=True)
 
 
-CASES = [("tri", 2, 2), ("tri", 2, 4), ("tet", 1, 2), ("tet", 1, 3), ("hex", 3, 2), ("tet", 2, 3)]
+CASES = [("tri", 2, 2), ("tri", 2, 4), ("tet", 1, 2), ("tet", 1, 3), ("hex", 3, 2), ("tet", 2, 3), ("tet", 1, 4)]
 
 
 @pytest.mark.parametrize("kind,nc,p", CASES)
@@ -74,7 +74,7 @@ def test_assembly_and_condensation_vs_oracle(oracle_mod, kind, nc, p):
 
 
 @pytest.mark.parametrize("name,kind,nc,p", [("tri2_p2", "tri", 2, 2), ("tet1_p3", "tet", 1, 3),
-                                            ("hex3_p2", "hex", 3, 2)])
+                                            ("hex3_p2", "hex", 3, 2), ("tet1_p4", "tet", 1, 4)])
 def test_against_reference_golden(name, kind, nc, p):
     """Directly against the reference element path's outputs (schurImplementation 0)."""
     from hfem_amd.lib import HDGContext, FP64
