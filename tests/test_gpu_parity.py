@@ -4,6 +4,11 @@ Tolerances (relative Frobenius norm, max over elements):
 * FP64 path: element matrices and condensed blocks <= 1e-13; the element
   residual Ru and inv(D) Ru <= 1e-10 (cancellation between O(1) volume and face
   terms; the reference's own impl 0 and impl 1 differ by ~1e-12 on it).
+  Ru is the difference of volume and face integrals of the size of Rh; at p=1
+  it nearly cancels (|Ru| ~ 1e-5 |Rh| on the seeded tets), so a rounding of
+  1e-16 |Rh| is 1e-11 of |Ru|.  Residual errors are therefore measured against
+  max(|Ru_e|, 1e-4 |Rh_e|) per element, and inv(D) Ru against that floor
+  carried through inv(D)'s gain on the element (|inv(D)Ru_e| / |Ru_e|).
 * Mixed path (FP32 factorisation + FP64 refinement): <= 1e-10 on AH, FH,
   inv(D)F, inv(D)Ru.
 """
@@ -28,11 +33,24 @@ def relerr(a, b):
     return float((np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)).max())
 
 
+def res_floor(Ru, Rh):
+    """Per-element floor 1e-4 |Rh_e| for the residual's relative error."""
+    n = len(Ru)
+    return 1e-4 * np.linalg.norm(np.asarray(Rh).reshape(n, -1), axis=1)
+
+
+def relerr_res(a, b, floor):
+    a = np.asarray(a).reshape(len(b), -1)
+    b = np.asarray(b).reshape(len(b), -1)
+    return float((np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), floor)).max())
+
+
 def say(*a):
     print(*a, flush=True)
 
 
-CASES = [("tri", 2, 2), ("tri", 2, 4), ("tet", 1, 2), ("tet", 1, 3), ("hex", 3, 2), ("tet", 2, 3), ("tet", 1, 4)]
+CASES = [("tri", 2, 2), ("tri", 2, 4), ("tet", 1, 2), ("tet", 1, 3), ("hex", 3, 2), ("tet", 2, 3), ("tet", 1, 4),
+         ("tri", 2, 1), ("tri", 2, 3), ("tet", 1, 1), ("hex", 3, 1)]
 
 
 @pytest.mark.parametrize("kind,nc,p", CASES)
@@ -50,15 +68,21 @@ def test_assembly_and_condensation_vs_oracle(oracle_mod, kind, nc, p):
         e = relerr(g[k], ref[k])
         say(f"  assemble {k}: {e:.2e}")
         assert e < RTOL_MAT, k
-    assert relerr(g["Ru"], ref["Ru"]) < RTOL_RES
+    fl_ru = res_floor(ref["Ru"], ref["Rh"])
+    e = relerr_res(g["Ru"], ref["Ru"], fl_ru)
+    say(f"  assemble Ru: {e:.2e} (relative {relerr(g['Ru'], ref['Ru']):.2e})")
+    assert e < RTOL_RES, "Ru"
     b = {k: v.copy() for k, v in ref.items()}
     O.schur(0, d["npv"], d["ncu"], b)
+    nr = np.linalg.norm(ref["Ru"].reshape(pr.ne, -1), axis=1)
+    fl_y = fl_ru * np.linalg.norm(b["Ru"].reshape(pr.ne, -1), axis=1) / np.maximum(nr, 1e-300)
+    floors = {"DinvRu": fl_y, "FH": 0.0}
     ctx.schur_element_matrix_vector(precision=FP64)
     c = ctx.read_condensed()
     assert np.all(c["info"] == 0)
     for name, got, want, tol in [("DinvF", c["DinvF"], b["F"], RTOL_MAT), ("AH", c["AH"], b["H"], RTOL_MAT),
                                  ("DinvRu", c["DinvRu"], b["Ru"], RTOL_RES), ("FH", c["FH"], b["Rh"], RTOL_RES)]:
-        e = relerr(got, want)
+        e = relerr_res(got, want, floors[name]) if name in floors else relerr(got, want)
         say(f"  fp64 condense {name}: {e:.2e}")
         assert e < tol, name
     # mixed precision on freshly assembled blocks
@@ -67,7 +91,7 @@ def test_assembly_and_condensation_vs_oracle(oracle_mod, kind, nc, p):
     c = ctx.read_condensed()
     for name, got, want in [("DinvF", c["DinvF"], b["F"]), ("AH", c["AH"], b["H"]), ("DinvRu", c["DinvRu"], b["Ru"]),
                             ("FH", c["FH"], b["Rh"])]:
-        e = relerr(got, want)
+        e = relerr_res(got, want, floors[name]) if name in floors else relerr(got, want)
         say(f"  mixed condense {name}: {e:.2e}")
         assert e < RTOL_MIXED, name
     ctx.close()
