@@ -32,6 +32,7 @@ CASES = {
     "tet1_p3": ("tet", 1, 3, [0], True, True),
     "hex3_p2": ("hex", 3, 2, [13], False, False),
     "tet1_p4": ("tet", 1, 4, [3], False, True),
+    "tri2_p5": ("tri", 2, 5, [0, 5], False, True),
 }
 
 

@@ -50,7 +50,7 @@ def say(*a):
 
 
 CASES = [("tri", 2, 2), ("tri", 2, 4), ("tet", 1, 2), ("tet", 1, 3), ("hex", 3, 2), ("tet", 2, 3), ("tet", 1, 4),
-         ("tri", 2, 1), ("tri", 2, 3), ("tet", 1, 1), ("hex", 3, 1)]
+         ("tri", 2, 1), ("tri", 2, 3), ("tet", 1, 1), ("hex", 3, 1), ("tri", 2, 5)]
 
 
 @pytest.mark.parametrize("kind,nc,p", CASES)
@@ -98,7 +98,8 @@ def test_assembly_and_condensation_vs_oracle(oracle_mod, kind, nc, p):
 
 
 @pytest.mark.parametrize("name,kind,nc,p", [("tri2_p2", "tri", 2, 2), ("tet1_p3", "tet", 1, 3),
-                                            ("hex3_p2", "hex", 3, 2), ("tet1_p4", "tet", 1, 4)])
+                                            ("hex3_p2", "hex", 3, 2), ("tet1_p4", "tet", 1, 4),
+                                            ("tri2_p5", "tri", 2, 5)])
 def test_against_reference_golden(name, kind, nc, p):
     """Directly against the reference element path's outputs (schurImplementation 0)."""
     from hfem_amd.lib import HDGContext, FP64

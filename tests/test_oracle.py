@@ -11,7 +11,8 @@ from hfem_amd.problem import make_problem, freestream_state
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 CASES = {"tri2_p2": ("tri", 2, 2), "tri2_p4": ("tri", 2, 4), "tet1_p2": ("tet", 1, 2),
-         "tet1_p3": ("tet", 1, 3), "hex3_p2": ("hex", 3, 2), "tet1_p4": ("tet", 1, 4)}
+         "tet1_p3": ("tet", 1, 3), "hex3_p2": ("hex", 3, 2), "tet1_p4": ("tet", 1, 4),
+         "tri2_p5": ("tri", 2, 5)}
 
 # Matrices and condensed trace blocks agree to a few ulps of their norm; the
 # element residual Ru (and inv(D) Ru) is a small difference of O(1) volume and
