@@ -168,6 +168,9 @@ def main():
     ctx = HDGContext(prob)
     f_asm = ASSEMBLY_FLOPS[(3, 0, 3)]
     f_cond = condense_flops(na, nb)
+    # rocBLAS solutions of the FP64 condensation GEMMs, picked on a sample before timing
+    gemm_sols, gemm_ms = ctx.tune_condensation()
+    print(f"[bench] condensation GEMM solutions {gemm_sols}: ms default/picked {gemm_ms}", file=sys.stderr, flush=True)
     res = {}
     precs = {"fp64": FP64, "mixed": MIXED}
     todo = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
@@ -228,6 +231,7 @@ def main():
         "mixed_speedup_vs_fp64": (res["mixed"]["eps"] / res["fp64"]["eps"]) if len(res) == 2 else None,
         "mixed_max_rel_err_vs_fp64": errs or None,
         "kernel_ms": {k: {"assemble": v["asm_ms"], "condense": v["cond_ms"]} for k, v in res.items()},
+        "condense_gemm_tuning": {"solutions": gemm_sols, "sample_ms_default_picked": gemm_ms},
         "roofline": {"bound": "mfma", "kernel": kname, "achieved": achieved / 1e12, "peak": peak / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / peak, "traffic": hbm_traffic(kname),
                      "flops_per_element": kflops, "elements_per_launch": ne,

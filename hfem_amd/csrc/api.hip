@@ -307,6 +307,128 @@ extern "C" int hfem_assemble(hfem_ctx *c, int e0, int n)
     return 0;
 }
 
+// The two batched GEMMs of the FP64 condensation, over n elements:
+//   which 0: X^T = [F Ru]^T inv(D)^T  (X row-major = column-major X^T, ld ldx)
+//   which 1: [AH FH] = [H Rh] - K X
+struct CondGemm {
+    rocblas_operation ta, tb;
+    int m, n, k;
+    const double *alpha, *A, *B, *beta;
+    double *C;
+    int lda, ldb, ldc;
+    long sa, sb, sc;
+    int batch;
+};
+
+static const double kOne = 1.0, kZero = 0.0, kMinusOne = -1.0;
+
+static CondGemm cond_gemm(const hfem_ctx *c, int which, const double *Fx, const double *dinv, const double *K,
+                          double *X, double *Hx, int n)
+{
+    const Dims &d = c->d;
+    const long sx = (long)d.na * c->ldx, sd = (long)d.na * d.na, sk = (long)d.nb * d.na,
+               sh = (long)d.nb * (d.nb + 1);
+    if (which == 0)
+        return CondGemm{rocblas_operation_none, rocblas_operation_transpose, d.nb + 1, d.na, d.na, &kOne, Fx, dinv,
+                        &kZero, X, c->ldx, d.na, c->ldx, sx, sd, sx, n};
+    return CondGemm{rocblas_operation_none, rocblas_operation_transpose, d.nb, d.nb + 1, d.na, &kMinusOne, K, X,
+                    &kOne, Hx, d.nb, c->ldx, d.nb, sk, sx, sh, n};
+}
+
+// sol 0: rocBLAS's own choice; otherwise a solution index from the list for this shape
+static rocblas_status cond_gemm_run(rocblas_handle h, const CondGemm &g, int sol)
+{
+    const rocblas_datatype t = rocblas_datatype_f64_r;
+    return rocblas_gemm_strided_batched_ex(h, g.ta, g.tb, g.m, g.n, g.k, g.alpha, g.A, t, g.lda, g.sa, g.B, t, g.ldb,
+                                           g.sb, g.beta, g.C, t, g.ldc, g.sc, g.C, t, g.ldc, g.sc, g.batch, t,
+                                           sol ? rocblas_gemm_algo_solution_index : rocblas_gemm_algo_standard, sol,
+                                           0);
+}
+
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wdeprecated-declarations"
+static rocblas_status cond_gemm_solutions(rocblas_handle h, const CondGemm &g, std::vector<rocblas_int> &out)
+{
+    const rocblas_datatype t = rocblas_datatype_f64_r;
+    rocblas_int n = 0;
+    rocblas_status st = rocblas_gemm_strided_batched_ex_get_solutions(
+        h, g.ta, g.tb, g.m, g.n, g.k, g.alpha, g.A, t, g.lda, g.sa, g.B, t, g.ldb, g.sb, g.beta, g.C, t, g.ldc, g.sc,
+        g.C, t, g.ldc, g.sc, g.batch, t, rocblas_gemm_algo_solution_index, 0, nullptr, &n);
+    if (st != rocblas_status_success) return st;
+    out.resize(n);
+    if (n == 0) return st;
+    return rocblas_gemm_strided_batched_ex_get_solutions(
+        h, g.ta, g.tb, g.m, g.n, g.k, g.alpha, g.A, t, g.lda, g.sa, g.B, t, g.ldb, g.sb, g.beta, g.C, t, g.ldc, g.sc,
+        g.C, t, g.ldc, g.sc, g.batch, t, rocblas_gemm_algo_solution_index, 0, out.data(), &n);
+}
+#pragma clang diagnostic pop
+
+// Picks, for each of the two condensation GEMMs, the fastest rocBLAS solution on
+// a sample of nsample elements (default min(ne, 32768): the winners change with
+// the batch size, so the default times the batch the condensation runs).  Inputs are the
+// context's Fx, inv(D) and K buffers (their values do not matter for timing);
+// outputs go to temporary buffers, so no element block of the context changes.
+// A solution is kept only if it beats rocBLAS's own choice by more than 2 %.
+extern "C" int hfem_tune_condensation(hfem_ctx *c, int nsample, int *solutions, float *ms)
+{
+    if (!c) return hfem_fail(-1, "hfem_tune_condensation: null context");
+    HIPCHK(hipSetDevice(c->device));
+    const Dims &d = c->d;
+    const int ne = (int)c->hd.ne;
+    if (nsample < 0) return hfem_fail(-1, "hfem_tune_condensation: nsample=%d < 0", nsample);
+    if (ne == 0) return 0;
+    const int ns = nsample > 0 ? std::min(nsample, ne) : std::min(ne, 32768);
+    double *Xt = nullptr, *Ht = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    int rc = 0;
+    if (hfem_dalloc(&Xt, (size_t)ns * d.na * c->ldx) != hipSuccess ||
+        hfem_dalloc(&Ht, (size_t)ns * d.nb * (d.nb + 1)) != hipSuccess || hipEventCreate(&t0) != hipSuccess ||
+        hipEventCreate(&t1) != hipSuccess)
+        rc = hfem_fail(-2, "hfem_tune_condensation: out of device memory");
+    if (!rc && (hipMemsetAsync(Xt, 0, (size_t)ns * d.na * c->ldx * 8, c->stream) != hipSuccess ||
+                hipMemsetAsync(Ht, 0, (size_t)ns * d.nb * (d.nb + 1) * 8, c->stream) != hipSuccess))
+        rc = hfem_fail(-2, "hfem_tune_condensation: hipMemsetAsync failed");
+    for (int which = 0; which < 2 && !rc; which++) {
+        const CondGemm g = cond_gemm(c, which, c->Fx, c->dinv, c->K, Xt, Ht, ns);
+        // best of two timed runs after one untimed run; -1 if the solution fails
+        auto time_it = [&](int sol) -> float {
+            if (cond_gemm_run(c->blas, g, sol) != rocblas_status_success) return -1.f;
+            float best = 1e30f;
+            for (int r = 0; r < 2; r++) {
+                float t = 0;
+                if (hipEventRecord(t0, c->stream) != hipSuccess || cond_gemm_run(c->blas, g, sol) != rocblas_status_success ||
+                    hipEventRecord(t1, c->stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
+                    hipEventElapsedTime(&t, t0, t1) != hipSuccess)
+                    return -1.f;
+                best = std::min(best, t);
+            }
+            return best;
+        };
+        const float tdef = time_it(0);
+        if (tdef < 0) { rc = hfem_fail(-2, "hfem_tune_condensation: rocBLAS GEMM %d failed", which); break; }
+        std::vector<rocblas_int> sols;
+        if (cond_gemm_solutions(c->blas, g, sols) != rocblas_status_success) sols.clear();
+        int bsol = 0;
+        float bt = tdef;
+        for (rocblas_int s : sols) {
+            if (s == 0) continue;
+            const float t = time_it(s);
+            if (t >= 0 && t < bt) { bt = t; bsol = s; }
+        }
+        if (bt > 0.98f * tdef) { bsol = 0; bt = tdef; }
+        c->gemm_sol[which] = bsol;
+        if (solutions) solutions[which] = bsol;
+        if (ms) { ms[2 * which] = tdef; ms[2 * which + 1] = bt; }
+    }
+    (void)hipStreamSynchronize(c->stream);
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    if (Xt) (void)hipFree(Xt);
+    if (Ht) (void)hipFree(Ht);
+    (void)hipGetLastError();
+    return rc;
+}
+
 extern "C" int hfem_condense(hfem_ctx *c, int e0, int n, int precision)
 {
     int rc = check_range(c, e0, n, "hfem_condense");
@@ -319,21 +441,20 @@ extern "C" int hfem_condense(hfem_ctx *c, int e0, int n, int precision)
     HIPCHK(hipEventRecord(c->ev[2], c->stream));
     if (precision == HFEM_FP64)
     {
-        // inv(D) per element, then X^T = [F Ru]^T inv(D)^T (X row-major = column-major X^T,
-        // ld ldx) and [AH FH] = [H Rh] - K X as strided-batched DGEMMs
+        // inv(D) per element, then X = inv(D) [F Ru] and [AH FH] = [H Rh] - K X as
+        // strided-batched DGEMMs (solutions as picked by hfem_tune_condensation)
         HIPCHK(launch_dinv(d.na, e0, n, c->D, c->dinv, c->info, c->stream));
         const long sx = (long)d.na * c->ldx, sd = (long)d.na * d.na, sk = (long)d.nb * d.na,
                    sh = (long)d.nb * (d.nb + 1);
-        const double one = 1.0, zero = 0.0, mone = -1.0;
-        rocblas_status st = rocblas_dgemm_strided_batched(
-            c->blas, rocblas_operation_none, rocblas_operation_transpose, d.nb + 1, d.na, d.na, &one,
-            c->Fx + e0 * sx, c->ldx, sx, c->dinv + e0 * sd, d.na, sd, &zero, c->X + e0 * sx, c->ldx, sx, n);
+        rocblas_status st = cond_gemm_run(
+            c->blas, cond_gemm(c, 0, c->Fx + e0 * sx, c->dinv + e0 * sd, nullptr, c->X + e0 * sx, nullptr, n),
+            c->gemm_sol[0]);
         if (st == rocblas_status_success)
-            st = rocblas_dgemm_strided_batched(c->blas, rocblas_operation_none, rocblas_operation_transpose, d.nb,
-                                               d.nb + 1, d.na, &mone, c->K + e0 * sk, d.nb, sk, c->X + e0 * sx,
-                                               c->ldx, sx, &one, c->Hx + e0 * sh, d.nb, sh, n);
+            st = cond_gemm_run(c->blas,
+                               cond_gemm(c, 1, nullptr, nullptr, c->K + e0 * sk, c->X + e0 * sx, c->Hx + e0 * sh, n),
+                               c->gemm_sol[1]);
         if (st != rocblas_status_success)
-            return hfem_fail(-2, "hfem_condense: rocblas_dgemm_strided_batched failed: %s", rocblas_status_to_string(st));
+            return hfem_fail(-2, "hfem_condense: rocBLAS strided-batched GEMM failed: %s", rocblas_status_to_string(st));
     }
     else
         HIPCHK(launch_condense_mixed(d.na, d.nb, c->ldx, e0, n, c->D, c->Fx, c->X, c->rwork, c->K, c->Hx, c->info,

@@ -5,6 +5,9 @@
 #include <string>
 #include <vector>
 
+// rocblas_gemm_strided_batched_ex_get_solutions (solution selection for the
+// condensation GEMMs) is declared under the beta API
+#define ROCBLAS_BETA_FEATURES_API
 #include <rocblas/rocblas.h>
 
 #include "../../include/hfem_amd.h"
@@ -130,6 +133,8 @@ struct hfem_ctx {
     double *dinv = nullptr, *rwork = nullptr, *fluxbuf = nullptr;   // dinv: ne x na x na
     double *spill = nullptr;   // per-workgroup element tables of families beyond the LDS
     rocblas_handle blas = nullptr;
+    // rocBLAS solution indices of the two condensation GEMMs (0: rocBLAS's own choice)
+    int gemm_sol[2] = {0, 0};
     long fstride = 0;
     int chunk = 0;
     int grid_asm = 0, grid_cond = 0;

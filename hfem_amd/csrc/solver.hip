@@ -239,32 +239,43 @@ __device__ __forceinline__ double wave_bcast(double v, int src)
 
 // acc_i += sum_cc B[i + bsz cc] z[cc] for the lane's rows i = lane, lane + 64, with z
 // held one entry per lane (z0 = z[lane], z1 = z[lane + 64]) and broadcast by readlane.
+// The columns go in chunks of CH whose loads are all issued before the first
+// multiply (addresses clamped into the block; columns past bsz are multiplied by
+// zero), so a block costs ceil(bsz / CH) memory round trips instead of one per
+// column pair.
+template <int CH, bool TWO>
+__device__ __forceinline__ void wave_block_matvec_ch(int bsz, const double *__restrict__ B, double z0, double z1,
+                                                     double &acc0, double &acc1)
+{
+    const int lane = threadIdx.x & 63;
+    const int r0 = lane < bsz ? lane : 0, r1 = lane + 64 < bsz ? lane + 64 : 0;
+    double s0 = 0.0, s1 = 0.0;
+    for (int c0 = 0; c0 < bsz; c0 += CH) {
+        double v0[CH], v1[CH];
+#pragma unroll
+        for (int j = 0; j < CH; j++) {
+            const long col = (long)bsz * min(c0 + j, bsz - 1);
+            v0[j] = B[r0 + col];
+            if (TWO) v1[j] = B[r1 + col];
+        }
+#pragma unroll
+        for (int j = 0; j < CH; j++) {
+            const int cc = c0 + j;
+            double zj = wave_bcast(cc < 64 ? z0 : z1, cc & 63);
+            zj = cc < bsz ? zj : 0.0;
+            s0 += v0[j] * zj;
+            if (TWO) s1 += v1[j] * zj;
+        }
+    }
+    if (lane < bsz) acc0 += s0;
+    if (TWO && lane + 64 < bsz) acc1 += s1;
+}
+
 __device__ __forceinline__ void wave_block_matvec(int bsz, const double *__restrict__ B, double z0, double z1,
                                                   double &acc0, double &acc1)
 {
-    const int lane = threadIdx.x & 63;
-    const bool two = bsz > 64;
-    double s0 = 0.0, s1 = 0.0, u0 = 0.0, u1 = 0.0;
-    int cc = 0;
-    for (; cc + 2 <= bsz; cc += 2) {
-        const double za = wave_bcast(cc < 64 ? z0 : z1, cc & 63);
-        const double zb = wave_bcast(cc + 1 < 64 ? z0 : z1, (cc + 1) & 63);
-        if (lane < bsz) {
-            s0 += B[lane + (long)bsz * cc] * za;
-            u0 += B[lane + (long)bsz * (cc + 1)] * zb;
-        }
-        if (two && lane + 64 < bsz) {
-            s1 += B[lane + 64 + (long)bsz * cc] * za;
-            u1 += B[lane + 64 + (long)bsz * (cc + 1)] * zb;
-        }
-    }
-    if (cc < bsz) {
-        const double za = wave_bcast(cc < 64 ? z0 : z1, cc & 63);
-        if (lane < bsz) s0 += B[lane + (long)bsz * cc] * za;
-        if (two && lane + 64 < bsz) s1 += B[lane + 64 + (long)bsz * cc] * za;
-    }
-    acc0 += s0 + u0;
-    acc1 += s1 + u1;
+    if (bsz > 64) wave_block_matvec_ch<16, true>(bsz, B, z0, z1, acc0, acc1);
+    else wave_block_matvec_ch<32, false>(bsz, B, z0, z1, acc0, acc1);
 }
 
 // One BILU0 sweep over the rows of one level (kernel/applyBILU0.m).  For row r,
@@ -310,13 +321,28 @@ bilu0_sweep_kernel(int bsz, const int *__restrict__ rowpts, const int *__restric
         if (BACKWARD) {
             __syncthreads();
             // z_r = inv(U_rr) t: wave w takes the columns cc = w, w + nw, ...
+            // (loads of 8 columns issued together, addresses clamped into the block)
             const double *Ui = Mg + (long)k0 * b2;
+            const int r0 = lane < bsz ? lane : 0, r1 = lane + 64 < bsz ? lane + 64 : 0;
             double a0 = 0.0, a1 = 0.0;
-            for (int cc = wv; cc < bsz; cc += nw) {
-                const double tc = tb[cc];
-                if (lane < bsz) a0 += Ui[lane + (long)bsz * cc] * tc;
-                if (lane + 64 < bsz) a1 += Ui[lane + 64 + (long)bsz * cc] * tc;
+            for (int c0 = wv; c0 < bsz; c0 += 8 * nw) {
+                double v0[8], v1[8];
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const long col = (long)bsz * min(c0 + j * nw, bsz - 1);
+                    v0[j] = Ui[r0 + col];
+                    v1[j] = Ui[r1 + col];
+                }
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const int cc = c0 + j * nw;
+                    const double tc = cc < bsz ? tb[min(cc, bsz - 1)] : 0.0;
+                    a0 += v0[j] * tc;
+                    a1 += v1[j] * tc;
+                }
             }
+            if (lane >= bsz) a0 = 0.0;
+            if (lane + 64 >= bsz) a1 = 0.0;
             red[wv][lane] = a0;
             red[wv][lane + 64] = a1;
             __syncthreads();

@@ -49,6 +49,7 @@ EXPORTS = {
     "hfem_get_solution": (ctypes.c_int, [_vp, _dp, _dp]),
     "hfem_assemble": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "hfem_condense": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "hfem_tune_condensation": (ctypes.c_int, [_vp, ctypes.c_int, _ip, ctypes.POINTER(ctypes.c_float)]),
     "hfem_read_elements": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
     "hfem_read_condensed": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _ip]),
     "hfem_assemble_linear_system": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
@@ -183,6 +184,15 @@ class HDGContext:
     def schur_element_matrix_vector(self, e0=0, n=None, precision=FP64):
         n = self.prob.ne - e0 if n is None else n
         _check(load().hfem_condense(self.h, e0, n, precision))
+
+    def tune_condensation(self, nsample=0):
+        """Pick the rocBLAS solutions of the FP64 condensation's two batched
+        GEMMs; returns (solutions[2], ms[4]) with ms = (GEMM 1 default, picked,
+        GEMM 2 default, picked) on the sample."""
+        sol = (ctypes.c_int * 2)()
+        ms = (ctypes.c_float * 4)()
+        _check(load().hfem_tune_condensation(self.h, nsample, sol, ms))
+        return list(sol), list(ms)
 
     def read_elements(self, e0=0, n=None, which=("Ru", "D", "F", "K", "H", "Rh")):
         d = self.d

@@ -146,6 +146,35 @@ def test_linear_system_and_matvec(oracle_mod, kind, nc, p):
     ctx.close()
 
 
+def test_tuned_condensation(oracle_mod):
+    """The rocBLAS solutions picked by hfem_tune_condensation give the same blocks
+    (1e-13, a different summation order is allowed) and tuning changes no element
+    block."""
+    from hfem_amd.lib import HDGContext, FP64
+    O = oracle_mod
+    pr = make_problem("tet", 2, 3, seed=6)
+    d = pr.dims
+    say(f"tuned condensation tet2 p3: ne={pr.ne}")
+    ref = O.Oracle(pr).assemble()
+    O.schur(0, d["npv"], d["ncu"], ref)
+    ctx = HDGContext(pr)
+    ctx.assemble_element_matrix_vector()
+    before = ctx.read_elements()
+    sols, ms = ctx.tune_condensation()
+    say(f"  solutions {sols} ms {ms}")
+    after = ctx.read_elements()
+    for k in before:
+        assert np.array_equal(before[k], after[k]), k
+    assert ms[1] <= ms[0] and ms[3] <= ms[2]
+    ctx.schur_element_matrix_vector(precision=FP64)
+    c = ctx.read_condensed()
+    for name, want in (("DinvF", ref["F"]), ("AH", ref["H"]), ("FH", ref["Rh"])):
+        e = relerr(c[name], want)
+        say(f"  tuned {name}: {e:.2e}")
+        assert e < RTOL_MAT, name
+    ctx.close()
+
+
 def test_errors_are_reported():
     from hfem_amd.lib import HDGContext, HfemError
     pr = make_problem("tri", 1, 1, seed=0)
@@ -154,4 +183,6 @@ def test_errors_are_reported():
         ctx.assemble_element_matrix_vector(0, pr.ne + 1)
     with pytest.raises(HfemError, match="precision"):
         ctx.schur_element_matrix_vector(0, 1, precision=7)
+    with pytest.raises(HfemError, match="nsample"):
+        ctx.tune_condensation(-1)
     ctx.close()
