@@ -264,8 +264,12 @@ extern "C" int hfem_precond_factor(hfem_ctx *c, int kind)
     std::vector<int> info(n);
     HIPCHK(hipMemcpyAsync(info.data(), s.d_info, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    for (int i = 0; i < n; i++)
+    for (int i = 0; i < n; i++) {
+        if (info[i] < 0)
+            return hfem_fail(-4, "hfem_precond_factor: row %d has %d blocks, more than the factorisation takes", i,
+                             -info[i]);
         if (info[i]) return hfem_fail(-4, "hfem_precond_factor: singular pivot block in row %d", i);
+    }
     s.prec = kind;
     s.ms_factor = now_ms() - t0;
     return 0;

@@ -150,8 +150,78 @@ mdf_weight_kernel(int nrows, int bsz, const int *__restrict__ rowpts, const doub
 // L_ik A_kl for the later common neighbours rl; finally A_ii is inverted.  Each
 // block of row ri receives the same updates in the same order as in the
 // reference's right-looking loop; rows of one level do not depend on each other.
-// The rows' own blocks are read and written with one fixed entry -> thread map,
-// so a thread only ever re-reads global entries it wrote itself.
+// The column lists of row ri and of the current neighbour row sit in LDS, so the
+// neighbour order and the target search cost no memory round trips; blocks are
+// staged into LDS with eight loads in flight per thread and multiplied on the
+// FP64 MFMA.  The blocks of row ri that are read again (the targets and the
+// diagonal) are always read and written with the fixed entry -> thread map of
+// wg_load_block / wg_subtract_block, so a thread only ever re-reads global
+// entries it wrote itself; L_ik, stored from the MFMA result layout, is not read
+// again by the row.
+constexpr int FACTOR_MAXROW = 64;     // longest BSR row the factorisation takes
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// The n x n product C = A B of two column-major LDS blocks on the FP64 MFMA: the
+// 16 x 16 tiles of C are spread over the waves, rows and columns past n are
+// masked to zero; out(i, j, c) receives each entry of C (i, j < n).  MFMA
+// 16x16x4 operands: lane l supplies A(l%16, l/16) and B(l/16, l%16); result lane
+// l, element v holds C(l/16 + 4v, l%16).
+template <typename OUT>
+__device__ __forceinline__ void lds_block_product(const double *A, const double *B, int n, OUT out)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int lr = lane & 15, lk = lane >> 4, nt = (n + 15) / 16;
+    for (int t = wave; t < nt * nt; t += nw) {
+        const int i0 = (t % nt) * 16, j0 = (t / nt) * 16;
+        const int ia = min(i0 + lr, n - 1), jb = min(j0 + lr, n - 1);
+        const bool va = i0 + lr < n, vb = j0 + lr < n;
+        f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+        for (int k0 = 0; k0 < n; k0 += 4) {
+            const int k = k0 + lk, kc = min(k, n - 1);
+            const double a = (va && k < n) ? A[ia + n * kc] : 0.0;
+            const double b = (vb && k < n) ? B[kc + n * jb] : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int i = i0 + lk + 4 * v, j = j0 + lr;
+            if (i < n && j < n) out(i, j, acc[v]);
+        }
+    }
+}
+
+// dst[i] = src[i] for i < n (n >= 1) by the workgroup, eight loads in flight per
+// thread (addresses clamped into the block).  Entry i is always handled by thread
+// i % blockDim.x.
+__device__ __forceinline__ void wg_load_block(double *dst, const double *__restrict__ src, long n)
+{
+    const int tid = threadIdx.x, nth = blockDim.x;
+    for (long b = tid; b < n; b += 8L * nth) {
+        double v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) v[j] = src[min(b + (long)j * nth, n - 1)];
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if (b + (long)j * nth < n) dst[b + (long)j * nth] = v[j];
+    }
+}
+
+// dst[i] -= d[i] for i < n, global dst, LDS d; same entry -> thread map as
+// wg_load_block.
+__device__ __forceinline__ void wg_subtract_block(double *dst, const double *d, long n)
+{
+    const int tid = threadIdx.x, nth = blockDim.x;
+    for (long b = tid; b < n; b += 8L * nth) {
+        double v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) v[j] = dst[min(b + (long)j * nth, n - 1)];
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if (b + (long)j * nth < n) dst[b + (long)j * nth] = v[j] - d[b + (long)j * nth];
+    }
+}
+
 __global__ void __launch_bounds__(SV_THREADS)
 bilu0_factor_kernel(int bsz, const int *__restrict__ rowpts, const int *__restrict__ colind,
                     const int *__restrict__ u2o, const int *__restrict__ rows, int nlev, double *__restrict__ Mg,
@@ -162,64 +232,70 @@ bilu0_factor_kernel(int bsz, const int *__restrict__ rowpts, const int *__restri
     const long b2 = (long)bsz * bsz;
     double *X = lds, *B = lds + b2, *L = lds + 2 * b2;
     LdsInv w = lds_inv_area(lds + 3 * b2, bsz);
-    int *lst = w.piv + bsz + 2;            // [0]: count, then pairs (block position, row)
+    // row ri: columns and their order numbers; the lower neighbours' positions by
+    // increasing order number; the current neighbour row's columns
+    int *rcol = w.piv + bsz + 2, *rord = rcol + FACTOR_MAXROW, *lst = rord + FACTOR_MAXROW,
+        *ncol = lst + FACTOR_MAXROW + 1;
     for (int t = blockIdx.x; t < nlev; t += gridDim.x) {
-        const int ri = rows[t], oi = u2o[ri], d0 = rowpts[ri], d1 = rowpts[ri + 1];
+        const int ri = rows[t], oi = u2o[ri], d0 = rowpts[ri], len = rowpts[ri + 1] - d0;
+        if (len > FACTOR_MAXROW) {
+            // workgroup-uniform: reported through info, the row is left as it is
+            if (tid == 0) info[ri] = -len;
+            continue;
+        }
+        if (tid < len) {
+            const int c = colind[d0 + tid];
+            rcol[tid] = c;
+            rord[tid] = u2o[c];
+        }
+        __syncthreads();
         if (tid == 0) {
             int cnt = 0;
-            for (int k = d0 + 1; k < d1; k++) {
-                const int rk = colind[k];
-                if (u2o[rk] >= oi) continue;
-                int q = cnt++;                   // insertion by increasing order number
-                while (q > 0 && u2o[lst[1 + 2 * (q - 1) + 1]] > u2o[rk]) {
-                    lst[1 + 2 * q] = lst[1 + 2 * (q - 1)];
-                    lst[2 + 2 * q] = lst[2 + 2 * (q - 1)];
-                    q--;
+            for (int q = 1; q < len; q++) {
+                if (rord[q] >= oi) continue;
+                int m = cnt++;                    // insertion by increasing order number
+                while (m > 0 && rord[lst[1 + m - 1]] > rord[q]) {
+                    lst[1 + m] = lst[1 + m - 1];
+                    m--;
                 }
-                lst[1 + 2 * q] = k;
-                lst[2 + 2 * q] = rk;
+                lst[1 + m] = q;
             }
             lst[0] = cnt;
         }
         __syncthreads();
         const int cnt = lst[0];
-        for (int q = 0; q < cnt; q++) {
-            const int ik = lst[1 + 2 * q], rk = lst[2 + 2 * q];
-            const int k0 = rowpts[rk], k1 = rowpts[rk + 1];
-            for (long i = tid; i < b2; i += nth) {
-                X[i] = Mg[(long)ik * b2 + i];
-                B[i] = Mg[(long)k0 * b2 + i];     // inv(U_kk)
-            }
+        for (int qq = 0; qq < cnt; qq++) {
+            const int q = lst[1 + qq], ik = d0 + q, rk = rcol[q], ork = rord[q];
+            const int k0 = rowpts[rk], klen = rowpts[rk + 1] - k0;
+            if (tid < klen && tid < FACTOR_MAXROW) ncol[tid] = colind[k0 + tid];
+            wg_load_block(X, Mg + (long)ik * b2, b2);
+            wg_load_block(B, Mg + (long)k0 * b2, b2);     // inv(U_kk)
             __syncthreads();
-            for (long idx = tid; idx < b2; idx += nth) {
-                const int i = idx % bsz, j = idx / bsz;
-                double s = 0.0;
-                for (int k = 0; k < bsz; k++) s += X[i + bsz * k] * B[k + bsz * j];
-                L[idx] = s;
-                Mg[(long)ik * b2 + idx] = s;
-            }
+            // L_ik = A_ik inv(U_kk), kept in LDS and stored in place of A_ik
+            double *Mik = Mg + (long)ik * b2;
+            lds_block_product(X, B, bsz, [&](int i, int j, double c) {
+                L[i + bsz * j] = c;
+                Mik[i + bsz * j] = c;
+            });
             __syncthreads();
             // targets in row ri: the diagonal (source: ri's position in row rk) and the
             // later common neighbours rl (source: rl's position in row rk)
-            for (int l = d0; l < d1; l++) {
-                const int rl = colind[l];
-                if (l != d0 && u2o[rl] <= u2o[rk]) continue;
+            for (int l = 0; l < len; l++) {
+                if (l != 0 && rord[l] <= ork) continue;
+                const int rl = rcol[l];
                 int km = -1;
-                for (int m = k0 + 1; m < k1; m++)
-                    if (colind[m] == rl) { km = m; break; }
+                for (int m = 1; m < klen && m < FACTOR_MAXROW; m++)
+                    if (ncol[m] == rl) { km = m; break; }
                 if (km < 0) continue;
-                for (long i = tid; i < b2; i += nth) B[i] = Mg[(long)km * b2 + i];
+                wg_load_block(B, Mg + (long)(k0 + km) * b2, b2);
                 __syncthreads();
-                for (long idx = tid; idx < b2; idx += nth) {
-                    const int i = idx % bsz, j = idx / bsz;
-                    double s = 0.0;
-                    for (int k = 0; k < bsz; k++) s += L[i + bsz * k] * B[k + bsz * j];
-                    Mg[(long)l * b2 + idx] -= s;
-                }
+                lds_block_product(L, B, bsz, [&](int i, int j, double c) { X[i + bsz * j] = c; });
+                __syncthreads();
+                wg_subtract_block(Mg + (long)(d0 + l) * b2, X, b2);
                 __syncthreads();
             }
         }
-        for (long i = tid; i < b2; i += nth) X[i] = Mg[(long)d0 * b2 + i];
+        wg_load_block(X, Mg + (long)d0 * b2, b2);
         __syncthreads();
         const int r = lds_inverse(X, bsz, w.piv, w.rowk, w.colk, w.red);
         for (long i = tid; i < b2; i += nth) Mg[(long)d0 * b2 + i] = X[i];
@@ -498,7 +574,10 @@ static hipError_t set_lds(const void *fn, size_t bytes)
 }
 
 size_t block_inverse_lds(int bsz) { return sizeof(double) * bsz * bsz + lds_inv_bytes(bsz); }
-size_t bilu0_factor_lds(int bsz) { return sizeof(double) * 3 * bsz * bsz + lds_inv_bytes(bsz) + sizeof(int) * 80; }
+size_t bilu0_factor_lds(int bsz)
+{
+    return sizeof(double) * 3 * bsz * bsz + lds_inv_bytes(bsz) + sizeof(int) * (4 * FACTOR_MAXROW + 1);
+}
 
 hipError_t launch_block_inverse(int nrows, int bsz, const int *rowpts, const double *Hg, double *Dinv, int *info,
                                 hipStream_t s)
