@@ -351,7 +351,7 @@ __device__ __forceinline__ void wave_block_matvec(int bsz, const double *__restr
                                                   double &acc0, double &acc1)
 {
     if (bsz > 64) wave_block_matvec_ch<16, true>(bsz, B, z0, z1, acc0, acc1);
-    else wave_block_matvec_ch<32, false>(bsz, B, z0, z1, acc0, acc1);
+    else wave_block_matvec_ch<64, false>(bsz, B, z0, z1, acc0, acc1);
 }
 
 // One BILU0 sweep over the rows of one level (kernel/applyBILU0.m).  For row r,
@@ -361,60 +361,88 @@ __device__ __forceinline__ void wave_block_matvec(int bsz, const double *__restr
 // 256-thread workgroup per row: the four waves take the neighbour blocks in turn
 // (lane = block row, the neighbour vector broadcast from registers), their partial
 // sums meet in LDS.
-template <bool BACKWARD>
+template <bool BACKWARD, bool SMALL>
 __global__ void __launch_bounds__(SV_THREADS)
 bilu0_sweep_kernel(int bsz, const int *__restrict__ rowpts, const int *__restrict__ colind,
                    const int *__restrict__ u2o, const int *__restrict__ rows, int nlev, const double *__restrict__ Mg,
                    const double *__restrict__ b, double *__restrict__ z)
 {
-    __shared__ double red[SV_THREADS / 64][128];
+    constexpr int NW = SV_THREADS / 64;
+    __shared__ double red[NW][128];
     __shared__ double tb[128];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+    __shared__ int scol[FACTOR_MAXROW], stake[FACTOR_MAXROW];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const long b2 = (long)bsz * bsz;
+    const int r0 = lane < bsz ? lane : 0, r1 = lane + 64 < bsz ? lane + 64 : 0;
     for (int t = blockIdx.x; t < nlev; t += gridDim.x) {
-        const int r = rows[t], o = u2o[r], k0 = rowpts[r], k1 = rowpts[r + 1];
+        const int r = rows[t], o = u2o[r], k0 = rowpts[r], len = min(rowpts[r + 1] - k0, FACTOR_MAXROW);
+        const double *Ui = Mg + (long)k0 * b2;
+        // the loads that depend on nothing else go out together: the row's column
+        // list (rows have at most FACTOR_MAXROW blocks: the factorisation rejects
+        // longer ones), its right-hand side and, backward, this wave's columns
+        // wv + NW j of inv(U_rr)
+        if (tid < len) {
+            const int c = colind[k0 + tid];
+            scol[tid] = c;
+            stake[tid] = BACKWARD ? (u2o[c] > o) : (u2o[c] < o);
+        }
+        const double bv = tid < bsz ? b[(long)r * bsz + tid] : 0.0;
+        constexpr int ND = (SMALL && BACKWARD) ? 64 / NW : 1;
+        double dv[ND];
+        if (SMALL && BACKWARD) {
+#pragma unroll
+            for (int j = 0; j < ND; j++) dv[j] = Ui[r0 + (long)bsz * min(wv + NW * j, bsz - 1)];
+        }
+        __syncthreads();
         double acc0 = 0.0, acc1 = 0.0;
         int sel = 0;
-        for (int k = k0 + 1; k < k1; k++) {
-            const int c = colind[k];
-            const bool take = BACKWARD ? (u2o[c] > o) : (u2o[c] < o);
-            if (!take) continue;
-            if ((sel++ % nw) != wv) continue;
+        for (int q = 1; q < len; q++) {
+            if (!stake[q]) continue;
+            if ((sel++ % NW) != wv) continue;
+            const int c = scol[q];
             const double *zc = z + (long)c * bsz;
             const double z0 = lane < bsz ? zc[lane] : 0.0, z1 = lane + 64 < bsz ? zc[lane + 64] : 0.0;
-            wave_block_matvec(bsz, Mg + (long)k * b2, z0, z1, acc0, acc1);
+            if (SMALL) wave_block_matvec_ch<64, false>(bsz, Mg + (long)(k0 + q) * b2, z0, z1, acc0, acc1);
+            else wave_block_matvec_ch<16, true>(bsz, Mg + (long)(k0 + q) * b2, z0, z1, acc0, acc1);
         }
         red[wv][lane] = acc0;
         red[wv][lane + 64] = acc1;
         __syncthreads();
-        for (int i = tid; i < bsz; i += blockDim.x) {
+        if (tid < bsz) {
             double sum = 0.0;
-            for (int w = 0; w < nw; w++) sum += red[w][i];
-            const double v = b[(long)r * bsz + i] - sum;
-            if (BACKWARD) tb[i] = v;
-            else z[(long)r * bsz + i] = v;
+#pragma unroll
+            for (int w = 0; w < NW; w++) sum += red[w][tid];
+            const double v = bv - sum;
+            if (BACKWARD) tb[tid] = v;
+            else z[(long)r * bsz + tid] = v;
         }
         if (BACKWARD) {
             __syncthreads();
-            // z_r = inv(U_rr) t: wave w takes the columns cc = w, w + nw, ...
-            // (loads of 8 columns issued together, addresses clamped into the block)
-            const double *Ui = Mg + (long)k0 * b2;
-            const int r0 = lane < bsz ? lane : 0, r1 = lane + 64 < bsz ? lane + 64 : 0;
+            // z_r = inv(U_rr) t: wave w takes the columns cc = w, w + NW, ...
             double a0 = 0.0, a1 = 0.0;
-            for (int c0 = wv; c0 < bsz; c0 += 8 * nw) {
-                double v0[8], v1[8];
+            if (SMALL) {
 #pragma unroll
-                for (int j = 0; j < 8; j++) {
-                    const long col = (long)bsz * min(c0 + j * nw, bsz - 1);
-                    v0[j] = Ui[r0 + col];
-                    v1[j] = Ui[r1 + col];
+                for (int j = 0; j < ND; j++) {
+                    const int cc = wv + NW * j;
+                    a0 += dv[j] * (cc < bsz ? tb[min(cc, bsz - 1)] : 0.0);
                 }
+            } else {
+                // (loads of 8 columns issued together, addresses clamped into the block)
+                for (int c0 = wv; c0 < bsz; c0 += 8 * NW) {
+                    double v0[8], v1[8];
 #pragma unroll
-                for (int j = 0; j < 8; j++) {
-                    const int cc = c0 + j * nw;
-                    const double tc = cc < bsz ? tb[min(cc, bsz - 1)] : 0.0;
-                    a0 += v0[j] * tc;
-                    a1 += v1[j] * tc;
+                    for (int j = 0; j < 8; j++) {
+                        const long col = (long)bsz * min(c0 + j * NW, bsz - 1);
+                        v0[j] = Ui[r0 + col];
+                        v1[j] = Ui[r1 + col];
+                    }
+#pragma unroll
+                    for (int j = 0; j < 8; j++) {
+                        const int cc = c0 + j * NW;
+                        const double tc = cc < bsz ? tb[min(cc, bsz - 1)] : 0.0;
+                        a0 += v0[j] * tc;
+                        a1 += v1[j] * tc;
+                    }
                 }
             }
             if (lane >= bsz) a0 = 0.0;
@@ -422,10 +450,11 @@ bilu0_sweep_kernel(int bsz, const int *__restrict__ rowpts, const int *__restric
             red[wv][lane] = a0;
             red[wv][lane + 64] = a1;
             __syncthreads();
-            for (int i = tid; i < bsz; i += blockDim.x) {
+            if (tid < bsz) {
                 double sum = 0.0;
-                for (int w = 0; w < nw; w++) sum += red[w][i];
-                z[(long)r * bsz + i] = sum;
+#pragma unroll
+                for (int w = 0; w < NW; w++) sum += red[w][tid];
+                z[(long)r * bsz + tid] = sum;
             }
         }
         __syncthreads();
@@ -616,8 +645,12 @@ hipError_t launch_bilu0_lsolve_level(int bsz, const int *rowpts, const int *coli
                                      int nlev, const double *Mg, const double *x, double *y, hipStream_t s)
 {
     if (bsz > 128) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(bilu0_sweep_kernel<false>, dim3(std::min(nlev, 8192)), dim3(SV_THREADS), 0, s, bsz, rowpts,
-                       colind, u2o, rows, nlev, Mg, x, y);
+    if (bsz <= 64)
+        hipLaunchKernelGGL((bilu0_sweep_kernel<false, true>), dim3(std::min(nlev, 8192)), dim3(SV_THREADS), 0, s, bsz,
+                           rowpts, colind, u2o, rows, nlev, Mg, x, y);
+    else
+        hipLaunchKernelGGL((bilu0_sweep_kernel<false, false>), dim3(std::min(nlev, 8192)), dim3(SV_THREADS), 0, s,
+                           bsz, rowpts, colind, u2o, rows, nlev, Mg, x, y);
     return hipGetLastError();
 }
 
@@ -625,8 +658,12 @@ hipError_t launch_bilu0_usolve_level(int bsz, const int *rowpts, const int *coli
                                      int nlev, const double *Mg, const double *y, double *v, hipStream_t s)
 {
     if (bsz > 128) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(bilu0_sweep_kernel<true>, dim3(std::min(nlev, 8192)), dim3(SV_THREADS), 0, s, bsz, rowpts,
-                       colind, u2o, rows, nlev, Mg, y, v);
+    if (bsz <= 64)
+        hipLaunchKernelGGL((bilu0_sweep_kernel<true, true>), dim3(std::min(nlev, 8192)), dim3(SV_THREADS), 0, s, bsz,
+                           rowpts, colind, u2o, rows, nlev, Mg, y, v);
+    else
+        hipLaunchKernelGGL((bilu0_sweep_kernel<true, false>), dim3(std::min(nlev, 8192)), dim3(SV_THREADS), 0, s, bsz,
+                           rowpts, colind, u2o, rows, nlev, Mg, y, v);
     return hipGetLastError();
 }
 
