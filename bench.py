@@ -106,6 +106,59 @@ def trace_solver(ctx, restart=30, tol=1e-8, maxit=100):
                 bilu0_apply_ms=st["precond_ms"] / max(st["preconds"], 1))
 
 
+def newton_step_2d(ncell=16):
+    """One Newton step's linear solve (assembleLinearSystem, computeOrdering,
+    computePreconditioner, GMRES(30) to 1e-10) on the 16x16-cell jittered square
+    split into 512 p=2 triangles, after one untimed solve of the same system."""
+    from hfem_amd.problem import make_problem
+    from hfem_amd.lib import HDGContext
+    prob = make_problem("tri", ncell, 2, seed=0)
+    ctx = HDGContext(prob)
+    ctx.tune_condensation()
+    trace_solver(ctx, tol=1e-10, maxit=1000)
+    ctx.sync()
+    t0 = time.perf_counter()
+    sol = trace_solver(ctx, tol=1e-10, maxit=1000)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    print(json.dumps({"mode": "newton", "workload": f"2D NS HDG p=2 triangles, {ncell}x{ncell}x2={prob.ne} "
+                      "elements, one Newton step's trace solve", "elements": prob.ne, "faces": prob.mesh.nf,
+                      "wall_ms": wall * 1e3, **{k: (float(v) if isinstance(v, (np.floating, float)) else v)
+                                                 for k, v in sol.items()}}))
+    ctx.close()
+
+
+def dirk_step_hex(ncell=24, dt=0.1):
+    """Wall time of DIRK(3,3) steps (solveUnsteadyProblem's inner step: three
+    Newton solves with GMRES(30) and BILU0 on the trace system) on 24^3 jittered
+    periodic p=2 hexes from the Taylor-Green vortex (velocity amplitude 0.1) plus
+    seeded noise 1e-3.  The first step also computes the MDF ordering, which the
+    second reuses; both are reported."""
+    from hfem_amd.problem import make_problem
+    from hfem_amd.lib import HDGContext
+    prob = make_problem("hex", ncell, 2, seed=0, state="taylor_green", amp=1e-3)
+    print(f"[bench] dirk: hex p=2 ne={prob.ne} nf={prob.mesh.nf}", file=sys.stderr, flush=True)
+    ctx = HDGContext(prob)
+    ctx.tune_condensation()
+    steps = []
+    for k in range(2):
+        ctx.sync()
+        t0 = time.perf_counter()
+        conv, nit = ctx.dirk_step(dt, 3, 3)
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        st = ctx.solver_stats()
+        steps.append({"wall_s": wall, "converged": conv, "newton_iterations": nit,
+                      "last_solve": {k2: float(v) for k2, v in st.items()}})
+        print(f"[bench] dirk step {k}: {wall:.3f} s, {nit} Newton iterations, converged {conv}",
+              file=sys.stderr, flush=True)
+    _, nlev = ctx.get_ordering()
+    print(json.dumps({"mode": "dirk", "workload": f"3D NS HDG p=2 hexes, {ncell}^3={prob.ne} elements, periodic, "
+                      "Taylor-Green 0.1 + noise 1e-3", "elements": prob.ne, "faces": prob.mesh.nf, "dt": dt,
+                      "stages": 3, "order": 3, "mdf_levels": [int(v) for v in nlev], "steps": steps}))
+    ctx.close()
+
+
 def cpu_baseline(prob, seconds=10.0):
     """The reference's own element path (oracle/_ref, MKL, 1 thread) on a bounded sample."""
     os.environ.setdefault("MKL_NUM_THREADS", "1")
@@ -146,8 +199,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
     ap.add_argument("--no-solver", action="store_true", help="skip the trace-system solve")
-    ap.add_argument("--mode", choices=["bench", "copy"], default="bench",
-                    help="copy: a plain 2 GiB device copy for the HBM counter calibration")
+    ap.add_argument("--mode", choices=["bench", "copy", "newton", "dirk"], default="bench",
+                    help="copy: a plain 2 GiB device copy for the HBM counter calibration; newton: one Newton "
+                         "step's trace solve on 512 p=2 triangles; dirk: one DIRK(3,3) step on 24^3 periodic "
+                         "p=2 hexes (these two print their own line, not the element-path metric)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -156,6 +211,12 @@ def main():
 
     if args.mode == "copy":
         copy_calibration()
+        return
+    if args.mode == "newton":
+        newton_step_2d()
+        return
+    if args.mode == "dirk":
+        dirk_step_hex()
         return
 
     from hfem_amd.problem import make_problem

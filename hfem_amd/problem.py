@@ -96,13 +96,32 @@ def _uhat_average(mesh, UDG, ncu):
     return UH / cnt[:, None]
 
 
+def taylor_green_state(x, gamma, minf, amp):
+    """Conservative state of the Taylor-Green vortex on the unit cube (period 1):
+    rho = 1, velocity amp (sin X cos Y cos Z, -cos X sin Y cos Z, 0) with
+    X = 2 pi x, ..., pressure 1/(gamma minf^2) + amp^2/16 (cos 2X + cos 2Y)(cos 2Z + 2).
+    x: (..., 3) points; returns (..., 5)."""
+    X, Y, Z = (2 * np.pi * x[..., i] for i in range(3))
+    u = amp * np.sin(X) * np.cos(Y) * np.cos(Z)
+    v = -amp * np.cos(X) * np.sin(Y) * np.cos(Z)
+    w = np.zeros_like(u)
+    p = 1.0 / (gamma * minf ** 2) + amp ** 2 / 16.0 * (np.cos(2 * X) + np.cos(2 * Y)) * (np.cos(2 * Z) + 2.0)
+    rho = np.ones_like(u)
+    E = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v + w * w)
+    return np.stack([rho, rho * u, rho * v, rho * w, E], axis=-1)
+
+
 def make_problem(kind, ncell, porder, seed=0, tau=2.0, re=1000.0, pr=0.72, minf=0.3, gamma=1.4,
-                 amp=1e-2, qamp=1e-2, jitter=0.05, angle=0.0, elevation=0.0, name=None):
+                 amp=1e-2, qamp=1e-2, jitter=0.05, angle=0.0, elevation=0.0, name=None, state="freestream",
+                 tg_amp=0.1):
     """Build a seeded case.  kind: 'tri' (2D), 'tet' (3D), 'hex' (3D periodic).
 
     ``angle`` turns the freestream in the x-y plane; at angle 0 the flow is
     tangential to the y-faces of the box, where the normal velocity crosses zero
-    and |A_n| of the far-field flux is not differentiable."""
+    and |A_n| of the far-field flux is not differentiable.  ``state`` is
+    'freestream' (the freestream plus a seeded low-mode perturbation of amplitude
+    ``amp``) or 'taylor_green' (3D only: the Taylor-Green vortex of velocity
+    amplitude ``tg_amp`` plus seeded uniform noise of amplitude ``amp``)."""
     if kind == "tri":
         msh = meshmod.square_tri_mesh(ncell, porder, seed=seed, jitter=jitter)
     elif kind == "tet":
@@ -120,13 +139,21 @@ def make_problem(kind, ncell, porder, seed=0, tau=2.0, re=1000.0, pr=0.72, minf=
     x = msh.dgnodes                                   # (ne, npv, nd)
     ne = msh.ne
     UDG = np.zeros((ne, nc, m.npv))
-    for c in range(ncu):
-        pert = np.zeros(x.shape[:2])
-        for _ in range(3):
-            kvec = rng.integers(1, 3, size=nd)
-            phase = rng.uniform(0, 2 * np.pi)
-            pert += np.sin(2 * np.pi * (x @ kvec) + phase)
-        UDG[:, c, :] = u0[c] + amp * pert / 3.0
+    if state == "taylor_green":
+        if nd != 3:
+            raise ValueError("the Taylor-Green state is 3D")
+        tg = taylor_green_state(x, gamma, minf, tg_amp)          # (ne, npv, 5)
+        UDG[:, :ncu, :] = np.transpose(tg, (0, 2, 1)) + rng.uniform(-amp, amp, size=(ne, ncu, m.npv))
+    elif state == "freestream":
+        for c in range(ncu):
+            pert = np.zeros(x.shape[:2])
+            for _ in range(3):
+                kvec = rng.integers(1, 3, size=nd)
+                phase = rng.uniform(0, 2 * np.pi)
+                pert += np.sin(2 * np.pi * (x @ kvec) + phase)
+            UDG[:, c, :] = u0[c] + amp * pert / 3.0
+    else:
+        raise ValueError(state)
     UDG[:, ncu:, :] = rng.uniform(-qamp, qamp, size=(ne, nc - ncu, m.npv))
     UH = _uhat_average(msh, UDG, ncu)
     param = np.array([gamma, 0.0, re, pr, minf, tau])
