@@ -159,6 +159,63 @@ def dirk_step_hex(ncell=24, dt=0.1):
     ctx.close()
 
 
+def ragged_batch(steps, warmup, total=20000, ncell=10, seed=0):
+    """Assembly + FP64 condensation of an element batch of p=1..4 tets, the order of
+    each of `total` elements drawn uniformly by the seed.  One context per order,
+    each on its own stream (elements of one order are contiguous in its context;
+    the meshes are the 10^3 x 6 jittered cube, of which the first count_p
+    elements are taken).  Timed two ways over the whole batch: every order's
+    launches queued before one synchronisation (kernels of different orders may
+    run side by side), and one order after another with a synchronisation in
+    between (the one-launch-per-order baseline)."""
+    from hfem_amd.problem import make_problem
+    from hfem_amd.lib import HDGContext, FP64
+    rng = np.random.default_rng(seed)
+    counts = np.bincount(rng.integers(1, 5, size=total), minlength=5)[1:]
+    ctxs = []
+    for p, cnt in zip((1, 2, 3, 4), counts):
+        prob = make_problem("tet", ncell, p, seed=seed)
+        assert prob.ne >= cnt
+        ctx = HDGContext(prob)
+        ctx.tune_condensation(int(cnt))
+        ctxs.append((p, int(cnt), ctx))
+    print(f"[bench] ragged batch: counts by order {dict((p, c) for p, c, _ in ctxs)}", file=sys.stderr, flush=True)
+
+    def run(concurrent):
+        for p, cnt, ctx in ctxs:
+            ctx.assemble_element_matrix_vector(0, cnt)
+            ctx.schur_element_matrix_vector(0, cnt, FP64)
+            if not concurrent:
+                ctx.sync()
+        for _, _, ctx in ctxs:
+            ctx.sync()
+
+    res = {}
+    for name, conc in (("per_order", False), ("concurrent", True)):
+        for _ in range(warmup):
+            run(conc)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            run(conc)
+        wall = (time.perf_counter() - t0) / steps
+        res[name] = {"ms_per_batch": wall * 1e3, "elements_per_s": total / wall}
+        print(f"[bench] ragged {name}: {wall * 1e3:.2f} ms per batch, {total / wall:.0f} elements/s",
+              file=sys.stderr, flush=True)
+    per_order_ms = {}
+    for p, cnt, ctx in ctxs:
+        ctx.assemble_element_matrix_vector(0, cnt)
+        ctx.schur_element_matrix_vector(0, cnt, FP64)
+        a, c = ctx.kernel_ms()
+        per_order_ms[f"p{p}"] = {"elements": cnt, "assemble_ms": a, "condense_ms": c}
+    print(json.dumps({"mode": "ragged", "workload": f"3D NS HDG tets of orders 1-4 drawn uniformly by the seed, "
+                      f"{total} elements, assemble+condense (FP64)", "elements": total, "steps": steps,
+                      "launch": "one context per order on its own stream; not a single persistent launch",
+                      **res, "speedup_concurrent_vs_per_order": res["per_order"]["ms_per_batch"]
+                      / res["concurrent"]["ms_per_batch"], "per_order": per_order_ms}))
+    for _, _, ctx in ctxs:
+        ctx.close()
+
+
 def cpu_baseline(prob, seconds=10.0):
     """The reference's own element path (oracle/_ref, MKL, 1 thread) on a bounded sample."""
     os.environ.setdefault("MKL_NUM_THREADS", "1")
@@ -199,10 +256,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
     ap.add_argument("--no-solver", action="store_true", help="skip the trace-system solve")
-    ap.add_argument("--mode", choices=["bench", "copy", "newton", "dirk"], default="bench",
+    ap.add_argument("--mode", choices=["bench", "copy", "newton", "dirk", "ragged"], default="bench",
                     help="copy: a plain 2 GiB device copy for the HBM counter calibration; newton: one Newton "
                          "step's trace solve on 512 p=2 triangles; dirk: one DIRK(3,3) step on 24^3 periodic "
-                         "p=2 hexes (these two print their own line, not the element-path metric)")
+                         "p=2 hexes; ragged: 20000 tets of orders 1-4 drawn by the seed (these three print "
+                         "their own line, not the element-path metric)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -217,6 +275,9 @@ def main():
         return
     if args.mode == "dirk":
         dirk_step_hex()
+        return
+    if args.mode == "ragged":
+        ragged_batch(args.steps, args.warmup)
         return
 
     from hfem_amd.problem import make_problem
