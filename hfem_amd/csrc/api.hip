@@ -90,14 +90,9 @@ extern "C" int hfem_create(hfem_ctx **out, const hfem_dims *hd, int device)
             delete c;
             return hfem_fail(-1, "hfem_create: no assembly kernel compiled for nd=%d npv=%d npf=%d", d.nd, d.npv, d.npf);
         }
-        if (condense_lds_bytes(d.na, true, sizeof(float)) > (size_t)maxlds) {
-            delete c;
-            return hfem_fail(-1, "hfem_create: na=%d needs more LDS than the device offers (%d bytes)", d.na, maxlds);
-        }
     }
     ALLOC(dinv, ne * d.na * d.na);
     if (assemble_spill_doubles(d) > 0) ALLOC(spill, (long)c->grid_asm * assemble_spill_doubles(d));
-    ALLOC(rwork, (long)c->grid_cond * d.na * c->ldx);
 #undef ALLOC
     if (e == hipSuccess) e = hipStreamCreate(&c->stream);
     if (e == hipSuccess && (rocblas_create_handle(&c->blas) != rocblas_status_success ||
@@ -437,7 +432,6 @@ extern "C" int hfem_condense(hfem_ctx *c, int e0, int n, int precision)
     if (n == 0) return 0;
     HIPCHK(hipSetDevice(c->device));
     const Dims &d = c->d;
-    const int grid = std::min(n, c->grid_cond);
     HIPCHK(hipEventRecord(c->ev[2], c->stream));
     if (precision == HFEM_FP64)
     {
@@ -456,9 +450,37 @@ extern "C" int hfem_condense(hfem_ctx *c, int e0, int n, int precision)
         if (st != rocblas_status_success)
             return hfem_fail(-2, "hfem_condense: rocBLAS strided-batched GEMM failed: %s", rocblas_status_to_string(st));
     }
-    else
-        HIPCHK(launch_condense_mixed(d.na, d.nb, c->ldx, e0, n, c->D, c->Fx, c->X, c->rwork, c->K, c->Hx, c->info,
-                                     grid, 2, c->stream));
+    else {
+        // FP32 Gauss-Jordan inv32 (widened to FP64), X0 = inv32 [F Ru], then two sweeps
+        // X += inv32 ([F Ru] - D X) with the residual in FP64, then [AH FH] = [H Rh] - K X
+        if (!c->rwork) HIPCHK(hfem_dalloc(&c->rwork, (size_t)c->hd.ne * d.na * c->ldx));
+        HIPCHK(launch_dinv_gj(d.na, e0, n, c->D, c->dinv, c->info, 1, c->stream));
+        const long sx = (long)d.na * c->ldx, sd = (long)d.na * d.na, sk = (long)d.nb * d.na,
+                   sh = (long)d.nb * (d.nb + 1);
+        const double *Fx = c->Fx + e0 * sx, *Dv = c->D + e0 * sd, *Iv = c->dinv + e0 * sd;
+        double *X = c->X + e0 * sx, *Rw = c->rwork + e0 * sx;
+        rocblas_status st = cond_gemm_run(c->blas, cond_gemm(c, 0, Fx, Iv, nullptr, X, nullptr, n), c->gemm_sol[0]);
+        for (int sweep = 0; sweep < 2 && st == rocblas_status_success; sweep++) {
+            // R^T = [F Ru]^T - X^T D^T, then X^T += R^T inv32^T (the shape of the first GEMM)
+            if (hipMemcpyAsync(Rw, Fx, sizeof(double) * sx * n, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+                return hfem_fail(-2, "hfem_condense: hipMemcpyAsync failed");
+            CondGemm g = cond_gemm(c, 0, X, Dv, nullptr, Rw, nullptr, n);
+            g.alpha = &kMinusOne;
+            g.beta = &kOne;
+            st = cond_gemm_run(c->blas, g, 0);
+            if (st != rocblas_status_success) break;
+            // (rocBLAS's own solution: the tuned one was picked for beta = 0)
+            g = cond_gemm(c, 0, Rw, Iv, nullptr, X, nullptr, n);
+            g.beta = &kOne;
+            st = cond_gemm_run(c->blas, g, 0);
+        }
+        if (st == rocblas_status_success)
+            st = cond_gemm_run(c->blas,
+                               cond_gemm(c, 1, nullptr, nullptr, c->K + e0 * sk, X, c->Hx + e0 * sh, n),
+                               c->gemm_sol[1]);
+        if (st != rocblas_status_success)
+            return hfem_fail(-2, "hfem_condense: rocBLAS strided-batched GEMM failed: %s", rocblas_status_to_string(st));
+    }
     HIPCHK(hipEventRecord(c->ev[3], c->stream));
     return 0;
 }
@@ -563,7 +585,7 @@ static int store_quasi_newton(hfem_ctx *c, int precision)
     const Dims &d = c->d;
     const int ne = c->hd.ne;
     const long nD = (long)ne * d.na * d.na, nK = (long)ne * d.nb * d.na;
-    // the mixed condensation does not form inv(D) in FP64
+    // the mixed condensation leaves the FP32 inverse in dinv: the stored one is FP64
     if (precision != HFEM_FP64) HIPCHK(launch_dinv(d.na, 0, ne, c->D, c->dinv, c->info, c->stream));
     if (c->qn_acc == 1) {
         HIPCHK(hipMemcpyAsync(c->qn_dinv, c->dinv, nD * sizeof(double), hipMemcpyDeviceToDevice, c->stream));

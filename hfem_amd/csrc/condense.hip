@@ -1,23 +1,15 @@
 // Static condensation of element blocks (assembly/elementschur.cpp:6-329) on gfx950.
 //
-// FP64 path: inv(D) by Gauss-Jordan elimination with partial pivoting, one
-// 256-thread workgroup per element with the na x na matrix distributed
-// cyclically over a 16 x 16 thread grid in registers (pivot column and rows
-// exchanged through LDS, double-buffered so each elimination step needs two
-// barriers); then X = inv(D) [F | Ru] and [AH | FH] = [H | Rh] - K X as
-// strided-batched DGEMMs over all elements.
-// Mixed path (one workgroup per element, all stages fused):
-//   1. inv(D) by the same Gauss-Jordan in FP32;
-//   2. X = inv(D) [F | Ru]                  (FP32 MFMA 16x16x4, inv(D) from LDS)
-//      refined against the FP64 D (FP64 MFMA residuals);
-//   3. [AH | FH] = [H | Rh] - K X           (FP64 MFMA 16x16x4, in place)
+// FP64 path: inv(D) per element by blocked Gauss-Jordan on the FP64 MFMA
+// (dinv_mfma_kernel); then X = inv(D) [F | Ru] and [AH | FH] = [H | Rh] - K X as
+// strided-batched DGEMMs over all elements (host side, hfem_condense).
+// Mixed path: inv(D) by Gauss-Jordan in FP32 (dinv_kernel<R, float>, widened to
+// FP64 on the way out), X0 = inv32 [F | Ru], then refinement sweeps
+// X += inv32 ([F | Ru] - D X) with the residual in FP64, then the same FP64
+// Schur product, all products as batched DGEMMs.
 // Internal layouts: D column-major (na x na); Fx row-major (na x ldx) holding
 // F then Ru in column nb; X the same layout; K column-major (nb x na); Hx
 // column-major (nb x (nb+1)) holding H then Rh in column nb.
-//
-// The mixed path factors in FP32 (inv(D) rounded to FP32, FP32 MFMA products)
-// and refines X against the FP64 D with FP64 residuals; the Schur product stays
-// FP64 because AH must carry the FP64 accuracy.
 #include "hfem_device.h"
 
 #include <algorithm>
@@ -25,15 +17,10 @@
 namespace hfem {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ f64x4 mfma64(double a, double b, f64x4 c)
 {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
-__device__ __forceinline__ f32x4 mfma32(float a, float b, f32x4 c)
-{
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -119,130 +106,38 @@ __device__ __forceinline__ int gauss_jordan(int na, T (&a)[R][R], T *colbuf, T *
     return sing;
 }
 
-// 32 x 32 output tile per wave: C(i0.., j0..) = sum_k A(i,k) B(k,j), k < kdim,
-// with masked rows/columns.  Loader functors return 0 outside the matrix.
-// Operand layout of the 16x16x4 MFMAs: lane l supplies A(l%16, l/16) and
-// B(l/16, l%16).  Result layout (measured on gfx950): FP64 -- lane l, element v
-// holds C(l/16 + 4v, l%16); FP32 -- C(4(l/16) + v, l%16).
-template <typename LA, typename LB>
-__device__ __forceinline__ void mfma_tile64(int kdim, int i0, int j0, LA la, LB lb, f64x4 (&acc)[2][2])
-{
-    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
-#pragma unroll
-    for (int x = 0; x < 2; x++)
-#pragma unroll
-        for (int y = 0; y < 2; y++) acc[x][y] = (f64x4){0.0, 0.0, 0.0, 0.0};
-    double a0 = la(i0 + lr, lk), a1 = la(i0 + 16 + lr, lk);
-    double b0 = lb(lk, j0 + lr), b1 = lb(lk, j0 + 16 + lr);
-    for (int k0 = 0; k0 < kdim; k0 += 4) {
-        const int kn = k0 + 4 + lk;
-        double na0 = la(i0 + lr, kn), na1 = la(i0 + 16 + lr, kn);
-        double nb0 = lb(kn, j0 + lr), nb1 = lb(kn, j0 + 16 + lr);
-        acc[0][0] = mfma64(a0, b0, acc[0][0]);
-        acc[0][1] = mfma64(a0, b1, acc[0][1]);
-        acc[1][0] = mfma64(a1, b0, acc[1][0]);
-        acc[1][1] = mfma64(a1, b1, acc[1][1]);
-        a0 = na0; a1 = na1; b0 = nb0; b1 = nb1;
-    }
-}
-
-template <typename LA, typename LB>
-__device__ __forceinline__ void mfma_tile32(int kdim, int i0, int j0, LA la, LB lb, f32x4 (&acc)[2][2])
-{
-    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
-#pragma unroll
-    for (int x = 0; x < 2; x++)
-#pragma unroll
-        for (int y = 0; y < 2; y++) acc[x][y] = (f32x4){0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < kdim; k0 += 4) {
-        const int k = k0 + lk;
-        float a0 = la(i0 + lr, k), a1 = la(i0 + 16 + lr, k);
-        float b0 = lb(k, j0 + lr), b1 = lb(k, j0 + 16 + lr);
-        acc[0][0] = mfma32(a0, b0, acc[0][0]);
-        acc[0][1] = mfma32(a0, b1, acc[0][1]);
-        acc[1][0] = mfma32(a1, b0, acc[1][0]);
-        acc[1][1] = mfma32(a1, b1, acc[1][1]);
-    }
-}
-
-// LDS leading dimension of inv(D) (column-major): >= na, = 16 (mod 32) so that
-// the four 16-lane groups of an MFMA operand load fall in distinct banks.
-__host__ __device__ inline int dinv_ld(int na)
-{
-    int l = (na + 15) / 16 * 16;
-    return (l % 32 == 0) ? l + 16 : l;
-}
-
-// LDS bytes: the double-buffered pivot column and row (192 each), the pivot
-// order and row labels, and (optionally) inv(D).
-size_t condense_lds_bytes(int na, bool dinv_in_lds, int elt)
-{
-    size_t head = (size_t)6 * 192 * elt + 192 * sizeof(int);
-    head = (head + 15) / 16 * 16;
-    size_t m = dinv_in_lds ? (size_t)dinv_ld(na) * na * elt : 0;
-    return head + m;
-}
-
-__device__ __forceinline__ size_t lds_head(int elt)
+// LDS bytes of dinv_kernel: the double-buffered pivot column and row (192 each)
+// and the pivot order and row labels.
+size_t condense_lds_bytes(int elt)
 {
     size_t head = (size_t)6 * 192 * elt + 192 * sizeof(int);
     return (head + 15) / 16 * 16;
 }
 
-// Stage 3 (shared by both precisions): Hx = Hx - K X over all (nb+1) columns.
-__device__ __forceinline__ void schur_update(int na, int nb, int ldx, const double *__restrict__ K,
-                                             const double *__restrict__ X, double *__restrict__ H)
-{
-    const int wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-    const int lane = threadIdx.x & 63, lr = lane & 15, lg = lane >> 4;
-    const int nbx = nb + 1;
-    const int tm = (nb + 31) / 32, tn = (nbx + 31) / 32;
-    for (int tile = wave; tile < tm * tn; tile += nwave) {
-        const int i0 = (tile % tm) * 32, j0 = (tile / tm) * 32;
-        f64x4 acc[2][2];
-        mfma_tile64(
-            na, i0, j0,
-            [&](int i, int k) { return (i < nb && k < na) ? K[i + (long)nb * k] : 0.0; },
-            [&](int k, int j) { return (k < na && j < nbx) ? X[(long)k * ldx + j] : 0.0; }, acc);
-#pragma unroll
-        for (int x = 0; x < 2; x++)
-#pragma unroll
-            for (int y = 0; y < 2; y++) {
-                const int j = j0 + 16 * y + lr;
-                if (j >= nbx) continue;
-#pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    const int i = i0 + 16 * x + lg + 4 * v;
-                    if (i < nb) H[i + (long)nb * j] -= acc[x][y][v];
-                }
-            }
-    }
-}
-
 // inv(D) of every element of [e0, e0+n) by Gauss-Jordan with partial pivoting
-// (schurImplementation 0 semantics), written column-major (ld na) to Dinv; the
-// products X = inv(D)[F Ru] and [AH FH] = [H Rh] - K X then run as batched
-// GEMMs.  One workgroup per element.
-template <int R>
+// (schurImplementation 0 semantics) in arithmetic T (D rounded to T on the way
+// in), written column-major (ld na) to Dinv in FP64.  One workgroup per element.
+// The mixed condensation runs it in FP32.
+template <int R, typename T>
 __global__ void __launch_bounds__(256)
 dinv_kernel(int na, int e0, int n, const double *__restrict__ Dm, double *__restrict__ Dinv, int *__restrict__ info)
 {
     extern __shared__ double lds[];
-    double *colbuf = lds, *rowbuf = lds + 2 * 192;
-    int *pof = (int *)(lds + 4 * 192), *label = pof + 192;
+    T *colbuf = (T *)lds, *rowbuf = colbuf + 2 * 192;
+    int *pof = (int *)(colbuf + 4 * 192), *label = pof + 192;
     const int tid = threadIdx.x, ti = tid & 15, tj = tid >> 4;
     for (int t = blockIdx.x; t < n; t += gridDim.x) {
         const long e = e0 + t;
         const double *D = Dm + e * (long)na * na;
-        double a[R][R];
+        T a[R][R];
 #pragma unroll
         for (int c = 0; c < R; c++)
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 int i = ti + 16 * r, k = tj + 16 * c;
-                a[r][c] = (i < na && k < na) ? D[i + (long)na * k] : 0.0;
+                a[r][c] = (i < na && k < na) ? (T)D[i + (long)na * k] : (T)0;
             }
-        int sing = gauss_jordan<R, double>(na, a, colbuf, rowbuf, pof, label);
+        int sing = gauss_jordan<R, T>(na, a, colbuf, rowbuf, pof, label);
         if (tid == 0 && info) info[e] = sing;
         double *Mi = Dinv + e * (long)na * na;
 #pragma unroll
@@ -250,105 +145,8 @@ dinv_kernel(int na, int e0, int n, const double *__restrict__ Dm, double *__rest
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 int i = ti + 16 * r, m = tj + 16 * c;
-                if (i < na && m < na) Mi[label[i] + (long)na * pof[m]] = a[r][c];
+                if (i < na && m < na) Mi[label[i] + (long)na * pof[m]] = (double)a[r][c];
             }
-        __syncthreads();
-    }
-}
-
-// Mixed: inv(D) in FP32 (GJ in FP32), X0 = inv32 Fx (FP32 MFMA), then `sweeps`
-// refinements X += inv32 (Fx - D X) with the residual in FP64 (FP64 MFMA),
-// then the FP64 Schur product.
-template <int R>
-__global__ void __launch_bounds__(256)
-condense_mixed_kernel(int na, int nb, int ldx, int e0, int n, const double *__restrict__ Dm,
-                      const double *__restrict__ Fx, double *__restrict__ Xm, double *__restrict__ Rw,
-                      const double *__restrict__ Km, double *__restrict__ Hx, int *__restrict__ info, int sweeps)
-{
-    extern __shared__ double lds[];
-    float *fl = (float *)lds;
-    float *colbuf = fl, *rowbuf = fl + 2 * 192;
-    int *pof = (int *)(fl + 4 * 192), *label = pof + 192;
-    float *Ml = (float *)((char *)lds + lds_head(sizeof(float)));
-    const int ldl = dinv_ld(na);
-    const int tid = threadIdx.x, ti = tid & 15, tj = tid >> 4;
-    const int wave = tid >> 6, nwave = blockDim.x >> 6;
-    const int lane = tid & 63, lr = lane & 15, lg = lane >> 4;
-    double *Rl = Rw + (long)blockIdx.x * na * ldx;   // per-workgroup FP64 residual
-    for (int t = blockIdx.x; t < n; t += gridDim.x) {
-        const long e = e0 + t;
-        const double *D = Dm + e * (long)na * na;
-        float a[R][R];
-#pragma unroll
-        for (int c = 0; c < R; c++)
-#pragma unroll
-            for (int r = 0; r < R; r++) {
-                int i = ti + 16 * r, k = tj + 16 * c;
-                a[r][c] = (i < na && k < na) ? (float)D[i + (long)na * k] : 0.f;
-            }
-        int sing = gauss_jordan<R, float>(na, a, colbuf, rowbuf, pof, label);
-        if (tid == 0 && info) info[e] = sing;
-#pragma unroll
-        for (int c = 0; c < R; c++)
-#pragma unroll
-            for (int r = 0; r < R; r++) {
-                int i = ti + 16 * r, m = tj + 16 * c;
-                if (i < na && m < na) Ml[label[i] + ldl * pof[m]] = a[r][c];
-            }
-        __syncthreads();
-        const double *F = Fx + e * (long)na * ldx;
-        double *X = Xm + e * (long)na * ldx;
-        const int nbx = nb + 1, tm = (na + 31) / 32, tn = (nbx + 31) / 32;
-        for (int s = 0; s <= sweeps; s++) {
-            // right-hand side: F (s == 0) or the FP64 residual
-            const double *B = s == 0 ? F : Rl;
-            for (int tile = wave; tile < tm * tn; tile += nwave) {
-                const int i0 = (tile % tm) * 32, j0 = (tile / tm) * 32;
-                f32x4 acc[2][2];
-                mfma_tile32(
-                    na, i0, j0, [&](int i, int k) { return (i < na && k < na) ? Ml[i + ldl * k] : 0.f; },
-                    [&](int k, int j) { return (k < na && j < nbx) ? (float)B[(long)k * ldx + j] : 0.f; }, acc);
-#pragma unroll
-                for (int x = 0; x < 2; x++)
-#pragma unroll
-                    for (int y = 0; y < 2; y++) {
-                        const int j = j0 + 16 * y + lr;
-                        if (j >= nbx) continue;
-#pragma unroll
-                        for (int v = 0; v < 4; v++) {
-                            const int i = i0 + 16 * x + 4 * lg + v;
-                            if (i < na) {
-                                if (s == 0) X[(long)i * ldx + j] = (double)acc[x][y][v];
-                                else X[(long)i * ldx + j] += (double)acc[x][y][v];
-                            }
-                        }
-                    }
-            }
-            __syncthreads();
-            if (s == sweeps) break;
-            // FP64 residual R = F - D X
-            for (int tile = wave; tile < tm * tn; tile += nwave) {
-                const int i0 = (tile % tm) * 32, j0 = (tile / tm) * 32;
-                f64x4 acc[2][2];
-                mfma_tile64(
-                    na, i0, j0, [&](int i, int k) { return (i < na && k < na) ? D[i + (long)na * k] : 0.0; },
-                    [&](int k, int j) { return (k < na && j < nbx) ? X[(long)k * ldx + j] : 0.0; }, acc);
-#pragma unroll
-                for (int x = 0; x < 2; x++)
-#pragma unroll
-                    for (int y = 0; y < 2; y++) {
-                        const int j = j0 + 16 * y + lr;
-                        if (j >= nbx) continue;
-#pragma unroll
-                        for (int v = 0; v < 4; v++) {
-                            const int i = i0 + 16 * x + lg + 4 * v;
-                            if (i < na) Rl[(long)i * ldx + j] = F[(long)i * ldx + j] - acc[x][y][v];
-                        }
-                    }
-            }
-            __syncthreads();
-        }
-        schur_update(na, nb, ldx, Km + e * (long)nb * na, X, Hx + e * (long)nb * (nb + 1));
         __syncthreads();
     }
 }
@@ -591,8 +389,6 @@ dinv_mfma_kernel(int na, int e0, int n, const double *__restrict__ D, double *__
     }
 }
 
-int condense_dinv_ld(int na) { return dinv_ld(na); }
-
 int condense_rows_per_thread(int na) { return na <= 32 ? 2 : (na + 15) / 16; }
 
 #define HFEM_R_CASES(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11)
@@ -618,19 +414,20 @@ hipError_t launch_dinv(int na, int e0, int n, const double *D, double *Dinv, int
     return hipGetLastError();
 }
 
-hipError_t launch_condense_mixed(int na, int nb, int ldx, int e0, int n, const double *D, const double *Fx, double *X,
-                                 double *Rw, const double *K, double *Hx, int *info, int grid, int sweeps,
-                                 hipStream_t s)
+// inv(D) by the register-grid Gauss-Jordan in FP32 (fp32 != 0) or FP64.
+hipError_t launch_dinv_gj(int na, int e0, int n, const double *D, double *Dinv, int *info, int fp32, hipStream_t s)
 {
     const int R = condense_rows_per_thread(na);
-    const size_t lds = condense_lds_bytes(na, true, sizeof(float));
+    const size_t lds = condense_lds_bytes(fp32 ? (int)sizeof(float) : (int)sizeof(double));
+    const int grid = std::min(n, 32768);
     switch (R) {
-#define CASE(RR)                                                                                                    \
-    case RR:                                                                                                        \
-        hipFuncSetAttribute((const void *)condense_mixed_kernel<RR>, hipFuncAttributeMaxDynamicSharedMemorySize,   \
-                            (int)lds);                                                                              \
-        hipLaunchKernelGGL(condense_mixed_kernel<RR>, dim3(grid), dim3(256), lds, s, na, nb, ldx, e0, n, D, Fx, X,  \
-                           Rw, K, Hx, info, sweeps);                                                                \
+#define CASE(RR)                                                                                                  \
+    case RR:                                                                                                      \
+        if (fp32)                                                                                                 \
+            hipLaunchKernelGGL((dinv_kernel<RR, float>), dim3(grid), dim3(256), lds, s, na, e0, n, D, Dinv, info); \
+        else                                                                                                      \
+            hipLaunchKernelGGL((dinv_kernel<RR, double>), dim3(grid), dim3(256), lds, s, na, e0, n, D, Dinv,      \
+                               info);                                                                             \
         break;
         HFEM_R_CASES(CASE)
 #undef CASE

@@ -20,10 +20,8 @@ hipError_t launch_assemble(const Dims &, const Master &, const Phys &, const dou
                            hipStream_t);
 size_t flux_lds_bytes(const Dims &d);
 hipError_t launch_dinv(int na, int e0, int n, const double *D, double *Dinv, int *info, hipStream_t s);
-hipError_t launch_condense_mixed(int na, int nb, int ldx, int e0, int n, const double *D, const double *Fx, double *X,
-                                 double *Rw, const double *K, double *Hx, int *info, int grid, int sweeps,
-                                 hipStream_t s);
-size_t condense_lds_bytes(int na, bool dinv_in_lds, int elt);
+hipError_t launch_dinv_gj(int na, int e0, int n, const double *D, double *Dinv, int *info, int fp32, hipStream_t s);
+size_t condense_lds_bytes(int elt);
 hipError_t launch_schur_vector(int na, int nb, int ldx, int ne, int fp32, const void *Dinv, const void *K,
                                const double *Ru, const double *Rh, double *X, double *Hx, hipStream_t s);
 hipError_t launch_to_f32(long n, const double *x, float *y, hipStream_t s);
@@ -130,7 +128,10 @@ struct hfem_ctx {
     int *info = nullptr;
     int ldx = 0;
     // work
-    double *dinv = nullptr, *rwork = nullptr, *fluxbuf = nullptr;   // dinv: ne x na x na
+    double *dinv = nullptr, *fluxbuf = nullptr;   // dinv: ne x na x na
+    // mixed condensation: the FP64 residual [F Ru] - D X (ne x na x ldx), allocated
+    // by the first mixed condensation
+    double *rwork = nullptr;
     double *spill = nullptr;   // per-workgroup element tables of families beyond the LDS
     rocblas_handle blas = nullptr;
     // rocBLAS solution indices of the two condensation GEMMs (0: rocBLAS's own choice)
