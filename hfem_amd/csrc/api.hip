@@ -313,6 +313,7 @@ struct CondGemm {
     int lda, ldb, ldc;
     long sa, sb, sc;
     int batch;
+    double *Dout = nullptr;     // output, same layout as C (nullptr: in place over C)
 };
 
 static const double kOne = 1.0, kZero = 0.0, kMinusOne = -1.0;
@@ -335,7 +336,8 @@ static rocblas_status cond_gemm_run(rocblas_handle h, const CondGemm &g, int sol
 {
     const rocblas_datatype t = rocblas_datatype_f64_r;
     return rocblas_gemm_strided_batched_ex(h, g.ta, g.tb, g.m, g.n, g.k, g.alpha, g.A, t, g.lda, g.sa, g.B, t, g.ldb,
-                                           g.sb, g.beta, g.C, t, g.ldc, g.sc, g.C, t, g.ldc, g.sc, g.batch, t,
+                                           g.sb, g.beta, g.C, t, g.ldc, g.sc, g.Dout ? g.Dout : g.C, t, g.ldc, g.sc,
+                                           g.batch, t,
                                            sol ? rocblas_gemm_algo_solution_index : rocblas_gemm_algo_standard, sol,
                                            0);
 }
@@ -461,10 +463,10 @@ extern "C" int hfem_condense(hfem_ctx *c, int e0, int n, int precision)
         double *X = c->X + e0 * sx, *Rw = c->rwork + e0 * sx;
         rocblas_status st = cond_gemm_run(c->blas, cond_gemm(c, 0, Fx, Iv, nullptr, X, nullptr, n), c->gemm_sol[0]);
         for (int sweep = 0; sweep < 2 && st == rocblas_status_success; sweep++) {
-            // R^T = [F Ru]^T - X^T D^T, then X^T += R^T inv32^T (the shape of the first GEMM)
-            if (hipMemcpyAsync(Rw, Fx, sizeof(double) * sx * n, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
-                return hfem_fail(-2, "hfem_condense: hipMemcpyAsync failed");
-            CondGemm g = cond_gemm(c, 0, X, Dv, nullptr, Rw, nullptr, n);
+            // R^T = [F Ru]^T - X^T D^T (C = [F Ru]^T read, D = R^T written), then
+            // X^T += R^T inv32^T (the shape of the first GEMM)
+            CondGemm g = cond_gemm(c, 0, X, Dv, nullptr, const_cast<double *>(Fx), nullptr, n);
+            g.Dout = Rw;
             g.alpha = &kMinusOne;
             g.beta = &kOne;
             st = cond_gemm_run(c->blas, g, 0);
