@@ -348,19 +348,42 @@ static rocblas_status cond_gemm_solutions(rocblas_handle h, const CondGemm &g, s
 {
     const rocblas_datatype t = rocblas_datatype_f64_r;
     rocblas_int n = 0;
+    double *D = g.Dout ? g.Dout : g.C;
     rocblas_status st = rocblas_gemm_strided_batched_ex_get_solutions(
         h, g.ta, g.tb, g.m, g.n, g.k, g.alpha, g.A, t, g.lda, g.sa, g.B, t, g.ldb, g.sb, g.beta, g.C, t, g.ldc, g.sc,
-        g.C, t, g.ldc, g.sc, g.batch, t, rocblas_gemm_algo_solution_index, 0, nullptr, &n);
+        D, t, g.ldc, g.sc, g.batch, t, rocblas_gemm_algo_solution_index, 0, nullptr, &n);
     if (st != rocblas_status_success) return st;
     out.resize(n);
     if (n == 0) return st;
     return rocblas_gemm_strided_batched_ex_get_solutions(
         h, g.ta, g.tb, g.m, g.n, g.k, g.alpha, g.A, t, g.lda, g.sa, g.B, t, g.ldb, g.sb, g.beta, g.C, t, g.ldc, g.sc,
-        g.C, t, g.ldc, g.sc, g.batch, t, rocblas_gemm_algo_solution_index, 0, out.data(), &n);
+        D, t, g.ldc, g.sc, g.batch, t, rocblas_gemm_algo_solution_index, 0, out.data(), &n);
 }
 #pragma clang diagnostic pop
 
-// Picks, for each of the two condensation GEMMs, the fastest rocBLAS solution on
+// The batched GEMMs of the condensation, as tuned by hfem_tune_condensation:
+//   0: X = inv(D) [F Ru]                      (both paths)
+//   1: [AH FH] = [H Rh] - K X                 (both paths)
+//   2: R = [F Ru] - D X   (C read, R written)  (mixed refinement residual)
+//   3: X += inv32 R                           (mixed refinement update)
+static CondGemm tuned_gemm(const hfem_ctx *c, int which, const double *Fx, const double *Dm, const double *dinv,
+                           const double *K, double *X, double *Hx, double *R, int n)
+{
+    if (which == 0) return cond_gemm(c, 0, Fx, dinv, nullptr, X, nullptr, n);
+    if (which == 1) return cond_gemm(c, 1, nullptr, nullptr, K, X, Hx, n);
+    if (which == 2) {
+        CondGemm g = cond_gemm(c, 0, X, Dm, nullptr, const_cast<double *>(Fx), nullptr, n);
+        g.Dout = R;
+        g.alpha = &kMinusOne;
+        g.beta = &kOne;
+        return g;
+    }
+    CondGemm g = cond_gemm(c, 0, R, dinv, nullptr, X, nullptr, n);
+    g.beta = &kOne;
+    return g;
+}
+
+// Picks, for each of the condensation GEMMs, the fastest rocBLAS solution on
 // a sample of nsample elements (default min(ne, 32768): the winners change with
 // the batch size, so the default times the batch the condensation runs).  Inputs are the
 // context's Fx, inv(D) and K buffers (their values do not matter for timing);
@@ -375,18 +398,20 @@ extern "C" int hfem_tune_condensation(hfem_ctx *c, int nsample, int *solutions, 
     if (nsample < 0) return hfem_fail(-1, "hfem_tune_condensation: nsample=%d < 0", nsample);
     if (ne == 0) return 0;
     const int ns = nsample > 0 ? std::min(nsample, ne) : std::min(ne, 32768);
-    double *Xt = nullptr, *Ht = nullptr;
+    double *Xt = nullptr, *Ht = nullptr, *Rt = nullptr;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     int rc = 0;
     if (hfem_dalloc(&Xt, (size_t)ns * d.na * c->ldx) != hipSuccess ||
+        hfem_dalloc(&Rt, (size_t)ns * d.na * c->ldx) != hipSuccess ||
         hfem_dalloc(&Ht, (size_t)ns * d.nb * (d.nb + 1)) != hipSuccess || hipEventCreate(&t0) != hipSuccess ||
         hipEventCreate(&t1) != hipSuccess)
         rc = hfem_fail(-2, "hfem_tune_condensation: out of device memory");
     if (!rc && (hipMemsetAsync(Xt, 0, (size_t)ns * d.na * c->ldx * 8, c->stream) != hipSuccess ||
+                hipMemsetAsync(Rt, 0, (size_t)ns * d.na * c->ldx * 8, c->stream) != hipSuccess ||
                 hipMemsetAsync(Ht, 0, (size_t)ns * d.nb * (d.nb + 1) * 8, c->stream) != hipSuccess))
         rc = hfem_fail(-2, "hfem_tune_condensation: hipMemsetAsync failed");
-    for (int which = 0; which < 2 && !rc; which++) {
-        const CondGemm g = cond_gemm(c, which, c->Fx, c->dinv, c->K, Xt, Ht, ns);
+    for (int which = 0; which < 4 && !rc; which++) {
+        const CondGemm g = tuned_gemm(c, which, c->Fx, c->D, c->dinv, c->K, Xt, Ht, Rt, ns);
         // best of two timed runs after one untimed run; -1 if the solution fails
         auto time_it = [&](int sol) -> float {
             if (cond_gemm_run(c->blas, g, sol) != rocblas_status_success) return -1.f;
@@ -421,6 +446,7 @@ extern "C" int hfem_tune_condensation(hfem_ctx *c, int nsample, int *solutions, 
     if (t0) (void)hipEventDestroy(t0);
     if (t1) (void)hipEventDestroy(t1);
     if (Xt) (void)hipFree(Xt);
+    if (Rt) (void)hipFree(Rt);
     if (Ht) (void)hipFree(Ht);
     (void)hipGetLastError();
     return rc;
@@ -442,13 +468,12 @@ extern "C" int hfem_condense(hfem_ctx *c, int e0, int n, int precision)
         HIPCHK(launch_dinv(d.na, e0, n, c->D, c->dinv, c->info, c->stream));
         const long sx = (long)d.na * c->ldx, sd = (long)d.na * d.na, sk = (long)d.nb * d.na,
                    sh = (long)d.nb * (d.nb + 1);
-        rocblas_status st = cond_gemm_run(
-            c->blas, cond_gemm(c, 0, c->Fx + e0 * sx, c->dinv + e0 * sd, nullptr, c->X + e0 * sx, nullptr, n),
-            c->gemm_sol[0]);
-        if (st == rocblas_status_success)
+        rocblas_status st = rocblas_status_success;
+        for (int which = 0; which < 2 && st == rocblas_status_success; which++)
             st = cond_gemm_run(c->blas,
-                               cond_gemm(c, 1, nullptr, nullptr, c->K + e0 * sk, c->X + e0 * sx, c->Hx + e0 * sh, n),
-                               c->gemm_sol[1]);
+                               tuned_gemm(c, which, c->Fx + e0 * sx, nullptr, c->dinv + e0 * sd, c->K + e0 * sk,
+                                          c->X + e0 * sx, c->Hx + e0 * sh, nullptr, n),
+                               c->gemm_sol[which]);
         if (st != rocblas_status_success)
             return hfem_fail(-2, "hfem_condense: rocBLAS strided-batched GEMM failed: %s", rocblas_status_to_string(st));
     }
@@ -461,25 +486,14 @@ extern "C" int hfem_condense(hfem_ctx *c, int e0, int n, int precision)
                    sh = (long)d.nb * (d.nb + 1);
         const double *Fx = c->Fx + e0 * sx, *Dv = c->D + e0 * sd, *Iv = c->dinv + e0 * sd;
         double *X = c->X + e0 * sx, *Rw = c->rwork + e0 * sx;
-        rocblas_status st = cond_gemm_run(c->blas, cond_gemm(c, 0, Fx, Iv, nullptr, X, nullptr, n), c->gemm_sol[0]);
-        for (int sweep = 0; sweep < 2 && st == rocblas_status_success; sweep++) {
-            // R^T = [F Ru]^T - X^T D^T (C = [F Ru]^T read, D = R^T written), then
-            // X^T += R^T inv32^T (the shape of the first GEMM)
-            CondGemm g = cond_gemm(c, 0, X, Dv, nullptr, const_cast<double *>(Fx), nullptr, n);
-            g.Dout = Rw;
-            g.alpha = &kMinusOne;
-            g.beta = &kOne;
-            st = cond_gemm_run(c->blas, g, 0);
-            if (st != rocblas_status_success) break;
-            // (rocBLAS's own solution: the tuned one was picked for beta = 0)
-            g = cond_gemm(c, 0, Rw, Iv, nullptr, X, nullptr, n);
-            g.beta = &kOne;
-            st = cond_gemm_run(c->blas, g, 0);
-        }
-        if (st == rocblas_status_success)
-            st = cond_gemm_run(c->blas,
-                               cond_gemm(c, 1, nullptr, nullptr, c->K + e0 * sk, X, c->Hx + e0 * sh, n),
-                               c->gemm_sol[1]);
+        const double *Kv = c->K + e0 * sk;
+        double *Hv = c->Hx + e0 * sh;
+        // X0, then per sweep R = [F Ru] - D X (the residual GEMM reads [F Ru] as its C
+        // operand and writes R, so no copy) and X += inv32 R, then the Schur product
+        const int seq[6] = {0, 2, 3, 2, 3, 1};
+        rocblas_status st = rocblas_status_success;
+        for (int i = 0; i < 6 && st == rocblas_status_success; i++)
+            st = cond_gemm_run(c->blas, tuned_gemm(c, seq[i], Fx, Dv, Iv, Kv, X, Hv, Rw, n), c->gemm_sol[seq[i]]);
         if (st != rocblas_status_success)
             return hfem_fail(-2, "hfem_condense: rocBLAS strided-batched GEMM failed: %s", rocblas_status_to_string(st));
     }

@@ -134,8 +134,9 @@ struct hfem_ctx {
     double *rwork = nullptr;
     double *spill = nullptr;   // per-workgroup element tables of families beyond the LDS
     rocblas_handle blas = nullptr;
-    // rocBLAS solution indices of the two condensation GEMMs (0: rocBLAS's own choice)
-    int gemm_sol[2] = {0, 0};
+    // rocBLAS solution indices of the condensation GEMMs (0: rocBLAS's own choice):
+    // X = inv(D)[F Ru], the Schur product, the mixed residual and update
+    int gemm_sol[4] = {0, 0, 0, 0};
     long fstride = 0;
     int chunk = 0;
     int grid_asm = 0, grid_cond = 0;

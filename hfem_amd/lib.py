@@ -186,11 +186,12 @@ class HDGContext:
         _check(load().hfem_condense(self.h, e0, n, precision))
 
     def tune_condensation(self, nsample=0):
-        """Pick the rocBLAS solutions of the FP64 condensation's two batched
-        GEMMs; returns (solutions[2], ms[4]) with ms = (GEMM 1 default, picked,
-        GEMM 2 default, picked) on the sample."""
-        sol = (ctypes.c_int * 2)()
-        ms = (ctypes.c_float * 4)()
+        """Pick the rocBLAS solutions of the condensation's batched GEMMs
+        (inv(D)[F Ru], the Schur product, the mixed refinement residual and
+        update); returns (solutions[4], ms[8]) with ms = (default, picked) per
+        GEMM on the sample."""
+        sol = (ctypes.c_int * 4)()
+        ms = (ctypes.c_float * 8)()
         _check(load().hfem_tune_condensation(self.h, nsample, sol, ms))
         return list(sol), list(ms)
 

@@ -81,12 +81,13 @@ int hfem_assemble(hfem_ctx *ctx, int e0, int n);
  * inv(D)Ru, AH (in place of H) and FH; D, Ru and Rh keep their assembled values. */
 int hfem_condense(hfem_ctx *ctx, int e0, int n, int precision);
 /* No counterpart in the reference (it calls MKL's DGEMM per element): picks, for
- * each of the two batched GEMMs of the FP64 hfem_condense (inv(D)[F Ru] and
- * [H Rh] - K X), the fastest rocBLAS solution on a sample of nsample elements
- * (0: min(ne, 32768)) and keeps it in the context.  Element blocks are not
- * changed.  solutions[2] (may be NULL) receives the picks (0 = rocBLAS's own
- * choice), ms[4] (may be NULL) {GEMM 1 default, GEMM 1 picked, GEMM 2
- * default, GEMM 2 picked} in milliseconds on the sample. */
+ * each batched GEMM of hfem_condense -- 0: inv(D)[F Ru], 1: [H Rh] - K X (both
+ * precisions), 2: the mixed path's refinement residual [F Ru] - D X, 3: its
+ * update X += inv32 R -- the fastest rocBLAS solution on a sample of nsample
+ * elements (0: min(ne, 32768)) and keeps it in the context.  Element blocks are
+ * not changed.  solutions[4] (may be NULL) receives the picks (0 = rocBLAS's own
+ * choice), ms[8] (may be NULL) the default and the picked time in milliseconds
+ * on the sample for each GEMM in turn. */
 int hfem_tune_condensation(hfem_ctx *ctx, int nsample, int *solutions, float *ms);
 
 /* Copy the assembled element blocks of [e0, e0+n) to the host (any pointer may
