@@ -177,6 +177,32 @@ def test_tuned_condensation(oracle_mod):
     ctx.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_condensation_by_parts(precision):
+    """Condensing [0, k) and then [k, ne) gives the blocks of one condensation of
+    [0, ne): the element offsets of the batched steps (inverse, GEMMs,
+    refinement residual) are right.  rocBLAS may pick other kernels for the
+    smaller batches, so the comparison allows rounding (1e-13)."""
+    from hfem_amd.lib import HDGContext, FP64, MIXED
+    prec = FP64 if precision == "fp64" else MIXED
+    pr = make_problem("tet", 2, 3, seed=8)
+    say(f"condensation by parts tet2 p3 {precision}: ne={pr.ne}")
+    ctx = HDGContext(pr)
+    ctx.assemble_element_matrix_vector()
+    ctx.schur_element_matrix_vector(precision=prec)
+    whole = ctx.read_condensed()
+    k = pr.ne // 3
+    ctx.assemble_element_matrix_vector()
+    ctx.schur_element_matrix_vector(0, k, prec)
+    ctx.schur_element_matrix_vector(k, pr.ne - k, prec)
+    parts = ctx.read_condensed()
+    for name in ("DinvF", "DinvRu", "AH", "FH"):
+        e = relerr(parts[name], whole[name])
+        say(f"  {name}: {e:.2e}")
+        assert e < RTOL_MAT, name
+    ctx.close()
+
+
 def test_errors_are_reported():
     from hfem_amd.lib import HDGContext, HfemError
     pr = make_problem("tri", 1, 1, seed=0)
