@@ -481,7 +481,7 @@ extern "C" int hfem_condense(hfem_ctx *c, int e0, int n, int precision)
         // FP32 Gauss-Jordan inv32 (widened to FP64), X0 = inv32 [F Ru], then two sweeps
         // X += inv32 ([F Ru] - D X) with the residual in FP64, then [AH FH] = [H Rh] - K X
         if (!c->rwork) HIPCHK(hfem_dalloc(&c->rwork, (size_t)c->hd.ne * d.na * c->ldx));
-        HIPCHK(launch_dinv_gj(d.na, e0, n, c->D, c->dinv, c->info, 1, c->stream));
+        HIPCHK(launch_dinv_fp32(d.na, e0, n, c->D, c->dinv, c->info, c->stream));
         const long sx = (long)d.na * c->ldx, sd = (long)d.na * d.na, sk = (long)d.nb * d.na,
                    sh = (long)d.nb * (d.nb + 1);
         const double *Fx = c->Fx + e0 * sx, *Dv = c->D + e0 * sd, *Iv = c->dinv + e0 * sd;

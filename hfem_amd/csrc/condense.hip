@@ -117,7 +117,7 @@ size_t condense_lds_bytes(int elt)
 // inv(D) of every element of [e0, e0+n) by Gauss-Jordan with partial pivoting
 // (schurImplementation 0 semantics) in arithmetic T (D rounded to T on the way
 // in), written column-major (ld na) to Dinv in FP64.  One workgroup per element.
-// The mixed condensation runs it in FP32.
+// The mixed condensation runs it with T = float.
 template <int R, typename T>
 __global__ void __launch_bounds__(256)
 dinv_kernel(int na, int e0, int n, const double *__restrict__ Dm, double *__restrict__ Dinv, int *__restrict__ info)
@@ -414,20 +414,17 @@ hipError_t launch_dinv(int na, int e0, int n, const double *D, double *Dinv, int
     return hipGetLastError();
 }
 
-// inv(D) by the register-grid Gauss-Jordan in FP32 (fp32 != 0) or FP64.
-hipError_t launch_dinv_gj(int na, int e0, int n, const double *D, double *Dinv, int *info, int fp32, hipStream_t s)
+// inv(D) by the register-grid Gauss-Jordan in FP32, widened to FP64 (the mixed
+// condensation's factorisation).
+hipError_t launch_dinv_fp32(int na, int e0, int n, const double *D, double *Dinv, int *info, hipStream_t s)
 {
     const int R = condense_rows_per_thread(na);
-    const size_t lds = condense_lds_bytes(fp32 ? (int)sizeof(float) : (int)sizeof(double));
+    const size_t lds = condense_lds_bytes((int)sizeof(float));
     const int grid = std::min(n, 32768);
     switch (R) {
 #define CASE(RR)                                                                                                  \
     case RR:                                                                                                      \
-        if (fp32)                                                                                                 \
-            hipLaunchKernelGGL((dinv_kernel<RR, float>), dim3(grid), dim3(256), lds, s, na, e0, n, D, Dinv, info); \
-        else                                                                                                      \
-            hipLaunchKernelGGL((dinv_kernel<RR, double>), dim3(grid), dim3(256), lds, s, na, e0, n, D, Dinv,      \
-                               info);                                                                             \
+        hipLaunchKernelGGL((dinv_kernel<RR, float>), dim3(grid), dim3(256), lds, s, na, e0, n, D, Dinv, info);     \
         break;
         HFEM_R_CASES(CASE)
 #undef CASE

@@ -20,7 +20,7 @@ hipError_t launch_assemble(const Dims &, const Master &, const Phys &, const dou
                            hipStream_t);
 size_t flux_lds_bytes(const Dims &d);
 hipError_t launch_dinv(int na, int e0, int n, const double *D, double *Dinv, int *info, hipStream_t s);
-hipError_t launch_dinv_gj(int na, int e0, int n, const double *D, double *Dinv, int *info, int fp32, hipStream_t s);
+hipError_t launch_dinv_fp32(int na, int e0, int n, const double *D, double *Dinv, int *info, hipStream_t s);
 size_t condense_lds_bytes(int elt);
 hipError_t launch_schur_vector(int na, int nb, int ldx, int ne, int fp32, const void *Dinv, const void *K,
                                const double *Ru, const double *Rh, double *X, double *Hx, hipStream_t s);
