@@ -37,3 +37,12 @@ def test_error_path_without_device():
     bad = lib.hfem_dims(4, 6, 10, 10, 4, 4, 3, 1, 1)   # nd = 4 is rejected
     rc = L.hfem_create(ctypes.byref(h), ctypes.byref(bad), 0)
     assert rc < 0 and b"nd must be 2 or 3" in L.hfem_last_error()
+
+
+def test_tuner_rejects_a_null_context():
+    from hfem_amd import lib
+    L = lib.load()
+    sol = (ctypes.c_int * 4)()
+    ms = (ctypes.c_float * 8)()
+    rc = L.hfem_tune_condensation(None, 0, sol, ms)
+    assert rc < 0 and b"null context" in L.hfem_last_error()
