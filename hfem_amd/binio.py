@@ -1,4 +1,4 @@
-"""The reference's binary solution files, read and written with numpy.
+"""The reference's binary files, read (and the solution files written) with numpy.
 
 * ``sol.bin`` -- the initial state the MATLAB preprocessor writes for a serial
   run (preprocessing/digasopre.m:126-129 and :283-297) and the C++ reads
@@ -10,6 +10,7 @@
   two doubles ``[len, n]``, then per element the first ``npv*ncu`` values of its
   UDG block (the u part; the whole ``npv*nc`` block of every element when q is
   written, writeQflag 1), then the ``n`` values of UH.
+* a serial ``mesh.bin`` (read only), in the order readmeshstruct reads it.
 
 Array layouts are the reference's, which the library uses as they are: UDG
 element-major ``(ne, nc, npv)`` (MATLAB ``npv x nc x ne``), UH ``(nnodes, nch)``
@@ -101,3 +102,77 @@ def load_sol_bin(ctx, path, endian="<"):
     UH = s["UH"].reshape(prob.UH.shape)
     ctx.set_solution(UDG, UH, np.zeros_like(UDG))
     return UDG, UH
+
+
+MESH_HEADER = 30
+
+
+def read_mesh_bin(path, endian="<"):
+    """Read a serial ``mesh.bin`` in the order readmeshstruct reads it
+    (preprocessing/readbinaryfiles.cpp, serial branch): 30 dimensions, then
+    elementtype (ne), t (nve x ne), dgnodes (npe x ncd x ne), elcon
+    (npf*nfe x ne), bf, t2f, t2t (nfe x ne each), f ((nvf+3) x nf) and isEDGface
+    (nf), all doubles, indices 0-based.  Returns the dimensions by name and the
+    arrays in their MATLAB shapes (column-major: the last index is the element
+    or face); the file must hold exactly these, as the preprocessor's size
+    formula (digasopre.m:278) gives it."""
+    raw = np.fromfile(path, dtype=_dt(endian)).astype(np.float64)
+    if raw.size < MESH_HEADER:
+        raise ValueError(f"{path}: shorter than the {MESH_HEADER}-value header")
+    nd = raw[:MESH_HEADER]
+    if not np.all(nd == np.round(nd)):
+        raise ValueError(f"{path}: header is not a list of integers (wrong byte order?)")
+    nd = nd.astype(np.int64)
+    names = ["dim", "ncd", "nfe", "nve", "nvf", "ne", "nf", "nv", "ndh", "npe", "npf", "nc", "ncu", "ncq", "ncp",
+             "nch"]
+    d = {k: int(nd[i]) for i, k in enumerate(names)}
+    d["hybrid"], d["nproc"] = int(nd[-2]), int(nd[-1])
+    if d["nproc"] != 1:
+        raise ValueError(f"{path}: a file of {d['nproc']} ranks; only serial meshes are read")
+    ne, nf, nfe, nve, nvf, npe, npf, ncd = (d[k] for k in ("ne", "nf", "nfe", "nve", "nvf", "npe", "npf", "ncd"))
+    shapes = [("elementtype", (ne,)), ("t", (ne, nve)), ("dgnodes", (ne, ncd, npe)), ("elcon", (ne, npf * nfe)),
+              ("bf", (ne, nfe)), ("t2f", (ne, nfe)), ("t2t", (ne, nfe)), ("f", (nf, nvf + 3)), ("isEDGface", (nf,))]
+    need = MESH_HEADER + sum(int(np.prod(s)) for _, s in shapes)
+    if need != raw.size:
+        raise ValueError(f"{path}: the dimensions need {need} values, the file holds {raw.size}")
+    out = {"ndims": d}
+    o = MESH_HEADER
+    for name, shp in shapes:
+        n = int(np.prod(shp))
+        a = raw[o:o + n].reshape(shp)       # the element (face) index slowest, as MATLAB writes
+        o += n
+        out[name] = a if name == "dgnodes" else a.astype(np.int64)
+    return out
+
+
+def mesh_from_bin(m, nd, elemtype, porder):
+    """The library's Mesh from the arrays of ``read_mesh_bin`` (HDG, one element
+    type): dgnodes, elcon, bf and t2f as the reference uses them, the face ->
+    element map and the trace graph (faces sharing an element) derived from t2f."""
+    from .mesh import Mesh
+    d = m["ndims"]
+    ne, nfe, npf, nf = d["ne"], d["nfe"], d["npf"], d["nf"]
+    if d["hybrid"] != 0:
+        raise ValueError("only HDG meshes (hybrid 0) are read")
+    t2f = m["t2f"]
+    f2e = -np.ones((nf, 4), dtype=np.int64)
+    for ie in range(ne):
+        for a in range(nfe):
+            fid = t2f[ie, a]
+            if f2e[fid, 0] < 0:
+                f2e[fid, 0], f2e[fid, 1] = ie, a
+            else:
+                f2e[fid, 2], f2e[fid, 3] = ie, a
+    nbr = [set() for _ in range(nf)]
+    for ie in range(ne):
+        fs = set(int(v) for v in t2f[ie])
+        for fi in fs:
+            nbr[fi] |= fs
+    start, cols = [0], []
+    for fi in range(nf):
+        cols.extend([fi] + sorted(nbr[fi] - {fi}))
+        start.append(len(cols))
+    return Mesh(nd=nd, elemtype=elemtype, porder=porder, p=np.zeros((d["nv"], nd)), t=m["t"],
+                dgnodes=np.transpose(m["dgnodes"], (0, 2, 1)).copy(), t2f=t2f, bf=m["bf"],
+                elcon=m["elcon"].reshape(ne, nfe, npf), f2e=f2e, nf=nf,
+                ent2entStart=np.array(start, dtype=np.int64), ent2ent=np.array(cols, dtype=np.int64))

@@ -73,3 +73,47 @@ def test_solution_output_rejects_other_sizes(tmp_path, prob):
     write_solution_file(path, prob.UDG, prob.UH, prob.ncu)
     with pytest.raises(ValueError):
         read_solution_file(path, ne + 1, npv, nc, prob.ncu)
+
+
+def write_mesh_file(path, msh, npe, ncu, nch, nc):
+    """A serial mesh.bin in readmeshstruct's order (the preprocessor's writer,
+    preprocessing/digasopre.m:266-278), from one of the build's meshes; f and
+    isEDGface are written as zeros (the library does not read them)."""
+    ne, nve = msh.t.shape
+    nfe = msh.t2f.shape[1]
+    npf = msh.elcon.shape[2]
+    nvf = 2 if msh.nd == 2 else (3 if msh.elemtype == 0 else 4)
+    nf = msh.nf
+    head = np.zeros(30)
+    head[:16] = [msh.nd, msh.nd, nfe, nve, nvf, ne, nf, msh.p.shape[0], nf * npf, npe, npf, nc, ncu, nc - ncu, 0,
+                 nch]
+    head[28], head[29] = 0, 1                                   # HDG, one rank
+    t2t = -np.ones((ne, nfe))
+    for e1, a1, e2, a2 in msh.f2e:
+        if e2 >= 0:
+            t2t[e1, a1], t2t[e2, a2] = e2, e1
+    parts = [head, np.zeros(ne), msh.t.ravel(), np.transpose(msh.dgnodes, (0, 2, 1)).ravel(), msh.elcon.ravel(),
+             msh.bf.ravel(), msh.t2f.ravel(), t2t.ravel(), np.zeros(nf * (nvf + 3)), np.zeros(nf)]
+    np.concatenate([np.asarray(p, dtype=np.float64).ravel() for p in parts]).tofile(path)
+    # the preprocessor's size formula for the serial file
+    return 30 + (1 + nve + npe * msh.nd + npf * nfe + nfe + nfe + nfe) * ne + (nvf + 3) * nf + nf
+
+
+@pytest.mark.parametrize("kind,nc_,p", [("tri", 2, 2), ("tet", 1, 3), ("hex", 3, 2)])
+def test_mesh_bin_to_library_mesh(tmp_path, kind, nc_, p):
+    from hfem_amd.binio import mesh_from_bin, read_mesh_bin
+    pr = make_problem(kind, nc_, p, seed=2)
+    msh = pr.mesh
+    d = pr.dims
+    size = write_mesh_file(tmp_path / "mesh.bin", msh, d["npv"], d["ncu"], d["nch"], d["nc"])
+    assert np.fromfile(tmp_path / "mesh.bin", dtype="<f8").size == size
+    m = read_mesh_bin(tmp_path / "mesh.bin")
+    assert m["ndims"]["ne"] == pr.ne and m["ndims"]["npf"] == d["npf"]
+    back = mesh_from_bin(m, msh.nd, msh.elemtype, msh.porder)
+    assert np.array_equal(back.dgnodes, msh.dgnodes)
+    for k in ("elcon", "bf", "t2f", "f2e", "ent2entStart", "ent2ent", "t"):
+        assert np.array_equal(getattr(back, k), getattr(msh, k)), k
+    # a file whose dimensions do not account for its length is refused
+    np.fromfile(tmp_path / "mesh.bin", dtype="<f8")[:-1].tofile(tmp_path / "short.bin")
+    with pytest.raises(ValueError, match="need"):
+        read_mesh_bin(tmp_path / "short.bin")

@@ -25,3 +25,28 @@ def test_sol_bin_into_a_context_and_back(tmp_path):
     Uu, Hr, with_q = read_solution_file(tmp_path / "out.bin", ne, npv, nc, prob.ncu)
     assert not with_q and np.array_equal(Uu, UDG[:, :prob.ncu]) and np.array_equal(Hr, UH.ravel())
     ctx.close()
+
+
+def test_mesh_bin_assembles_like_the_original(tmp_path):
+    """A problem on the mesh read back from a serial mesh.bin assembles and
+    condenses to the same blocks, bit for bit."""
+    import dataclasses
+    import sys, os
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from test_binio import write_mesh_file
+    from hfem_amd.binio import mesh_from_bin, read_mesh_bin
+    from hfem_amd.lib import HDGContext, FP64
+    prob = make_problem("tet", 1, 3, seed=5)
+    d = prob.dims
+    write_mesh_file(tmp_path / "mesh.bin", prob.mesh, d["npv"], d["ncu"], d["nch"], d["nc"])
+    msh = mesh_from_bin(read_mesh_bin(tmp_path / "mesh.bin"), prob.mesh.nd, prob.mesh.elemtype, prob.mesh.porder)
+    prob2 = dataclasses.replace(prob, mesh=msh)
+    out = []
+    for pr in (prob, prob2):
+        ctx = HDGContext(pr)
+        ctx.assemble_element_matrix_vector()
+        ctx.schur_element_matrix_vector(precision=FP64)
+        out.append(ctx.read_condensed())
+        ctx.close()
+    for k in ("AH", "FH", "DinvF", "DinvRu"):
+        assert np.array_equal(out[0][k], out[1][k]), k
