@@ -10,7 +10,8 @@
   two doubles ``[len, n]``, then per element the first ``npv*ncu`` values of its
   UDG block (the u part; the whole ``npv*nc`` block of every element when q is
   written, writeQflag 1), then the ``n`` values of UH.
-* a serial ``mesh.bin`` (read only), in the order readmeshstruct reads it.
+* a serial ``mesh.bin`` and ``master.bin`` (read only), in the order
+  readmeshstruct and readmasterstruct read them.
 
 Array layouts are the reference's, which the library uses as they are: UDG
 element-major ``(ne, nc, npv)`` (MATLAB ``npv x nc x ne``), UH ``(nnodes, nch)``
@@ -176,3 +177,75 @@ def mesh_from_bin(m, nd, elemtype, porder):
                 dgnodes=np.transpose(m["dgnodes"], (0, 2, 1)).copy(), t2f=t2f, bf=m["bf"],
                 elcon=m["elcon"].reshape(ne, nfe, npf), f2e=f2e, nf=nf,
                 ent2entStart=np.array(start, dtype=np.int64), ent2ent=np.array(cols, dtype=np.int64))
+
+
+MASTER_HEADER = 20
+
+
+def read_master_bin(path, nelem=1, endian="<"):
+    """Read ``master.bin`` in the order readmasterstruct reads it
+    (preprocessing/readbinaryfiles.cpp): per element type 20 dimensions (dim,
+    elemtype, nodetype, npe, nme, nge, ngeR, nfe, nve, nle, nplmax, ...), the
+    per-face counts npf, nmf, ngf, ngfR, nvf (nfe each) and npl (nle), the
+    volume tables (plocvl, gpvl, gwvl, shapvt, shapvg, shapmv, shapvgdotshapvl
+    and their R variants, permnode, permedge), then per face plocfc, gpfc, gwfc,
+    shapft, shapfg, shapmf, shapfgdotshapfc, the R variants, perm, permgeom and
+    face.  Arrays come back flat, column-major as written; index arrays as ints.
+    Returns a list with one dict per element type."""
+    raw = np.fromfile(path, dtype=_dt(endian)).astype(np.float64)
+    o = 0
+    out = []
+
+    def take(n, integer=False):
+        nonlocal o
+        if o + n > raw.size:
+            raise ValueError(f"{path}: ends inside the master data")
+        a = raw[o:o + n]
+        o += n
+        return a.astype(np.int64) if integer else a.copy()
+
+    for _ in range(nelem):
+        nd = take(MASTER_HEADER)
+        if not np.all(nd == np.round(nd)):
+            raise ValueError(f"{path}: header is not a list of integers (wrong byte order?)")
+        dim, elemtype, nodetype, npe, nme, nge, ngeR, nfe, nve, nle, nplmax = (int(v) for v in nd[:11])
+        m = dict(dim=dim, elemtype=elemtype, nodetype=nodetype, npe=npe, nme=nme, nge=nge, ngeR=ngeR, nfe=nfe,
+                 nve=nve, nle=nle, nplmax=nplmax)
+        for k in ("npf", "nmf", "ngf", "ngfR", "nvf"):
+            m[k] = take(nfe, True)
+        m["npl"] = take(nle, True)
+        d1 = dim + 1
+        for k, n in (("plocvl", npe * dim), ("gpvl", nge * dim), ("gwvl", nge), ("shapvt", npe * nge * d1),
+                     ("shapvg", npe * nge * d1), ("shapmv", nme * nge * d1), ("shapvgdotshapvl", npe * npe * nge * d1),
+                     ("gpvlR", ngeR * dim), ("gwvlR", ngeR), ("shapvtR", npe * ngeR * d1), ("shapvgR", npe * ngeR * d1),
+                     ("shapmvR", nme * ngeR * d1)):
+            m[k] = take(n)
+        m["permnode"] = take(nve, True)
+        m["permedge"] = take(nle * nplmax, True)
+        faces = []
+        for j in range(nfe):
+            npf, nmf, ngf, ngfR, nvf = (int(m[k][j]) for k in ("npf", "nmf", "ngf", "ngfR", "nvf"))
+            f = {}
+            for k, n in (("plocfc", npf * (dim - 1)), ("gpfc", ngf * (dim - 1)), ("gwfc", ngf),
+                         ("shapft", npf * ngf * dim), ("shapfg", npf * ngf * dim), ("shapmf", nmf * ngf * dim),
+                         ("shapfgdotshapfc", npf * npf * ngf * dim), ("gpfcR", ngfR * (dim - 1)), ("gwfcR", ngfR),
+                         ("shapftR", npf * ngfR * dim), ("shapfgR", npf * ngfR * dim), ("shapmfR", nmf * ngfR * dim)):
+                f[k] = take(n)
+            f["perm"] = take(npf, True)
+            f["permgeom"] = take(nmf, True)
+            f["face"] = take(nvf, True)
+            faces.append(f)
+        m["faces"] = faces
+        out.append(m)
+    if o != raw.size:
+        raise ValueError(f"{path}: {raw.size - o} values left after {nelem} element type(s)")
+    return out
+
+
+def master_tables(m):
+    """The tables hfem_set_master takes, from one element type of
+    ``read_master_bin``: shapvt, shapvg, face 0's shapft and shapfg (the C++
+    element path uses face 0's tables for every face,
+    assembly/elementassembly.cpp:161,593) and perm (npf x nfe, column-major)."""
+    return dict(shapvt=m["shapvt"], shapvg=m["shapvg"], shapft=m["faces"][0]["shapft"],
+                shapfg=m["faces"][0]["shapfg"], perm=np.concatenate([f["perm"] for f in m["faces"]]))

@@ -132,19 +132,29 @@ def _i32(a):
 class HDGContext:
     """Device-resident HDG element path for one Problem (hfem_amd.problem)."""
 
-    def __init__(self, prob, device=0):
+    def __init__(self, prob, device=0, master_tables=None):
+        """``master_tables``: dict(shapvt, shapvg, shapft, shapfg, perm) of flat
+        column-major tables to use instead of the problem's master element (for
+        instance those of a reference master.bin, hfem_amd.binio.master_tables);
+        their sizes must be those of the problem's element."""
         L = load()
         self.prob = prob
         m = prob.master
         d = prob.dims
         self.d = d
         dims = hfem_dims(prob.nd, prob.ncu, m.npv, m.ngv, m.npf, m.ngf, m.nfe, prob.ne, prob.mesh.nf)
+        t = master_tables or dict(shapvt=m.flat("shapvt"), shapvg=m.flat("shapvg"), shapft=m.flat("shapft"),
+                                  shapfg=m.flat("shapfg"), perm=m.perm.reshape(-1, order="F"))
+        for k, n in (("shapvt", m.npv * m.ngv * (prob.nd + 1)), ("shapvg", m.npv * m.ngv * (prob.nd + 1)),
+                     ("shapft", m.npf * m.ngf * prob.nd), ("shapfg", m.npf * m.ngf * prob.nd),
+                     ("perm", m.npf * m.nfe)):
+            if np.asarray(t[k]).size != n:
+                raise HfemError(f"master table {k} has {np.asarray(t[k]).size} entries, the element needs {n}")
         h = _vp()
         _check(L.hfem_create(ctypes.byref(h), ctypes.byref(dims), device))
         self.h = h
-        _check(L.hfem_set_master(h, _d(_f64(m.flat("shapvt"))), _d(_f64(m.flat("shapvg"))),
-                                 _d(_f64(m.flat("shapft"))), _d(_f64(m.flat("shapfg"))),
-                                 _i(_i32(m.perm.reshape(-1, order="F")))))
+        _check(L.hfem_set_master(h, _d(_f64(t["shapvt"])), _d(_f64(t["shapvg"])), _d(_f64(t["shapft"])),
+                                 _d(_f64(t["shapfg"])), _i(_i32(t["perm"]))))
         self.set_physics(prob.fc_u, prob.fc_q, prob.tdep)
         msh = prob.mesh
         _check(L.hfem_set_mesh(h, _d(_f64(prob.dgnodes_ref())), _i(_i32(msh.elcon)), _i(_i32(msh.bf)),

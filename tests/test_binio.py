@@ -117,3 +117,43 @@ def test_mesh_bin_to_library_mesh(tmp_path, kind, nc_, p):
     np.fromfile(tmp_path / "mesh.bin", dtype="<f8")[:-1].tofile(tmp_path / "short.bin")
     with pytest.raises(ValueError, match="need"):
         read_mesh_bin(tmp_path / "short.bin")
+
+
+def write_master_file(path, m):
+    """One element type of master.bin in readmasterstruct's order (the
+    preprocessor's writemaster), from the build's master element; the tables the
+    library does not use (shapmv, the R variants' own rules, permnode, permedge)
+    are zeros or copies of the standard ones, of the sizes the reader expects."""
+    nd, npv, ngv, npf, ngf, nfe = m.nd, m.npv, m.ngv, m.npf, m.ngf, m.nfe
+    nvf = nd if m.elemtype == 0 else 2 ** (nd - 1)
+    nve = nd + 1 if m.elemtype == 0 else 2 ** nd
+    head = np.zeros(20)
+    head[:11] = [nd, m.elemtype, 0, npv, npv, ngv, ngv, nfe, nve, 0, 0]
+    F = lambda name: m.flat(name)
+    parts = [head, [npf] * nfe, [npf] * nfe, [ngf] * nfe, [ngf] * nfe, [nvf] * nfe,
+             F("plocvl"), F("gpvl"), F("gwvl"), F("shapvt"), F("shapvg"), np.zeros(npv * ngv * (nd + 1)),
+             F("shapvgdotshapvl"), F("gpvl"), F("gwvl"), F("shapvt"), F("shapvg"), np.zeros(npv * ngv * (nd + 1)),
+             np.zeros(nve)]
+    for j in range(nfe):
+        parts += [np.zeros(npf * (nd - 1)), F("gpfc"), F("gwfc"), F("shapft"), F("shapfg"), np.zeros(npf * ngf * nd),
+                  F("shapfgdotshapfc"), F("gpfc"), F("gwfc"), F("shapft"), F("shapfg"), np.zeros(npf * ngf * nd),
+                  m.perm[:, j], m.perm[:, j], np.arange(nvf)]
+    np.concatenate([np.asarray(p, dtype=np.float64).ravel() for p in parts]).tofile(path)
+
+
+@pytest.mark.parametrize("kind,nc_,p", [("tri", 2, 3), ("tet", 1, 2), ("hex", 3, 2)])
+def test_master_bin_tables(tmp_path, kind, nc_, p):
+    from hfem_amd.binio import master_tables, read_master_bin
+    m = make_problem(kind, nc_, p, seed=0).master
+    write_master_file(tmp_path / "master.bin", m)
+    (mm,) = read_master_bin(tmp_path / "master.bin")
+    assert (mm["npe"], mm["nge"], mm["nfe"], int(mm["npf"][0]), int(mm["ngf"][0])) == (m.npv, m.ngv, m.nfe, m.npf,
+                                                                                          m.ngf)
+    t = master_tables(mm)
+    for k in ("shapvt", "shapvg", "shapft", "shapfg"):
+        assert np.array_equal(t[k], m.flat(k)), k
+    assert np.array_equal(t["perm"], m.perm.reshape(-1, order="F"))
+    raw = np.fromfile(tmp_path / "master.bin", dtype="<f8")
+    raw[:-1].tofile(tmp_path / "short.bin")
+    with pytest.raises(ValueError):
+        read_master_bin(tmp_path / "short.bin")

@@ -50,3 +50,25 @@ def test_mesh_bin_assembles_like_the_original(tmp_path):
         ctx.close()
     for k in ("AH", "FH", "DinvF", "DinvRu"):
         assert np.array_equal(out[0][k], out[1][k]), k
+
+
+def test_master_bin_tables_drive_the_library(tmp_path):
+    """A context given the tables read from a master.bin condenses to the same
+    blocks as one built on the problem's own master element, bit for bit."""
+    import sys, os
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from test_binio import write_master_file
+    from hfem_amd.binio import master_tables, read_master_bin
+    from hfem_amd.lib import HDGContext, FP64
+    prob = make_problem("tri", 2, 3, seed=6)
+    write_master_file(tmp_path / "master.bin", prob.master)
+    tables = master_tables(read_master_bin(tmp_path / "master.bin")[0])
+    out = []
+    for mt in (None, tables):
+        ctx = HDGContext(prob, master_tables=mt)
+        ctx.assemble_element_matrix_vector()
+        ctx.schur_element_matrix_vector(precision=FP64)
+        out.append(ctx.read_condensed())
+        ctx.close()
+    for k in ("AH", "FH", "DinvF", "DinvRu"):
+        assert np.array_equal(out[0][k], out[1][k]), k
