@@ -10,8 +10,8 @@
   two doubles ``[len, n]``, then per element the first ``npv*ncu`` values of its
   UDG block (the u part; the whole ``npv*nc`` block of every element when q is
   written, writeQflag 1), then the ``n`` values of UH.
-* a serial ``mesh.bin`` and ``master.bin`` (read only), in the order
-  readmeshstruct and readmasterstruct read them.
+* a serial ``mesh.bin``, ``master.bin`` and ``app.bin`` (read only), in the
+  order readmeshstruct, readmasterstruct and readappstruct read them.
 
 Array layouts are the reference's, which the library uses as they are: UDG
 element-major ``(ne, nc, npv)`` (MATLAB ``npv x nc x ne``), UH ``(nnodes, nch)``
@@ -249,3 +249,66 @@ def master_tables(m):
     assembly/elementassembly.cpp:161,593) and perm (npf x nfe, column-major)."""
     return dict(shapvt=m["shapvt"], shapvg=m["shapvg"], shapft=m["faces"][0]["shapft"],
                 shapfg=m["faces"][0]["shapfg"], perm=np.concatenate([f["perm"] for f in m["faces"]]))
+
+
+APP_HEADER = 40
+_APP_ARRAYS = [("meshsize", True), ("solsize", True), ("porder", True), ("elemtype", True), ("nodetype", True),
+               ("pgauss", True), ("pgaussR", True), ("quadtype", True), ("bcm", True), ("bcs", False), ("bcd", True),
+               ("bcv", False), ("dt", False), ("flag", True), ("factor", False), ("problem", True),
+               ("physicsparam", False), ("solversparam", False)]
+
+
+def read_app_bin(path, endian="<"):
+    """Read ``app.bin`` in the order readappstruct reads it
+    (preprocessing/readbinaryfiles.cpp): 40 dimensions (the lengths of the 18
+    arrays, then nproc, nfile, nd, ncd, nc, ncu, nch, ncq, ncp, nco), then
+    meshsize, solsize, porder, elemtype, nodetype, pgauss, pgaussR, quadtype,
+    bcm, bcs, bcd, bcv, dt, flag, factor, problem, physicsparam, solversparam."""
+    raw = np.fromfile(path, dtype=_dt(endian)).astype(np.float64)
+    if raw.size < APP_HEADER:
+        raise ValueError(f"{path}: shorter than the {APP_HEADER}-value header")
+    nd = raw[:APP_HEADER]
+    if not np.all(nd == np.round(nd)) or np.any(nd < 0):
+        raise ValueError(f"{path}: header is not a list of counts (wrong byte order?)")
+    nd = nd.astype(np.int64)
+    need = APP_HEADER + int(nd[:18].sum())
+    if need != raw.size:
+        raise ValueError(f"{path}: the dimensions need {need} values, the file holds {raw.size}")
+    app = {"ndims": nd}
+    o = APP_HEADER
+    for i, (name, integer) in enumerate(_APP_ARRAYS):
+        a = raw[o:o + nd[i]]
+        o += int(nd[i])
+        app[name] = a.astype(np.int64) if integer else a.copy()
+    for i, k in enumerate(("nproc", "nfile", "nd", "ncd", "nc", "ncu", "nch", "ncq", "ncp", "nco")):
+        app[k] = int(nd[18 + i])
+    return app
+
+
+def app_settings(app):
+    """The reference's meaning of the app arrays (initializestructs.cpp:91-143,
+    :185-201) as the library takes them: the physics (physicsparam, fc_u =
+    factor[0], fc_q = factor[1], tdep = flag[0]), DIRK stages and order
+    (problem[4], problem[3]), the time steps, and the solver parameters
+    (hfem_solver_params fields): Newton tolerance solversparam[0], GMRES
+    tolerance solversparam[1], Newton iterations problem[13], GMRES(200) with at
+    most 1000 iterations as the C++ fixes them, preconditioner flag[16] (0 RAS and
+    1 subdomain block Jacobi are BILU0 on one rank, 2 entity-wise block Jacobi) or
+    none when flag[17] is -1, orthogonalisation flag[19], ordering flag[20] (MDF
+    when nonzero), ordering reuse flag[10], quasi-Newton flag[15] with storage
+    accuracy flag[18]."""
+    flag, fac, prob, sp = app["flag"], app["factor"], app["problem"], app["solversparam"]
+
+    def at(a, i, default):
+        return a[i] if len(a) > i else default
+    prec = int(at(flag, 16, 0))
+    params = dict(newton_tol=float(at(sp, 0, 1e-7)), gmres_tol=float(at(sp, 1, 1e-6)),
+                  newton_maxiter=int(at(prob, 13, 10)), true_newton_maxiter=int(at(prob, 13, 10)),
+                  restart=200, gmres_maxit=1000,
+                  precond=-1 if int(at(flag, 17, 0)) == -1 else (2 if prec == 2 else 0),
+                  orthog=int(at(flag, 19, 1)), ordering=1 if int(at(flag, 20, 1)) != 0 else 0,
+                  reuse_ordering=int(at(flag, 10, 1)), quasi_newton=int(at(flag, 15, 0)),
+                  qn_accuracy=int(at(flag, 18, 1)))
+    return dict(physicsparam=app["physicsparam"], fc_u=float(at(fac, 0, 0.0)), fc_q=float(at(fac, 1, 1.0)),
+                tdep=int(at(flag, 0, 0)), dirk_order=int(at(prob, 3, 0)), dirk_stages=int(at(prob, 4, 0)),
+                dt=app["dt"], solver_params=params)

@@ -157,3 +157,35 @@ def test_master_bin_tables(tmp_path, kind, nc_, p):
     raw[:-1].tofile(tmp_path / "short.bin")
     with pytest.raises(ValueError):
         read_master_bin(tmp_path / "short.bin")
+
+
+def test_app_bin_and_its_settings(tmp_path):
+    """app.bin in readappstruct's order (as writeappstruct writes it), and the
+    reference's meaning of its flags (initializestructs.cpp) as solver
+    parameters the library takes."""
+    from hfem_amd.binio import app_settings, read_app_bin
+    from hfem_amd.lib import hfem_solver_params
+    flag = np.zeros(24)
+    flag[0], flag[10], flag[15], flag[16], flag[17], flag[18], flag[19], flag[20] = 1, 1, 1, 2, 0, 0, 0, 1
+    problem = np.zeros(16)
+    problem[3], problem[4], problem[13] = 3, 3, 7
+    arrays = [[0, 123], [0, 45], [3], [0], [0], [6], [6], [0], [1, 2], [0.5, 0.25], [1], [1.0, 0.0, 0.0, 0.0, 2.5],
+              [0.01, 0.02], flag, [4.0, 1.0, 0.0], problem, [1.4, 0.0, 1000.0, 0.72, 0.3, 2.0], [1e-9, 1e-11]]
+    head = np.zeros(40)
+    head[:18] = [len(a) for a in arrays]
+    head[18:28] = [1, 1, 3, 3, 20, 5, 5, 15, 0, 0]
+    np.concatenate([head] + [np.asarray(a, dtype=np.float64) for a in arrays]).tofile(tmp_path / "app.bin")
+    app = read_app_bin(tmp_path / "app.bin")
+    assert app["nd"] == 3 and app["ncu"] == 5 and list(app["porder"]) == [3] and list(app["bcm"]) == [1, 2]
+    assert np.array_equal(app["physicsparam"], [1.4, 0.0, 1000.0, 0.72, 0.3, 2.0])
+    st = app_settings(app)
+    assert st["fc_u"] == 4.0 and st["fc_q"] == 1.0 and st["tdep"] == 1
+    assert st["dirk_order"] == 3 and st["dirk_stages"] == 3 and np.array_equal(st["dt"], [0.01, 0.02])
+    p = st["solver_params"]
+    assert (p["newton_tol"], p["gmres_tol"], p["newton_maxiter"], p["precond"], p["orthog"], p["ordering"]) == \
+        (1e-9, 1e-11, 7, 2, 0, 1)
+    assert (p["quasi_newton"], p["qn_accuracy"], p["reuse_ordering"]) == (1, 0, 1)
+    hfem_solver_params.make(**p)                   # the library's parameter block takes them as they are
+    np.fromfile(tmp_path / "app.bin", dtype="<f8")[:-1].tofile(tmp_path / "short.bin")
+    with pytest.raises(ValueError, match="need"):
+        read_app_bin(tmp_path / "short.bin")
