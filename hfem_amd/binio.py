@@ -312,3 +312,37 @@ def app_settings(app):
     return dict(physicsparam=app["physicsparam"], fc_u=float(at(fac, 0, 0.0)), fc_q=float(at(fac, 1, 1.0)),
                 tdep=int(at(flag, 0, 0)), dirk_order=int(at(prob, 3, 0)), dirk_stages=int(at(prob, 4, 0)),
                 dt=app["dt"], solver_params=params)
+
+
+def problem_from_case(prefix, endian="<"):
+    """A problem from a reference case: the files ``prefix + 'app.bin'``,
+    ``'master.bin'``, ``'mesh.bin'`` and ``'sol.bin'`` the preprocessor writes
+    for a serial run (digasopre.m).  Returns (problem, master tables, solver
+    parameters): pass the tables as ``HDGContext(problem, master_tables=...)``
+    and the parameters to ``solve_steady`` / ``dirk_step``.  The library's
+    boundary flux is the far-field one with one state for every boundary face,
+    taken from the first boundary's entries of app.bcs (``bcs[ib*nch + k]``,
+    assembly/elementassembly.cpp:693); a mesh with more than one boundary is
+    refused."""
+    from .problem import Problem
+    app = read_app_bin(prefix + "app.bin", endian)
+    nd, ncu, nch, nc = app["nd"], app["ncu"], app["nch"], app["nc"]
+    if len(app["elemtype"]) != 1:
+        raise ValueError("meshes of one element type only")
+    elemtype, porder = int(app["elemtype"][0]), int(app["porder"][0])
+    msh = mesh_from_bin(read_mesh_bin(prefix + "mesh.bin", endian), nd, elemtype, porder)
+    ids = set(int(v) for v in msh.bf[msh.bf < 0].ravel())
+    if ids - {-1}:
+        raise ValueError(f"boundary ids {sorted(ids)}: the library has one far-field boundary")
+    masters = read_master_bin(prefix + "master.bin", 1, endian)
+    tables = master_tables(masters[0])
+    s = read_sol_bin(prefix + "sol.bin", endian)
+    ne, npv = msh.ne, masters[0]["npe"]
+    npf = msh.elcon.shape[2]
+    UDG = s["UDG"].reshape(ne, nc, npv)
+    UH = s["UH"].reshape(msh.nf * npf, nch)
+    st = app_settings(app)
+    prob = Problem(name=prefix, mesh=msh, nd=nd, ncu=ncu, nch=nch, nc=nc, param=np.asarray(st["physicsparam"]),
+                   ui=np.asarray(app["bcs"][:nch]), UDG=UDG, UH=UH, SH=np.zeros_like(UDG), fc_u=st["fc_u"],
+                   fc_q=st["fc_q"], tdep=st["tdep"])
+    return prob, tables, st["solver_params"]

@@ -189,3 +189,37 @@ def test_app_bin_and_its_settings(tmp_path):
     np.fromfile(tmp_path / "app.bin", dtype="<f8")[:-1].tofile(tmp_path / "short.bin")
     with pytest.raises(ValueError, match="need"):
         read_app_bin(tmp_path / "short.bin")
+
+
+def write_case(prefix, pr, params):
+    """The four files of a serial reference case for one of the build's problems."""
+    from hfem_amd.binio import write_sol_bin
+    d = pr.dims
+    write_mesh_file(prefix + "mesh.bin", pr.mesh, d["npv"], d["ncu"], d["nch"], d["nc"])
+    write_master_file(prefix + "master.bin", pr.master)
+    write_sol_bin(prefix + "sol.bin", pr.UDG, pr.UH)
+    flag = np.zeros(24)
+    flag[0], flag[10], flag[19], flag[20] = pr.tdep, 1, 1, 1
+    problem = np.zeros(16)
+    problem[13] = params["newton_maxiter"]
+    arrays = [[0], [0], [pr.mesh.porder], [pr.mesh.elemtype], [0], [2 * pr.mesh.porder], [2 * pr.mesh.porder], [0],
+              [1], list(pr.ui), [1], [0.0], [0.0], flag, [pr.fc_u, pr.fc_q, 0.0], problem, list(pr.param),
+              [params["newton_tol"], params["gmres_tol"]]]
+    head = np.zeros(40)
+    head[:18] = [len(a) for a in arrays]
+    head[18:28] = [1, 1, pr.nd, pr.nd, pr.nc, pr.ncu, pr.nch, pr.nc - pr.ncu, 0, 0]
+    np.concatenate([head] + [np.asarray(a, dtype=np.float64) for a in arrays]).tofile(prefix + "app.bin")
+
+
+def test_problem_from_case(tmp_path):
+    from hfem_amd.binio import problem_from_case
+    pr = make_problem("tet", 1, 2, seed=3)
+    write_case(str(tmp_path / "c_"), pr, dict(newton_maxiter=9, newton_tol=1e-8, gmres_tol=1e-9))
+    p2, tables, params = problem_from_case(str(tmp_path / "c_"))
+    assert np.array_equal(p2.UDG, pr.UDG) and np.array_equal(p2.UH, pr.UH)
+    assert np.array_equal(p2.param, pr.param) and np.array_equal(p2.ui, pr.ui)
+    assert (p2.fc_u, p2.fc_q, p2.tdep) == (pr.fc_u, pr.fc_q, pr.tdep)
+    assert np.array_equal(p2.mesh.elcon, pr.mesh.elcon) and np.array_equal(p2.mesh.dgnodes, pr.mesh.dgnodes)
+    assert p2.dims == pr.dims
+    assert np.array_equal(tables["shapvt"], pr.master.flat("shapvt"))
+    assert (params["newton_maxiter"], params["newton_tol"], params["gmres_tol"]) == (9, 1e-8, 1e-9)

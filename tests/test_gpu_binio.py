@@ -72,3 +72,32 @@ def test_master_bin_tables_drive_the_library(tmp_path):
         ctx.close()
     for k in ("AH", "FH", "DinvF", "DinvRu"):
         assert np.array_equal(out[0][k], out[1][k]), k
+
+
+def test_reference_case_runs_like_the_original(tmp_path):
+    """A problem read back from the four files of a reference case condenses and
+    takes Newton steps exactly as the problem the files were written from, with
+    the solver parameters the app file carries."""
+    import sys, os
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from test_binio import write_case
+    from hfem_amd.binio import problem_from_case
+    from hfem_amd.lib import HDGContext, FP64
+    pr = make_problem("tri", 3, 2, seed=5, angle=0.3)
+    write_case(str(tmp_path / "c_"), pr, dict(newton_maxiter=4, newton_tol=1e-10, gmres_tol=1e-12))
+    p2, tables, params = problem_from_case(str(tmp_path / "c_"))
+    res = []
+    for p, mt in ((pr, None), (p2, tables)):
+        ctx = HDGContext(p, master_tables=mt)
+        ctx.assemble_element_matrix_vector()
+        ctx.schur_element_matrix_vector(precision=FP64)
+        c = ctx.read_condensed()
+        conv, nit, hist = ctx.solve_steady(**params)
+        U, UH = ctx.get_solution()
+        res.append((c, nit, hist, U, UH))
+        ctx.close()
+    (c0, n0, h0, U0, H0), (c1, n1, h1, U1, H1) = res
+    for k in ("AH", "FH"):
+        assert np.array_equal(c0[k], c1[k]), k
+    assert n0 == n1 and np.array_equal(h0[:, :3], h1[:, :3])
+    assert np.array_equal(U0, U1) and np.array_equal(H0, H1)
