@@ -49,6 +49,10 @@ def read_sol_bin(path, endian="<"):
     if not np.all(nd == np.round(nd)) or np.any(nd < 0):
         raise ValueError(f"{path}: header {nd[:3]} is not a list of counts (wrong byte order?)")
     nudg, nuh, npdg = (int(v) for v in nd[:3])
+    if SOL_HEADER + nudg + nuh == raw.size and npdg > 0:
+        # the preprocessor writes numel(PDG) into the header but the PDG values only
+        # for the wave formulation (digasopre.m:129, :292-297): no payload, no PDG
+        npdg = 0
     if SOL_HEADER + nudg + nuh + npdg != raw.size:
         raise ValueError(f"{path}: header counts {nudg}+{nuh}+{npdg} do not match {raw.size - SOL_HEADER} values")
     o = SOL_HEADER
@@ -155,6 +159,8 @@ def mesh_from_bin(m, nd, elemtype, porder):
     ne, nfe, npf, nf = d["ne"], d["nfe"], d["npf"], d["nf"]
     if d["hybrid"] != 0:
         raise ValueError("only HDG meshes (hybrid 0) are read")
+    if d["ncd"] < nd:
+        raise ValueError(f"mesh.bin has ncd={d['ncd']} coordinate components, the geometry needs nd={nd}")
     t2f = m["t2f"]
     f2e = -np.ones((nf, 4), dtype=np.int64)
     for ie in range(ne):
@@ -174,7 +180,7 @@ def mesh_from_bin(m, nd, elemtype, porder):
         cols.extend([fi] + sorted(nbr[fi] - {fi}))
         start.append(len(cols))
     return Mesh(nd=nd, elemtype=elemtype, porder=porder, p=np.zeros((d["nv"], nd)), t=m["t"],
-                dgnodes=np.transpose(m["dgnodes"], (0, 2, 1)).copy(), t2f=t2f, bf=m["bf"],
+                dgnodes=np.transpose(m["dgnodes"][:, :nd, :], (0, 2, 1)).copy(), t2f=t2f, bf=m["bf"],
                 elcon=m["elcon"].reshape(ne, nfe, npf), f2e=f2e, nf=nf,
                 ent2entStart=np.array(start, dtype=np.int64), ent2ent=np.array(cols, dtype=np.int64))
 
@@ -314,6 +320,41 @@ def app_settings(app):
                 dt=app["dt"], solver_params=params)
 
 
+# app.bin settings the library implements, as (array, index, required value,
+# meaning): anything else selects physics or a scheme the element path does not
+# have (the reference dispatches on them: boundary type bn = bcm[ib],
+# assembly/elementassembly.cpp:1102-1104, application/FM/fbou.cpp types 1-11;
+# stabilisation, application/FM/stabilizationTensor.cpp:3494-3528; the
+# appname, viscosity, SGS, ALE and AV switches, preprocessing/initializestructs.cpp:91-143)
+SUPPORTED_APP = [("problem", 0, 0, "hybrid (0 = HDG)"), ("problem", 1, 1, "appname (1 = Navier-Stokes)"),
+                 ("problem", 5, 0, "convStabMethod (0 = constant tau)"),
+                 ("problem", 6, 0, "diffStabMethod (0 = constant tau)"),
+                 ("problem", 7, 0, "rotatingFrame"), ("problem", 8, 0, "viscosityModel (0 = constant)"),
+                 ("problem", 9, 0, "SGSmodel"), ("problem", 10, 0, "ALE"), ("problem", 11, 0, "AV"),
+                 ("flag", 1, 0, "wave")]
+
+
+def check_supported(app, has_boundary=True):
+    """Raise ValueError unless the app settings are those the library implements:
+    HDG, Navier-Stokes, constant tau, constant viscosity, no SGS/ALE/AV, no wave
+    formulation, at least the six physics parameters {gamma, eps, Re, Pr, Minf,
+    tau} and, on a mesh with boundary faces, a far-field first boundary (bcm[0]
+    = 1) with a state of nch values in bcs."""
+    for arr, i, want, what in SUPPORTED_APP:
+        a = app[arr]
+        have = int(a[i]) if len(a) > i else 0
+        if have != want:
+            raise ValueError(f"app.bin {arr}[{i}] = {have}: {what} must be {want} for this library")
+    if len(app["physicsparam"]) < 6:
+        raise ValueError(f"app.bin physicsparam has {len(app['physicsparam'])} values, the NS flux needs 6")
+    if has_boundary:
+        bcm = app["bcm"]
+        if len(bcm) < 1 or int(bcm[0]) != 1:
+            raise ValueError(f"app.bin bcm = {list(bcm)}: boundary 0 must be far field (type 1) for this library")
+        if len(app["bcs"]) < app["nch"]:
+            raise ValueError(f"app.bin bcs has {len(app['bcs'])} values, the far-field state needs nch={app['nch']}")
+
+
 def problem_from_case(prefix, endian="<"):
     """A problem from a reference case: the files ``prefix + 'app.bin'``,
     ``'master.bin'``, ``'mesh.bin'`` and ``'sol.bin'`` the preprocessor writes
@@ -322,10 +363,12 @@ def problem_from_case(prefix, endian="<"):
     and the parameters to ``solve_steady`` / ``dirk_step``.  The library's
     boundary flux is the far-field one with one state for every boundary face,
     taken from the first boundary's entries of app.bcs (``bcs[ib*nch + k]``,
-    assembly/elementassembly.cpp:693); a mesh with more than one boundary is
+    assembly/elementassembly.cpp:693); a mesh with more than one boundary, and
+    app settings that select other physics or schemes (``check_supported``), are
     refused."""
     from .problem import Problem
     app = read_app_bin(prefix + "app.bin", endian)
+    check_supported(app, has_boundary=False)
     nd, ncu, nch, nc = app["nd"], app["ncu"], app["nch"], app["nc"]
     if len(app["elemtype"]) != 1:
         raise ValueError("meshes of one element type only")
@@ -334,6 +377,7 @@ def problem_from_case(prefix, endian="<"):
     ids = set(int(v) for v in msh.bf[msh.bf < 0].ravel())
     if ids - {-1}:
         raise ValueError(f"boundary ids {sorted(ids)}: the library has one far-field boundary")
+    check_supported(app, has_boundary=bool(ids))
     masters = read_master_bin(prefix + "master.bin", 1, endian)
     tables = master_tables(masters[0])
     s = read_sol_bin(prefix + "sol.bin", endian)
@@ -342,7 +386,8 @@ def problem_from_case(prefix, endian="<"):
     UDG = s["UDG"].reshape(ne, nc, npv)
     UH = s["UH"].reshape(msh.nf * npf, nch)
     st = app_settings(app)
+    # a mesh without boundary faces never reads the far-field state
+    ui = np.asarray(app["bcs"][:nch]) if ids else np.zeros(nch)
     prob = Problem(name=prefix, mesh=msh, nd=nd, ncu=ncu, nch=nch, nc=nc, param=np.asarray(st["physicsparam"]),
-                   ui=np.asarray(app["bcs"][:nch]), UDG=UDG, UH=UH, SH=np.zeros_like(UDG), fc_u=st["fc_u"],
-                   fc_q=st["fc_q"], tdep=st["tdep"])
+                   ui=ui, UDG=UDG, UH=UH, SH=np.zeros_like(UDG), fc_u=st["fc_u"], fc_q=st["fc_q"], tdep=st["tdep"])
     return prob, tables, st["solver_params"]

@@ -538,6 +538,10 @@ extern "C" int hfem_solve_steady(hfem_ctx *c, const hfem_solver_params *p, int *
     const int true_max = p->true_newton_maxiter > 0 ? p->true_newton_maxiter : INT_MAX;
     double rNorm = p->newton_tol + 1.0;
     int it = 0, true_it = 0;
+    // robustMode: reset at the start of a nonlinear solve; a failed line search goes
+    // back to the previous iterate only while it is 0, or with quasi-Newton
+    // (solvers/dgsolvers.cpp:114, :175-177)
+    int robust = 0;
     // quasi-Newton bookkeeping (sys.linearSolvesClocks, sys.lastAssemblyAndPrecClocks)
     bool reuse = false;
     double lin_time = 0.0, jac_time = 0.0;
@@ -571,8 +575,8 @@ extern "C" int hfem_solve_steady(hfem_ctx *c, const hfem_solver_params *p, int *
         RC(hfem_gmres(c, p->restart, p->gmres_tol, p->gmres_maxit, p->orthog, &gi, nullptr, &gf));
         lin_time += secs(t2);
         if (history) {
-            history[4 * (it - 1)] = oldNorm;
-            history[4 * (it - 1) + 2] = reused ? -gi : gi;
+            history[HFEM_HIST * (it - 1)] = oldNorm;
+            history[HFEM_HIST * (it - 1) + 2] = reused ? -gi : gi;
         }
         if (qn) {
             if (lin_time < p->qn_reuse_ratio * jac_time) reuse = true;
@@ -580,23 +584,29 @@ extern "C" int hfem_solve_steady(hfem_ctx *c, const hfem_solver_params *p, int *
         }
         if (it == 1 && oldNorm < p->newton_tol) {
             rNorm = oldNorm;
-            if (history) { history[1] = rNorm; history[3] = 1.0; }
+            if (history) { history[1] = rNorm; history[3] = 0.0; history[4] = 0.0; }
             break;
         }
         double alpha = 1.0, applied = 0.0;
+        int steps = 0;
         while (true) {
             RC(hfem_update_solution(c, alpha));
             applied += alpha;
+            steps++;
             RC(hfem_residual_norm(c, &rNorm));
             if (!std::isfinite(rNorm) || (rNorm > oldNorm && rNorm > p->newton_tol)) {
                 const double a = std::fabs(alpha / 2);
                 alpha = -a;
                 if (qn) recompute_jacobian();
                 if (a < alpha_min) {
-                    // back to the previous iterate (robust mode, dgsolvers.cpp:176-180)
-                    alpha = -2.0 * std::fabs(alpha);
-                    RC(hfem_update_solution(c, alpha));
-                    applied += alpha;
+                    if (robust == 0 || qn) {
+                        // back to the previous iterate (dgsolvers.cpp:175-180)
+                        robust = 1;
+                        alpha = -2.0 * std::fabs(alpha);
+                        RC(hfem_update_solution(c, alpha));
+                        applied += alpha;
+                        steps++;
+                    }
                     break;
                 }
             } else {
@@ -604,8 +614,9 @@ extern "C" int hfem_solve_steady(hfem_ctx *c, const hfem_solver_params *p, int *
             }
         }
         if (history) {
-            history[4 * (it - 1) + 1] = rNorm;
-            history[4 * (it - 1) + 3] = applied;
+            history[HFEM_HIST * (it - 1) + 1] = rNorm;
+            history[HFEM_HIST * (it - 1) + 3] = applied;
+            history[HFEM_HIST * (it - 1) + 4] = steps;
         }
     }
     if (converged) *converged = rNorm < p->newton_tol;

@@ -294,6 +294,9 @@ bilu0_factor_kernel(int bsz, const int *__restrict__ rowpts, const int *__restri
                 wg_subtract_block(Mg + (long)(d0 + l) * b2, X, b2);
                 __syncthreads();
             }
+            // targets skipped above run no barrier: every wave must be done scanning
+            // ncol before the next neighbour's column list overwrites it
+            __syncthreads();
         }
         wg_load_block(X, Mg + (long)d0 * b2, b2);
         __syncthreads();

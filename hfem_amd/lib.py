@@ -28,6 +28,7 @@ MIXED = 1
 PREC_NONE, PREC_BILU0, PREC_BLOCK_JACOBI = -1, 0, 2
 ORDER_NATURAL, ORDER_MDF = 0, 1
 MGS, ICGS = 0, 1
+HIST = 5            # doubles per Newton iteration in the solve_steady history (HFEM_HIST)
 
 
 class HfemError(RuntimeError):
@@ -173,6 +174,11 @@ class HDGContext:
     # -- state ---------------------------------------------------------------
     def set_physics(self, fc_u, fc_q=1.0, tdep=0):
         p = self.prob
+        param, ui = np.asarray(p.param).ravel(), np.asarray(p.ui).ravel()
+        if param.size < 6:
+            raise HfemError(f"physics parameters: {param.size} values, {{gamma, eps, Re, Pr, Minf, tau}} needs 6")
+        if ui.size != self.d["nch"]:
+            raise HfemError(f"far-field state: {ui.size} values, nch={self.d['nch']} needed")
         _check(load().hfem_set_physics(self.h, _d(_f64(p.param)), _d(_f64(p.ui)), fc_u, fc_q, tdep))
 
     def set_solution(self, UDG, UH, SH=None):
@@ -321,14 +327,15 @@ class HDGContext:
 
     # -- drivers (solveNonlinearProblem, solveUnsteadyProblem) -------------------
     def solve_steady(self, params=None, **kw):
-        """Newton with line search; returns (converged, iterations, history[it, 4]);
-        history[:, 2] is negative for iterations that reused the Jacobian."""
+        """Newton with line search; returns (converged, iterations, history[it, 5]):
+        residual before and after, GMRES iterations (negative for iterations that
+        reused the Jacobian), accumulated step, line-search steps applied."""
         p = params or hfem_solver_params.make(**kw)
         conv = ctypes.c_int()
         nit = ctypes.c_int()
-        hist = np.zeros(4 * p.newton_maxiter)
+        hist = np.zeros(HIST * p.newton_maxiter)
         _check(load().hfem_solve_steady(self.h, ctypes.byref(p), ctypes.byref(conv), ctypes.byref(nit), _d(hist)))
-        return bool(conv.value), nit.value, hist.reshape(-1, 4)[:nit.value]
+        return bool(conv.value), nit.value, hist.reshape(-1, HIST)[:nit.value]
 
     def dirk_step(self, dt, nstage=3, order=3, params=None, **kw):
         p = params or hfem_solver_params.make(**kw)

@@ -179,11 +179,15 @@ typedef struct {
 } hfem_solver_params;
 
 /* solveNonlinearProblem (solvers/dgsolvers.cpp:142-205): Newton with the
- * reference's halving line search and, with quasi_newton, its Jacobian reuse
- * (solveLinearProblem :40-105; a damped step or a reuse test that fails
- * recomputes the Jacobian).  history (4 doubles per iteration, may be NULL):
- * residual before, residual after, GMRES iterations (negative when the
- * iteration reused the Jacobian), accumulated step. */
+ * reference's halving line search (a failed line search returns to the previous
+ * iterate the first time in a solve, or always with quasi-Newton: robustMode,
+ * :114, :175-177) and, with quasi_newton, its Jacobian reuse (solveLinearProblem
+ * :40-105; a damped step or a reuse test that fails recomputes the Jacobian).
+ * history (HFEM_HIST doubles per iteration, may be NULL): residual before,
+ * residual after, GMRES iterations (negative when the iteration reused the
+ * Jacobian), accumulated step, number of line-search steps applied
+ * (updateSol calls). */
+#define HFEM_HIST 5
 int hfem_solve_steady(hfem_ctx *ctx, const hfem_solver_params *p, int *converged, int *niter, double *history);
 /* One time step of solveUnsteadyProblem (solvers/dgsolvers.cpp:221-444) with
  * DIRK(nstage, order) of solvers/DIRKcoeff.cpp; each stage is a steady solve

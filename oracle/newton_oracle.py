@@ -64,6 +64,38 @@ def linear_solve(prob, M, Rg, prec="bilu0", restart=30, gmres_tol=1e-10, maxit=1
     return O.gmres(M, Rg, O.PREC_NONE, restart=restart, tol=gmres_tol, maxit=maxit, orthog=orthog)
 
 
+def line_search(apply_step, residual, old, newton_tol, alpha_min=1e-4, robust=None, quasi_newton=False):
+    """The line search of solveNonlinearProblem (solvers/dgsolvers.cpp:152-187):
+    apply alpha = 1; while the new residual is NaN/Inf or above both the old one
+    and the tolerance, retract by half of the last step (sol.alpha = -|alpha/2|,
+    applied on top of what is there); once the step falls below alpha_min, go back
+    to the previous iterate (-2|alpha|) -- but only in the first such failure of a
+    nonlinear solve or with quasi-Newton (robustMode, reset to 0 at the start of
+    the solve, :114, set at :177).  apply_step(alpha) applies a step,
+    residual() returns the residual norm after it.  ``robust`` is a one-element
+    list holding robustMode across the Newton iterations of one solve.  Returns
+    (alphas applied, final residual, whether the line search failed)."""
+    robust = [0] if robust is None else robust
+    alphas = []
+    alpha = 1.0
+    while True:
+        apply_step(alpha)
+        alphas.append(alpha)
+        rnorm = residual()
+        if not np.isfinite(rnorm) or (rnorm > old and rnorm > newton_tol):
+            a = abs(alpha / 2)
+            alpha = -a
+            if a < alpha_min:
+                if robust[0] == 0 or quasi_newton:
+                    robust[0] = 1
+                    alpha = -2.0 * abs(alpha)
+                    apply_step(alpha)
+                    alphas.append(alpha)
+                return alphas, rnorm, True
+        else:
+            return alphas, rnorm, False
+
+
 def newton_solve(prob, UDG, UH, SH, newton_tol=1e-7, maxiter=10, alpha_min=1e-4, order=None, **lin):
     """Returns (UDG, UH, converged, history); history holds one dict per Newton iteration.
     The ordering is computed at the first iteration and reused (reuseOrdering,
@@ -74,6 +106,7 @@ def newton_solve(prob, UDG, UH, SH, newton_tol=1e-7, maxiter=10, alpha_min=1e-4,
     rnorm = newton_tol + 1.0
     history = []
     it = 0
+    robust = [0]
     while rnorm > newton_tol and it < maxiter:
         it += 1
         M, Rg, old, blocks = O.trace_system(prob, UDG=UDG, UH=UH, SH=SH)
@@ -86,21 +119,14 @@ def newton_solve(prob, UDG, UH, SH, newton_tol=1e-7, maxiter=10, alpha_min=1e-4,
             rnorm = old
             rec["new"] = rnorm
             break
-        alpha = 1.0
-        while True:
-            UDG, UH = update_solution(prob, UDG, UH, SH, blocks, x, alpha)
-            rec["alphas"].append(alpha)
-            rnorm = residual_norm(prob, UDG, UH, SH)
-            if not np.isfinite(rnorm) or (rnorm > old and rnorm > newton_tol):
-                a = abs(alpha / 2)
-                alpha = -a
-                if a < alpha_min:
-                    alpha = -2.0 * abs(alpha)
-                    UDG, UH = update_solution(prob, UDG, UH, SH, blocks, x, alpha)
-                    rec["alphas"].append(alpha)
-                    break
-            else:
-                break
+        state = [UDG, UH]
+
+        def apply_step(alpha):
+            state[0], state[1] = update_solution(prob, state[0], state[1], SH, blocks, x, alpha)
+
+        rec["alphas"], rnorm, _ = line_search(apply_step, lambda: residual_norm(prob, state[0], state[1], SH),
+                                              old, newton_tol, alpha_min, robust)
+        UDG, UH = state
         rec["new"] = rnorm
     return UDG, UH, rnorm < newton_tol, history
 
@@ -245,21 +271,15 @@ def newton_solve_qn(prob, UDG, UH, SH, newton_tol=1e-7, maxiter=10, alpha_min=1e
             rnorm = old
             rec["new"] = rnorm
             break
-        alpha = 1.0
-        while True:
-            UDG, UH = update_solution(prob, UDG, UH, SH, upd, x, alpha)
-            rec["alphas"].append(alpha)
-            rnorm = residual_norm(prob, UDG, UH, SH)
-            if not np.isfinite(rnorm) or (rnorm > old and rnorm > newton_tol):
-                a = abs(alpha / 2)
-                alpha = -a
-                reuse = False
-                if a < alpha_min:
-                    alpha = -2.0 * abs(alpha)
-                    UDG, UH = update_solution(prob, UDG, UH, SH, upd, x, alpha)
-                    rec["alphas"].append(alpha)
-                    break
-            else:
-                break
+        state = [UDG, UH]
+
+        def apply_step(alpha):
+            state[0], state[1] = update_solution(prob, state[0], state[1], SH, upd, x, alpha)
+
+        rec["alphas"], rnorm, _ = line_search(apply_step, lambda: residual_norm(prob, state[0], state[1], SH),
+                                              old, newton_tol, alpha_min, quasi_newton=True)
+        UDG, UH = state
+        if len(rec["alphas"]) > 1:
+            reuse = False                      # a damped step recomputes the Jacobian (:171)
         rec["new"] = rnorm
     return UDG, UH, rnorm < newton_tol, history

@@ -201,10 +201,17 @@ def write_case(prefix, pr, params):
     flag = np.zeros(24)
     flag[0], flag[10], flag[19], flag[20] = pr.tdep, 1, 1, 1
     problem = np.zeros(16)
+    problem[1] = 1                                  # appname: Navier-Stokes
     problem[13] = params["newton_maxiter"]
+    for (arr, i), v in params.get("override", {}).items():
+        if arr == "problem":
+            problem[i] = v
+        elif arr == "flag":
+            flag[i] = v
+    bcm = params.get("bcm", [1])
     arrays = [[0], [0], [pr.mesh.porder], [pr.mesh.elemtype], [0], [2 * pr.mesh.porder], [2 * pr.mesh.porder], [0],
-              [1], list(pr.ui), [1], [0.0], [0.0], flag, [pr.fc_u, pr.fc_q, 0.0], problem, list(pr.param),
-              [params["newton_tol"], params["gmres_tol"]]]
+              bcm, list(pr.ui), [1], [0.0], [0.0], flag, [pr.fc_u, pr.fc_q, 0.0], problem,
+              list(params.get("param", pr.param)), [params["newton_tol"], params["gmres_tol"]]]
     head = np.zeros(40)
     head[:18] = [len(a) for a in arrays]
     head[18:28] = [1, 1, pr.nd, pr.nd, pr.nc, pr.ncu, pr.nch, pr.nc - pr.ncu, 0, 0]
@@ -223,3 +230,67 @@ def test_problem_from_case(tmp_path):
     assert p2.dims == pr.dims
     assert np.array_equal(tables["shapvt"], pr.master.flat("shapvt"))
     assert (params["newton_maxiter"], params["newton_tol"], params["gmres_tol"]) == (9, 1e-8, 1e-9)
+
+
+@pytest.mark.parametrize("arr,i,v", [("problem", 0, 1), ("problem", 1, 0), ("problem", 5, 1), ("problem", 6, 2),
+                                     ("problem", 7, 1), ("problem", 8, 1), ("problem", 9, 1), ("problem", 10, 1),
+                                     ("problem", 11, 1), ("flag", 1, 1)])
+def test_problem_from_case_refuses_other_physics(tmp_path, arr, i, v):
+    """A case that selects physics or a scheme the element path does not have
+    (another appname, LF/Roe tau, a viscosity model, SGS/ALE/AV, the wave form,
+    EDG) is refused instead of being solved as constant-tau far-field NS."""
+    from hfem_amd.binio import problem_from_case
+    pr = make_problem("tet", 1, 2, seed=3)
+    write_case(str(tmp_path / "c_"), pr, dict(newton_maxiter=9, newton_tol=1e-8, gmres_tol=1e-9,
+                                              override={(arr, i): v}))
+    with pytest.raises(ValueError, match=rf"{arr}\[{i}\]"):
+        problem_from_case(str(tmp_path / "c_"))
+
+
+def test_problem_from_case_refuses_wall_and_short_parameters(tmp_path):
+    from hfem_amd.binio import problem_from_case
+    pr = make_problem("tet", 1, 2, seed=3)
+    base = dict(newton_maxiter=9, newton_tol=1e-8, gmres_tol=1e-9)
+    write_case(str(tmp_path / "w_"), pr, dict(base, bcm=[2]))          # a wall boundary
+    with pytest.raises(ValueError, match="far field"):
+        problem_from_case(str(tmp_path / "w_"))
+    write_case(str(tmp_path / "s_"), pr, dict(base, param=list(pr.param[:5])))
+    with pytest.raises(ValueError, match="physicsparam"):
+        problem_from_case(str(tmp_path / "s_"))
+
+
+def test_mesh_bin_with_extra_coordinate_components(tmp_path):
+    """ncd > nd (extra dgnodes components, digasopre.m:97): only the first nd
+    coordinates are the geometry; fewer than nd are refused."""
+    from hfem_amd.binio import mesh_from_bin, read_mesh_bin
+    pr = make_problem("tri", 2, 2, seed=2)
+    msh, d = pr.mesh, pr.dims
+    write_mesh_file(tmp_path / "mesh.bin", msh, d["npv"], d["ncu"], d["nch"], d["nc"])
+    raw = np.fromfile(tmp_path / "mesh.bin", dtype="<f8")
+    ne, npv, nve = pr.ne, d["npv"], msh.t.shape[1]
+    o = 30 + ne + ne * nve
+    dg = raw[o:o + ne * 2 * npv].reshape(ne, 2, npv)
+    extra = np.concatenate([dg, np.full((ne, 1, npv), 7.5)], axis=1)   # a third, unrelated component
+    head = raw[:30].copy()
+    head[1] = 3
+    np.concatenate([head, raw[30:o], extra.ravel(), raw[o + dg.size:]]).tofile(tmp_path / "mesh3.bin")
+    m = read_mesh_bin(tmp_path / "mesh3.bin")
+    back = mesh_from_bin(m, 2, msh.elemtype, msh.porder)
+    assert np.array_equal(back.dgnodes, msh.dgnodes)
+    m["ndims"]["ncd"] = 1
+    with pytest.raises(ValueError, match="ncd"):
+        mesh_from_bin(m, 2, msh.elemtype, msh.porder)
+
+
+def test_sol_bin_pdg_count_without_payload(tmp_path, prob):
+    """The preprocessor writes numel(PDG) into the header but the PDG values only
+    for the wave formulation (digasopre.m:129, :292-297): such a file is read
+    with an empty PDG."""
+    from hfem_amd.binio import read_sol_bin, write_sol_bin
+    path = tmp_path / "sol.bin"
+    write_sol_bin(path, prob.UDG, prob.UH)
+    raw = np.fromfile(path, dtype="<f8")
+    raw[2] = 17
+    raw.tofile(path)
+    s = read_sol_bin(path)
+    assert s["PDG"].size == 0 and np.array_equal(s["UH"], prob.UH.ravel())

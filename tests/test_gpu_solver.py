@@ -159,3 +159,90 @@ def test_dirk_step_matches_oracle(oracle_mod):
     assert conv
     assert rel(U, Ur) < 1e-9 and rel(UH, UHr) < 1e-9
     ctx.close()
+
+
+def test_newton_line_search_retraction_matches_oracle(oracle_mod):
+    """A state far from the solution (perturbation 0.5 on u and q) whose full
+    Newton steps raise the residual: the line search retracts (alphas 1, -1/2 in
+    the first iterations, solvers/dgsolvers.cpp:152-187).  The GPU driver's
+    per-iteration line-search steps, accumulated step, GMRES counts and residual
+    history equal the oracle's."""
+    import newton_oracle as N
+    from hfem_amd.lib import HDGContext
+    prob = make_problem("tri", 2, 2, seed=5, angle=0.3, amp=0.5, qamp=0.5)
+    kw = dict(restart=30, gmres_tol=1e-12)
+    Ur, UHr, okr, hr = N.newton_solve(prob, prob.UDG, prob.UH, prob.SH, newton_tol=1e-10, maxiter=10, **kw)
+    assert okr and any(len(h["alphas"]) > 1 for h in hr), "the case must exercise the retraction"
+    ctx = HDGContext(prob)
+    conv, nit, hist = ctx.solve_steady(newton_tol=1e-10, newton_maxiter=10, ordering=1, **kw)
+    say(f"  retraction gpu steps {hist[:, 4].tolist()} applied {hist[:, 3].tolist()}; "
+        f"oracle {[h['alphas'] for h in hr]}")
+    assert conv and nit == len(hr)
+    for k, h in enumerate(hr):
+        assert hist[k, 4] == len(h["alphas"])
+        assert hist[k, 3] == sum(h["alphas"])
+        assert hist[k, 2] == h["gmres_iters"]
+        assert abs(hist[k, 0] - h["old"]) <= 1e-8 * h["old"] + 1e-12
+    U, UH = ctx.get_solution()
+    assert rel(U, Ur) < 1e-10 and rel(UH, UHr) < 1e-10
+    ctx.close()
+
+
+def test_mixed_newton_same_iterations_as_fp64():
+    """The mixed condensation (FP32 factorisation, FP64 refinement) inside Newton:
+    the same Newton and GMRES iteration counts as the FP64 path and a residual
+    history within 1e-10 relative of it (above the 1e-12 level)."""
+    from hfem_amd.lib import HDGContext, FP64, MIXED
+    prob = make_problem("tet", 1, 3, seed=5, angle=0.3, elevation=0.2)
+    runs = {}
+    for prec in (FP64, MIXED):
+        ctx = HDGContext(prob)
+        conv, nit, hist = ctx.solve_steady(newton_tol=1e-11, newton_maxiter=8, restart=30, gmres_tol=1e-12,
+                                           ordering=1, precision=prec)
+        U, UH = ctx.get_solution()
+        runs[prec] = (conv, nit, hist, U, UH)
+        ctx.close()
+    (c0, n0, h0, U0, UH0), (c1, n1, h1, U1, UH1) = runs[FP64], runs[MIXED]
+    say(f"  newton fp64 {h0[:, [0, 2]].tolist()} mixed {h1[:, [0, 2]].tolist()}")
+    assert c0 and c1 and n0 == n1
+    assert np.array_equal(h0[:, 2], h1[:, 2]) and np.array_equal(h0[:, 4], h1[:, 4])
+    for k in range(n0):
+        assert abs(h1[k, 0] - h0[k, 0]) <= 1e-10 * h0[k, 0] + 1e-12
+    assert rel(U1, U0) < 1e-10 and rel(UH1, UH0) < 1e-10
+
+
+def test_mixed_dirk_step_same_iterations_as_fp64():
+    from hfem_amd.lib import HDGContext, FP64, MIXED
+    prob = make_problem("tri", 2, 3, seed=9, angle=0.3)
+    runs = {}
+    for prec in (FP64, MIXED):
+        ctx = HDGContext(prob)
+        conv, nit = ctx.dirk_step(0.05, 3, 3, newton_tol=1e-11, newton_maxiter=8, restart=30, gmres_tol=1e-12,
+                                  precision=prec)
+        runs[prec] = (conv, nit) + ctx.get_solution()
+        ctx.close()
+    (c0, n0, U0, UH0), (c1, n1, U1, UH1) = runs[FP64], runs[MIXED]
+    say(f"  dirk newton its fp64 {n0} mixed {n1}; UDG {rel(U1, U0):.2e} UH {rel(UH1, UH0):.2e}")
+    assert c0 and c1 and n0 == n1
+    assert rel(U1, U0) < 1e-10 and rel(UH1, UH0) < 1e-10
+
+
+def test_dirk_step_periodic_hex_matches_oracle(oracle_mod):
+    """The DIRK(3,3) configuration on p=2 hexes at 3^3 periodic cells (the
+    Taylor-Green state of the 24^3 benchmark case, 45 x 45 trace blocks, rows of
+    11 blocks): Newton iteration counts per stage and the final state equal the
+    oracle's."""
+    import newton_oracle as N
+    from hfem_amd.lib import HDGContext
+    prob = make_problem("hex", 3, 2, seed=0, state="taylor_green", amp=1e-3)
+    say(f"  hex3 p2 periodic: ne={prob.ne} nf={prob.mesh.nf}")
+    Ur, UHr, SHr, hist = N.dirk_step(prob, prob.UDG, prob.UH, prob.SH, 0.1, newton_tol=1e-10, maxiter=8,
+                                     restart=30, gmres_tol=1e-12)
+    ctx = HDGContext(prob)
+    conv, nit = ctx.dirk_step(0.1, 3, 3, newton_tol=1e-10, newton_maxiter=8, restart=30, gmres_tol=1e-12)
+    U, UH = ctx.get_solution()
+    say(f"  dirk hex: newton its gpu {nit} oracle {sum(len(h) for h in hist)}; "
+        f"UDG {rel(U, Ur):.2e} UH {rel(UH, UHr):.2e}")
+    assert conv and nit == sum(len(h) for h in hist)
+    assert rel(U, Ur) < 1e-10 and rel(UH, UHr) < 1e-10
+    ctx.close()

@@ -233,3 +233,155 @@ def test_dirk_step_keeps_a_uniform_state(oracle_mod):
     np.testing.assert_allclose(UDG, prob.UDG, rtol=0, atol=1e-12)
     np.testing.assert_allclose(UH, prob.UH, rtol=0, atol=1e-12)
     assert all(len(h) == 1 and h[0]["old"] < 1e-10 for h in hist)
+
+
+def scalar_block_graph(O, tri_system, seed=7):
+    """The trace graph of a small triangle mesh (every row couples to 4 faces,
+    pairs of neighbours that are not neighbours of each other, so eliminating a
+    row discards fill) with blocks h_ij * I, h_ij random and not symmetric, and
+    every diagonal block sqrt(bsz) * I."""
+    _, M, *_ = tri_system
+    rng = np.random.default_rng(seed)
+    b = M.bsz
+    blocks = []
+    for i in range(M.nrows):
+        for k in range(M.rowpts[i], M.rowpts[i + 1]):
+            h = np.sqrt(b) if M.colind[k] == i else rng.uniform(0.05, 2.0)
+            blocks.append((h * np.eye(b)).reshape(-1))
+    return O.BSR(M.rowpts, M.colind, np.concatenate(blocks), b)
+
+
+def test_approximate_mdf_equals_exact_on_scalar_blocks(oracle_mod, tri_system):
+    """With blocks h_ij I and diagonal blocks sqrt(bsz) I, the approximate weight
+    C_ik C_kj = |h_ik| |h_kj| of kernel/computeMDFordering.m:24-47 equals the exact
+    discarded-fill weight ||H_ik inv(H_kk) H_kj||_F of computeExactMDFordering.m,
+    so the two greedy orderings coincide -- and both equal a brute-force greedy
+    that, at every step, eliminates densely each unnumbered row k of the original
+    matrix and measures the fill it would discard among the unnumbered rows
+    (pairs i, j of neighbours of k that are not neighbours of each other), taking
+    the smallest (lowest row on ties).  A weight that counts adjacent pairs,
+    numbered rows or the diagonal block, or a wrong update set, fails it."""
+    O = oracle_mod
+    M = scalar_block_graph(O, tri_system)
+    o2u_a, _ = O.mdf_order(M, exact=False)
+    o2u_e, _ = O.mdf_order(M, exact=True)
+    A, b, n = M.dense(), M.bsz, M.nrows
+    pattern = np.zeros((n, n), bool)
+    for i in range(n):
+        pattern[i, M.colind[M.rowpts[i]:M.rowpts[i + 1]]] = True
+    numbered = np.zeros(n, bool)
+    brute = []
+    for _ in range(n):
+        w = np.full(n, np.inf)
+        for k in np.flatnonzero(~numbered):
+            K = slice(k * b, (k + 1) * b)
+            upd = A[:, K] @ np.linalg.solve(A[K, K], A[K, :])
+            live = [i for i in range(n) if not numbered[i] and i != k and pattern[i, k]]
+            w[k] = np.sqrt(sum(np.sum(upd[i * b:(i + 1) * b, j * b:(j + 1) * b] ** 2)
+                               for i in live for j in live if i != j and not pattern[i, j]))
+        p = int(np.flatnonzero(w <= w.min() * (1 + 1e-12))[0])
+        brute.append(p)
+        numbered[p] = True
+    assert list(o2u_e) == brute
+    assert list(o2u_a) == brute
+
+
+def test_mdf_exact_first_pivot_on_scalar_blocks(oracle_mod, tri_system):
+    """On the scalar-block graph the first pivot of both orderings is the row whose
+    dense block elimination discards the least fill."""
+    O = oracle_mod
+    M = scalar_block_graph(O, tri_system)
+    A, b = M.dense(), M.bsz
+    pattern = np.zeros((M.nrows, M.nrows), bool)
+    for i in range(M.nrows):
+        pattern[i, M.colind[M.rowpts[i]:M.rowpts[i + 1]]] = True
+    w = np.zeros(M.nrows)
+    for k in range(M.nrows):
+        K = slice(k * b, (k + 1) * b)
+        upd = A[:, K] @ np.linalg.solve(A[K, K], A[K, :])
+        w[k] = np.sqrt(sum(np.sum(upd[i * b:(i + 1) * b, j * b:(j + 1) * b] ** 2)
+                           for i in range(M.nrows) for j in range(M.nrows)
+                           if len({i, j, k}) == 3 and pattern[i, k] and pattern[k, j] and not pattern[i, j]))
+    for exact in (False, True):
+        assert O.mdf_order(M, exact=exact)[0][0] == int(np.argmin(w))
+
+
+def _ls_run(N, r_of_net, old, tol=1e-12, robust=None, qn=False):
+    net = [0.0]
+
+    def apply_step(a):
+        net[0] += a
+
+    alphas, r, failed = N.line_search(apply_step, lambda: r_of_net(net[0]), old, tol, 1e-4, robust, qn)
+    return alphas, net[0], r, failed
+
+
+def test_line_search_known_answers():
+    """solveNonlinearProblem's line search (solvers/dgsolvers.cpp:152-187) on
+    residuals of the net step s, against sequences derived by hand from that code:
+    a full step that overshoots is retracted by half (1, -1/2); two retractions
+    give 1, -1/2, -1/4; a step that never helps is halved 14 times until alpha
+    = 2^-14 < 1e-4, then the last applied step is undone twice over (-2^-13)
+    back to the previous iterate -- the first time in a solve only (robustMode);
+    a second failure in the same solve stays at net 2^-13; with quasi-Newton it
+    always goes back."""
+    import newton_oracle as N
+    a, net, r, f = _ls_run(N, lambda s: abs(s - 0.45) + 0.01, old=0.46)
+    assert a == [1.0, -0.5] and net == 0.5 and not f
+    a, net, r, f = _ls_run(N, lambda s: abs(s - 0.2) + 0.01, old=0.21)
+    assert a == [1.0, -0.5, -0.25] and net == 0.25 and not f
+    halvings = [-(2.0 ** -k) for k in range(1, 14)]
+    robust = [0]
+    a, net, r, f = _ls_run(N, lambda s: 1.0 + s, old=1.0, robust=robust)
+    assert f and a == [1.0] + halvings + [-(2.0 ** -13)] and net == 0.0 and robust == [1]
+    a, net, r, f = _ls_run(N, lambda s: 1.0 + s, old=1.0, robust=robust)
+    assert f and a == [1.0] + halvings and net == 2.0 ** -13
+    a, net, r, f = _ls_run(N, lambda s: 1.0 + s, old=1.0, robust=[1], qn=True)
+    assert f and a[-1] == -(2.0 ** -13) and net == 0.0
+    # NaN counts as a failed step; a residual below the Newton tolerance is accepted
+    a, net, r, f = _ls_run(N, lambda s: float("nan") if s > 0.6 else 0.5, old=0.4)
+    assert a[:2] == [1.0, -0.5] and net == 0.5 and not f and r == 0.5 or (f and net == 0.0)
+    a, net, r, f = _ls_run(N, lambda s: 1e-13 + s * 0, old=1e-14, tol=1e-12)
+    assert a == [1.0] and not f
+
+
+@pytest.mark.parametrize("nstage,order", [(1, 1), (2, 2), (2, 3), (3, 3), (3, 4)])
+def test_dirk_stage_sources_reproduce_butcher_dirk(nstage, order):
+    """The stage sources of solveUnsteadyProblem (solvers/dgsolvers.cpp:316-341)
+    with fc_u = d_ii/dt, on the linear test equation u' = -lam u solved exactly
+    per stage ((d_ii/dt + lam) U_i = SH_i), give the stage values of the DIRK
+    defined by its Butcher matrix A = inv(d): (I + dt lam A) U = u_n 1.  The
+    Butcher matrices of (3,3) and (2,2) are kernel/hdg_solve_dirk.m:218-227's; the
+    others are inv(d) of the coefficient table (pinned for (3,3) and (2,2) by
+    test_dirk_coefficients_from_butcher_tables).  A transposed d index, a dropped
+    stage term or the wrong previous-stage source fails it."""
+    import newton_oracle as N
+    c, d, t = N.dirk_coefficients(nstage, order)
+    dm = d.reshape(nstage, nstage, order="F")
+    if (nstage, order) == (3, 3):
+        a1 = 0.4358665215
+        t1 = (1 + a1) / 2
+        A = np.array([[a1, 0, 0], [t1 - a1, a1, 0],
+                      [-(6 * a1 ** 2 - 16 * a1 + 1) / 4, (6 * a1 ** 2 - 20 * a1 + 5) / 4, a1]])
+        tol = 1e-9
+    elif (nstage, order) == (2, 2):
+        s2 = np.sqrt(2.0)
+        A = np.array([[1 - s2 / 2, 0], [s2 / 2, 1 - s2 / 2]])
+        tol = 1e-12
+    else:
+        A = np.linalg.inv(dm)
+        tol = 1e-12
+    rng = np.random.default_rng(nstage * 10 + order)
+    dt = 0.3
+    un = rng.uniform(0.5, 2.0, size=(4, 3, 5))
+    lam = rng.uniform(0.1, 5.0, size=un.shape)
+    U, SH = un.copy(), np.zeros_like(un)
+    stages = []
+    for s in range(nstage):
+        SH = N.stage_source(s, nstage, d, dt, un, U, SH)
+        U = SH / (dm[s, s] / dt + lam)
+        stages.append(U)
+    for idx in np.ndindex(un.shape):
+        direct = np.linalg.solve(np.eye(nstage) + dt * lam[idx] * A, un[idx] * np.ones(nstage))
+        got = np.array([st[idx] for st in stages])
+        np.testing.assert_allclose(got, direct, rtol=tol, atol=0)
