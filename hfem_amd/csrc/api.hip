@@ -34,6 +34,8 @@ static hipError_t dalloc(T **p, size_t n)
     return hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T));
 }
 
+static void default_choices(hfem_ctx *c);
+
 extern "C" int hfem_create(hfem_ctx **out, const hfem_dims *hd, int device)
 {
     if (!out || !hd) return hfem_fail(-1, "hfem_create: null argument");
@@ -92,6 +94,10 @@ extern "C" int hfem_create(hfem_ctx **out, const hfem_dims *hd, int device)
         }
     }
     ALLOC(dinv, ne * d.na * d.na);
+    ALLOC(rdmax, ne);
+    ALLOC(rcmax, ne);
+    ALLOC(rdprev, ne);
+    ALLOC(rcount, 1);
     if (assemble_spill_doubles(d) > 0) ALLOC(spill, (long)c->grid_asm * assemble_spill_doubles(d));
 #undef ALLOC
     if (e == hipSuccess) e = hipStreamCreate(&c->stream);
@@ -103,6 +109,7 @@ extern "C" int hfem_create(hfem_ctx **out, const hfem_dims *hd, int device)
         hfem_destroy(c);
         return hfem_fail(-2, "hfem_create: allocation failed: %s", hipGetErrorString(e));
     }
+    default_choices(c);
     *out = c;
     return 0;
 }
@@ -117,7 +124,7 @@ extern "C" int hfem_destroy(hfem_ctx *c)
                     c->X, c->K, c->Hx, c->Ru, c->Rh, c->ru2, c->info, c->dinv, c->spill, c->rwork, c->fluxbuf, c->Hg, c->Rg,
                     c->r, c->partial, c->xv, c->yv, c->sv.d_o2u, c->sv.d_u2o, c->sv.d_lrows, c->sv.d_urows,
                     c->sv.Mg, c->sv.Dinv, c->sv.Cw, c->sv.d_info, c->sv.x, c->sv.V, c->sv.b, c->sv.t, c->sv.dots,
-                    c->sv.coef, c->sv.UDGi, c->qn_dinv, c->qn_K};
+                    c->sv.coef, c->sv.UDGi, c->qn_dinv, c->qn_K, c->rdmax, c->rcmax, c->rdprev, c->rcount};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (auto &ev : c->ev)
@@ -383,13 +390,90 @@ static CondGemm tuned_gemm(const hfem_ctx *c, int which, const double *Fx, const
     return g;
 }
 
-// Picks, for each of the condensation GEMMs, the fastest rocBLAS solution on
-// a sample of nsample elements (default min(ne, 32768): the winners change with
-// the batch size, so the default times the batch the condensation runs).  Inputs are the
-// context's Fx, inv(D) and K buffers (their values do not matter for timing);
-// outputs go to temporary buffers, so no element block of the context changes.
-// A solution is kept only if it beats rocBLAS's own choice by more than 2 %.
-extern "C" int hfem_tune_condensation(hfem_ctx *c, int nsample, int *solutions, float *ms)
+// The condensation's products in the batched-GEMM form of bgemm.hip, over the
+// elements [e0, e0+n):
+//   0: X = inv(D) [F Ru]              FP64 MFMA
+//   1: [AH FH] = [H Rh] - K X         FP64 MFMA (computed transposed: AH^T = H^T - X^T K^T)
+//   2: X = inv32 [F Ru]               FP32 MFMA (mixed X0)
+//   3: R = [F Ru] - D X               FP64 MFMA (mixed refinement residual)
+//   4: X += inv32 R                   FP32 MFMA (mixed correction; per-element maxima)
+// inv points at the FP64 (0) or FP32 (2, 4) inverse; R at the residual buffer.
+static int prod_mode(int which) { return (which == 2 || which == 4) ? 1 : 0; }
+
+static BgemmArgs prod_args(const hfem_ctx *c, int which, int e0, int n, const void *inv, const double *Fx,
+                           const double *Dm, const double *K, double *X, double *Hx, double *R)
+{
+    const Dims &d = c->d;
+    const long sx = (long)d.na * c->ldx, sd = (long)d.na * d.na, sk = (long)d.nb * d.na,
+               sh = (long)d.nb * (d.nb + 1);
+    BgemmArgs g{};
+    g.e0 = e0;
+    g.n = n;
+    if (which == 1) {
+        g.M = d.nb + 1; g.N = d.nb; g.K = d.na;
+        g.A = X; g.sa = sx; g.lda = c->ldx;
+        g.B = K; g.sb = sk; g.ldb = d.nb;
+        g.alpha = -1.0; g.beta = 1.0;
+        g.C0 = Hx; g.sc0 = sh; g.ldc0 = d.nb;
+        g.C = Hx; g.sc = sh; g.ldc = d.nb;
+        return g;
+    }
+    g.M = d.na; g.N = d.nb + 1; g.K = d.na;
+    g.sa = sd; g.lda = d.na;
+    g.sb = sx; g.ldb = c->ldx;
+    g.sc = sx; g.ldc = c->ldx;
+    g.sc0 = sx; g.ldc0 = c->ldx;
+    g.alpha = 1.0;
+    if (which == 0 || which == 2) { g.A = inv; g.B = Fx; g.C = X; }
+    else if (which == 3) { g.A = Dm; g.B = X; g.C0 = Fx; g.beta = 1.0; g.alpha = -1.0; g.C = R; }
+    else { g.A = inv; g.B = R; g.C0 = X; g.beta = 1.0; g.C = X; g.dmax = c->rdmax + e0; g.cmax = c->rcmax + e0; }
+    return g;
+}
+
+// rocBLAS form of products 0, 1, 3 (the measured comparison; tuned_gemm numbering)
+static int prod_rocblas_index(int which) { return which == 0 ? 0 : which == 1 ? 1 : which == 3 ? 2 : -1; }
+
+static hipError_t run_prod(hfem_ctx *c, int which, int var, int e0, int n, const void *inv, const double *Fx,
+                           const double *Dm, const double *K, double *X, double *Hx, double *R, int sol = 0)
+{
+    if (var < 0) {
+        const int rw = prod_rocblas_index(which);
+        if (rw < 0) return hipErrorInvalidValue;
+        const Dims &d = c->d;
+        const long sx = (long)d.na * c->ldx, sd = (long)d.na * d.na, sk = (long)d.nb * d.na,
+                   sh = (long)d.nb * (d.nb + 1);
+        const CondGemm g = tuned_gemm(c, rw, Fx + e0 * sx, Dm ? Dm + e0 * sd : nullptr,
+                                      inv ? (const double *)inv + e0 * sd : nullptr, K ? K + e0 * sk : nullptr,
+                                      X + e0 * sx, Hx ? Hx + e0 * sh : nullptr, R ? R + e0 * sx : nullptr, n);
+        return cond_gemm_run(c->blas, g, sol) == rocblas_status_success ? hipSuccess : hipErrorLaunchFailure;
+    }
+    return launch_bgemm(prod_mode(which), var, prod_args(c, which, e0, n, inv, Fx, Dm, K, X, Hx, R),
+                        (size_t)c->maxlds, c->stream);
+}
+
+// Default launch choices before any tuning: the first hand-written variant that
+// fits each product.
+static void default_choices(hfem_ctx *c)
+{
+    const Dims &d = c->d;
+    for (int w = 0; w < 5; w++) {
+        const int M = w == 1 ? d.nb + 1 : d.na, N = w == 1 ? d.nb : d.nb + 1;
+        c->prod_var[w] = -1;
+        for (int v = 0; v < bgemm_num_variants(); v++)
+            if (bgemm_fits(prod_mode(w), v, M, N, (size_t)c->maxlds)) { c->prod_var[w] = v; break; }
+    }
+}
+
+// Times the launch candidates of every stage of the condensation on a sample of
+// nsample elements (default min(ne, 32768); the winners change with the batch
+// size) and keeps the fastest: the inverse kernels' launch shapes (FP64, FP32),
+// and for each product the hand-written variants that fit its shape plus, for the
+// FP64 products 0, 1 and 3, rocBLAS (its own choice and every solution it lists).
+// Inputs are the context's D, [F Ru] and K; outputs go to temporary buffers, so
+// no element block of the context changes.  choice[7] (may be NULL) receives
+// {inverse FP64, inverse FP32, product 0..4} (product: variant >= 0, or -1 for
+// rocBLAS), ms[7] (may be NULL) the best time of each in milliseconds.
+extern "C" int hfem_tune_condensation(hfem_ctx *c, int nsample, int *choice, float *ms)
 {
     if (!c) return hfem_fail(-1, "hfem_tune_condensation: null context");
     HIPCHK(hipSetDevice(c->device));
@@ -398,58 +482,136 @@ extern "C" int hfem_tune_condensation(hfem_ctx *c, int nsample, int *solutions, 
     if (nsample < 0) return hfem_fail(-1, "hfem_tune_condensation: nsample=%d < 0", nsample);
     if (ne == 0) return 0;
     const int ns = nsample > 0 ? std::min(nsample, ne) : std::min(ne, 32768);
-    double *Xt = nullptr, *Ht = nullptr, *Rt = nullptr;
+    double *Xt = nullptr, *Ht = nullptr, *Rt = nullptr, *It = nullptr;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     int rc = 0;
+    // Rt holds the timed inverses (ns na^2 doubles) and then the sample's FP32 inverse
     if (hfem_dalloc(&Xt, (size_t)ns * d.na * c->ldx) != hipSuccess ||
-        hfem_dalloc(&Rt, (size_t)ns * d.na * c->ldx) != hipSuccess ||
+        hfem_dalloc(&Rt, (size_t)ns * d.na * std::max(c->ldx, d.na)) != hipSuccess ||
+        hfem_dalloc(&It, (size_t)ns * d.na * d.na) != hipSuccess ||
         hfem_dalloc(&Ht, (size_t)ns * d.nb * (d.nb + 1)) != hipSuccess || hipEventCreate(&t0) != hipSuccess ||
         hipEventCreate(&t1) != hipSuccess)
         rc = hfem_fail(-2, "hfem_tune_condensation: out of device memory");
-    if (!rc && (hipMemsetAsync(Xt, 0, (size_t)ns * d.na * c->ldx * 8, c->stream) != hipSuccess ||
-                hipMemsetAsync(Rt, 0, (size_t)ns * d.na * c->ldx * 8, c->stream) != hipSuccess ||
-                hipMemsetAsync(Ht, 0, (size_t)ns * d.nb * (d.nb + 1) * 8, c->stream) != hipSuccess))
-        rc = hfem_fail(-2, "hfem_tune_condensation: hipMemsetAsync failed");
-    for (int which = 0; which < 4 && !rc; which++) {
-        const CondGemm g = tuned_gemm(c, which, c->Fx, c->D, c->dinv, c->K, Xt, Ht, Rt, ns);
-        // best of two timed runs after one untimed run; -1 if the solution fails
-        auto time_it = [&](int sol) -> float {
-            if (cond_gemm_run(c->blas, g, sol) != rocblas_status_success) return -1.f;
-            float best = 1e30f;
-            for (int r = 0; r < 2; r++) {
-                float t = 0;
-                if (hipEventRecord(t0, c->stream) != hipSuccess || cond_gemm_run(c->blas, g, sol) != rocblas_status_success ||
-                    hipEventRecord(t1, c->stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
-                    hipEventElapsedTime(&t, t0, t1) != hipSuccess)
-                    return -1.f;
-                best = std::min(best, t);
+    // realistic operands: the inverses of the sample's D, X = inv(D)[F Ru], H
+    if (!rc && (launch_dinv(d.na, 0, ns, c->D, It, c->info, c->stream) != hipSuccess ||
+                hipMemcpyAsync(Ht, c->Hx, (size_t)ns * d.nb * (d.nb + 1) * 8, hipMemcpyDeviceToDevice,
+                               c->stream) != hipSuccess))
+        rc = hfem_fail(-2, "hfem_tune_condensation: setup failed");
+    // best of two timed runs after one untimed run; -1 if the launch fails
+    auto time_it = [&](auto &&launch) -> float {
+        if (launch() != hipSuccess) { (void)hipGetLastError(); return -1.f; }
+        float best = 1e30f;
+        for (int r = 0; r < 2; r++) {
+            float t = 0;
+            if (hipEventRecord(t0, c->stream) != hipSuccess || launch() != hipSuccess ||
+                hipEventRecord(t1, c->stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
+                hipEventElapsedTime(&t, t0, t1) != hipSuccess) {
+                (void)hipGetLastError();
+                return -1.f;
             }
-            return best;
-        };
-        const float tdef = time_it(0);
-        if (tdef < 0) { rc = hfem_fail(-2, "hfem_tune_condensation: rocBLAS GEMM %d failed", which); break; }
-        std::vector<rocblas_int> sols;
-        if (cond_gemm_solutions(c->blas, g, sols) != rocblas_status_success) sols.clear();
-        int bsol = 0;
-        float bt = tdef;
-        for (rocblas_int s : sols) {
-            if (s == 0) continue;
-            const float t = time_it(s);
-            if (t >= 0 && t < bt) { bt = t; bsol = s; }
+            best = std::min(best, t);
         }
-        if (bt > 0.98f * tdef) { bsol = 0; bt = tdef; }
-        c->gemm_sol[which] = bsol;
-        if (solutions) solutions[which] = bsol;
-        if (ms) { ms[2 * which] = tdef; ms[2 * which + 1] = bt; }
+        return best;
+    };
+    float best[7];
+    for (int p = 0; p < 2 && !rc; p++) {
+        int bv = 0;
+        float bt = 1e30f;
+        for (int v = 0; v < kDinvVariants; v++) {
+            const float t = time_it([&]() {
+                return p == 0 ? launch_dinv(d.na, 0, ns, c->D, Rt, c->info, c->stream, v)
+                              : launch_dinv32(d.na, 0, ns, c->D, (float *)Rt, c->info, c->stream, v);
+            });
+            if (t >= 0 && t < bt) { bt = t; bv = v; }
+        }
+        c->inv_var[p] = bv;
+        best[p] = bt;
+    }
+    float *It32 = (float *)Rt;   // FP32 inverse of the sample for the mixed products (in Rt's first half)
+    double *R2 = nullptr;
+    if (!rc && (launch_dinv32(d.na, 0, ns, c->D, It32, c->info, c->stream) != hipSuccess ||
+                hfem_dalloc(&R2, (size_t)ns * d.na * c->ldx) != hipSuccess ||
+                hipMemsetAsync(R2, 0, (size_t)ns * d.na * c->ldx * 8, c->stream) != hipSuccess ||
+                launch_refine_reset(ns, c->rdmax, c->rcmax, c->rcount, c->stream) != hipSuccess))
+        rc = hfem_fail(-2, "hfem_tune_condensation: setup failed");
+    for (int w = 0; w < 5 && !rc; w++) {
+        const int M = w == 1 ? d.nb + 1 : d.na, N = w == 1 ? d.nb : d.nb + 1;
+        const void *inv = prod_mode(w) ? (const void *)It32 : (const void *)It;
+        int bv = -2;
+        float bt = 1e30f;
+        for (int v = 0; v < bgemm_num_variants(); v++) {
+            if (!bgemm_fits(prod_mode(w), v, M, N, (size_t)c->maxlds)) continue;
+            const float t = time_it([&]() { return run_prod(c, w, v, 0, ns, inv, c->Fx, c->D, c->K, Xt, Ht, R2); });
+            if (t >= 0 && t < bt) { bt = t; bv = v; }
+        }
+        const int rw = prod_rocblas_index(w);
+        if (rw >= 0) {
+            // rocBLAS: its own choice, then each listed solution
+            const CondGemm g = tuned_gemm(c, rw, c->Fx, c->D, It, c->K, Xt, Ht, R2, ns);
+            std::vector<rocblas_int> sols{0};
+            std::vector<rocblas_int> more;
+            if (cond_gemm_solutions(c->blas, g, more) == rocblas_status_success) sols.insert(sols.end(), more.begin(), more.end());
+            for (rocblas_int sol : sols) {
+                const float t = time_it([&]() {
+                    return cond_gemm_run(c->blas, g, sol) == rocblas_status_success ? hipSuccess : hipErrorLaunchFailure;
+                });
+                if (t >= 0 && t < bt) { bt = t; bv = -1; c->gemm_sol[rw] = sol; }
+            }
+        }
+        if (bv == -2) { rc = hfem_fail(-2, "hfem_tune_condensation: no launch candidate runs product %d", w); break; }
+        c->prod_var[w] = bv;
+        best[2 + w] = bt;
     }
     (void)hipStreamSynchronize(c->stream);
+    if (!rc) {
+        if (choice) {
+            choice[0] = c->inv_var[0];
+            choice[1] = c->inv_var[1];
+            for (int w = 0; w < 5; w++) choice[2 + w] = c->prod_var[w];
+        }
+        if (ms)
+            for (int i = 0; i < 7; i++) ms[i] = best[i];
+    }
     if (t0) (void)hipEventDestroy(t0);
     if (t1) (void)hipEventDestroy(t1);
-    if (Xt) (void)hipFree(Xt);
-    if (Rt) (void)hipFree(Rt);
-    if (Ht) (void)hipFree(Ht);
+    for (double *p : {Xt, Rt, Ht, It, R2})
+        if (p) (void)hipFree(p);
     (void)hipGetLastError();
     return rc;
+}
+
+// Number of hand-written product variants (choices 0 .. n-1 of each product).
+extern "C" int hfem_condense_variants(void) { return bgemm_num_variants(); }
+
+// Sets the launch choices in the tuner's layout (e.g. a pick kept from an earlier
+// tuning), after checking that each fits this context's shapes.
+extern "C" int hfem_set_condensation_choice(hfem_ctx *c, const int *choice)
+{
+    if (!c || !choice) return hfem_fail(-1, "hfem_set_condensation_choice: null argument");
+    const Dims &d = c->d;
+    for (int p = 0; p < 2; p++)
+        if (choice[p] < 0 || choice[p] >= kDinvVariants)
+            return hfem_fail(-1, "hfem_set_condensation_choice: inverse variant %d out of range", choice[p]);
+    for (int w = 0; w < 5; w++) {
+        const int v = choice[2 + w], M = w == 1 ? d.nb + 1 : d.na, N = w == 1 ? d.nb : d.nb + 1;
+        if (v == -1 && prod_rocblas_index(w) >= 0) continue;
+        if (!bgemm_fits(prod_mode(w), v, M, N, (size_t)c->maxlds))
+            return hfem_fail(-1, "hfem_set_condensation_choice: product %d cannot run variant %d", w, v);
+    }
+    c->inv_var[0] = choice[0];
+    c->inv_var[1] = choice[1];
+    for (int w = 0; w < 5; w++) c->prod_var[w] = choice[2 + w];
+    return 0;
+}
+
+// The FP64 condensation of [e0, e0+n): inv(D), X = inv(D)[F Ru], [AH FH] = [H Rh] - K X.
+static int condense_fp64(hfem_ctx *c, int e0, int n)
+{
+    const Dims &d = c->d;
+    HIPCHK(launch_dinv(d.na, e0, n, c->D, c->dinv, c->info, c->stream, c->inv_var[0]));
+    HIPCHK(run_prod(c, 0, c->prod_var[0], e0, n, c->dinv, c->Fx, c->D, c->K, c->X, c->Hx, nullptr, c->gemm_sol[0]));
+    HIPCHK(run_prod(c, 1, c->prod_var[1], e0, n, c->dinv, c->Fx, c->D, c->K, c->X, c->Hx, nullptr, c->gemm_sol[1]));
+    return 0;
 }
 
 extern "C" int hfem_condense(hfem_ctx *c, int e0, int n, int precision)
@@ -461,43 +623,66 @@ extern "C" int hfem_condense(hfem_ctx *c, int e0, int n, int precision)
     HIPCHK(hipSetDevice(c->device));
     const Dims &d = c->d;
     HIPCHK(hipEventRecord(c->ev[2], c->stream));
-    if (precision == HFEM_FP64)
-    {
-        // inv(D) per element, then X = inv(D) [F Ru] and [AH FH] = [H Rh] - K X as
-        // strided-batched DGEMMs (solutions as picked by hfem_tune_condensation)
-        HIPCHK(launch_dinv(d.na, e0, n, c->D, c->dinv, c->info, c->stream));
-        const long sx = (long)d.na * c->ldx, sd = (long)d.na * d.na, sk = (long)d.nb * d.na,
-                   sh = (long)d.nb * (d.nb + 1);
-        rocblas_status st = rocblas_status_success;
-        for (int which = 0; which < 2 && st == rocblas_status_success; which++)
-            st = cond_gemm_run(c->blas,
-                               tuned_gemm(c, which, c->Fx + e0 * sx, nullptr, c->dinv + e0 * sd, c->K + e0 * sk,
-                                          c->X + e0 * sx, c->Hx + e0 * sh, nullptr, n),
-                               c->gemm_sol[which]);
-        if (st != rocblas_status_success)
-            return hfem_fail(-2, "hfem_condense: rocBLAS strided-batched GEMM failed: %s", rocblas_status_to_string(st));
-    }
-    else {
-        // FP32 Gauss-Jordan inv32 (widened to FP64), X0 = inv32 [F Ru], then two sweeps
-        // X += inv32 ([F Ru] - D X) with the residual in FP64, then [AH FH] = [H Rh] - K X
+    c->last_sweeps = 0;
+    c->last_unconverged = 0;
+    c->last_fallback = 0;
+    if (precision == HFEM_FP64) {
+        if ((rc = condense_fp64(c, e0, n))) return rc;
+    } else {
+        // FP32 MFMA Gauss-Jordan inv32, X0 = inv32 [F Ru] on the FP32 MFMA, then
+        // sweeps R = [F Ru] - D X (FP64) and X += inv32 R (FP32 products, FP64
+        // update) until every element's estimated remaining error is below
+        // mix_tol |X| (at most mix_max_sweeps), then the FP64 Schur product.
+        // Elements that do not converge send the range through the FP64 path.
         if (!c->rwork) HIPCHK(hfem_dalloc(&c->rwork, (size_t)c->hd.ne * d.na * c->ldx));
-        HIPCHK(launch_dinv_fp32(d.na, e0, n, c->D, c->dinv, c->info, c->stream));
-        const long sx = (long)d.na * c->ldx, sd = (long)d.na * d.na, sk = (long)d.nb * d.na,
-                   sh = (long)d.nb * (d.nb + 1);
-        const double *Fx = c->Fx + e0 * sx, *Dv = c->D + e0 * sd, *Iv = c->dinv + e0 * sd;
-        double *X = c->X + e0 * sx, *Rw = c->rwork + e0 * sx;
-        const double *Kv = c->K + e0 * sk;
-        double *Hv = c->Hx + e0 * sh;
-        // X0, then per sweep R = [F Ru] - D X (the residual GEMM reads [F Ru] as its C
-        // operand and writes R, so no copy) and X += inv32 R, then the Schur product
-        const int seq[6] = {0, 2, 3, 2, 3, 1};
-        rocblas_status st = rocblas_status_success;
-        for (int i = 0; i < 6 && st == rocblas_status_success; i++)
-            st = cond_gemm_run(c->blas, tuned_gemm(c, seq[i], Fx, Dv, Iv, Kv, X, Hv, Rw, n), c->gemm_sol[seq[i]]);
-        if (st != rocblas_status_success)
-            return hfem_fail(-2, "hfem_condense: rocBLAS strided-batched GEMM failed: %s", rocblas_status_to_string(st));
+        float *inv32 = (float *)c->dinv;
+        HIPCHK(launch_dinv32(d.na, e0, n, c->D, inv32, c->info, c->stream, c->inv_var[1]));
+        HIPCHK(run_prod(c, 2, c->prod_var[2], e0, n, inv32, c->Fx, c->D, c->K, c->X, c->Hx, c->rwork));
+        int left = n;
+        for (int sweep = 1; sweep <= c->mix_max_sweeps && left > 0; sweep++) {
+            HIPCHK(run_prod(c, 3, c->prod_var[3], e0, n, inv32, c->Fx, c->D, c->K, c->X, c->Hx, c->rwork,
+                            c->gemm_sol[2]));
+            HIPCHK(launch_refine_reset(n, c->rdmax + e0, c->rcmax + e0, c->rcount, c->stream));
+            HIPCHK(run_prod(c, 4, c->prod_var[4], e0, n, inv32, c->Fx, c->D, c->K, c->X, c->Hx, c->rwork));
+            HIPCHK(launch_refine_count(n, c->rdmax + e0, c->rcmax + e0, c->rdprev + e0, sweep == 1, c->mix_tol,
+                                       c->rcount, c->stream));
+            HIPCHK(hipMemcpyAsync(&left, c->rcount, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            c->last_sweeps = sweep;
+        }
+        c->last_unconverged = left;
+        if (left > 0) {
+            c->last_fallback = 1;
+            if ((rc = condense_fp64(c, e0, n))) return rc;
+        } else {
+            HIPCHK(run_prod(c, 1, c->prod_var[1], e0, n, nullptr, c->Fx, c->D, c->K, c->X, c->Hx, nullptr,
+                            c->gemm_sol[1]));
+        }
     }
     HIPCHK(hipEventRecord(c->ev[3], c->stream));
+    return 0;
+}
+
+// Refinement report of the last hfem_condense: sweeps run, elements that did not
+// meet the tolerance, whether the range was redone in FP64 (mixed path only).
+extern "C" int hfem_condense_report(hfem_ctx *c, int *sweeps, int *unconverged, int *fallback)
+{
+    if (!c) return hfem_fail(-1, "hfem_condense_report: null context");
+    if (sweeps) *sweeps = c->last_sweeps;
+    if (unconverged) *unconverged = c->last_unconverged;
+    if (fallback) *fallback = c->last_fallback;
+    return 0;
+}
+
+// Mixed-path refinement parameters: tolerance on the estimated remaining error
+// relative to max |X| per element, and the sweep cap.
+extern "C" int hfem_set_refinement(hfem_ctx *c, double tol, int max_sweeps)
+{
+    if (!c) return hfem_fail(-1, "hfem_set_refinement: null context");
+    if (!(tol > 0.0) || max_sweeps < 1 || max_sweeps > 16)
+        return hfem_fail(-1, "hfem_set_refinement: need tol > 0 and 1 <= max_sweeps <= 16");
+    c->mix_tol = tol;
+    c->mix_max_sweeps = max_sweeps;
     return 0;
 }
 

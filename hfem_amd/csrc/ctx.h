@@ -19,9 +19,37 @@ hipError_t launch_assemble(const Dims &, const Master &, const Phys &, const dou
                            int, double *, double *, double *, double *, double *, double *, double *, double *,
                            hipStream_t);
 size_t flux_lds_bytes(const Dims &d);
-hipError_t launch_dinv(int na, int e0, int n, const double *D, double *Dinv, int *info, hipStream_t s);
-hipError_t launch_dinv_fp32(int na, int e0, int n, const double *D, double *Dinv, int *info, hipStream_t s);
-size_t condense_lds_bytes(int elt);
+hipError_t launch_dinv(int na, int e0, int n, const double *D, double *Dinv, int *info, hipStream_t s,
+                       int variant = 0);
+hipError_t launch_dinv32(int na, int e0, int n, const double *D, float *Dinv, int *info, hipStream_t s,
+                         int variant = 0);
+constexpr int kDinvVariants = 3;
+hipError_t launch_refine_reset(int n, unsigned long long *dmax, unsigned long long *cmax, int *count, hipStream_t s);
+hipError_t launch_refine_count(int n, const unsigned long long *dmax, const unsigned long long *cmax,
+                               unsigned long long *dprev, int first, double tol, int *count, hipStream_t s);
+// bgemm.hip: batched per-element products (see the file's header for the layout)
+struct BgemmArgs {
+    int M, N, K;
+    const void *A;
+    long sa;
+    int lda;
+    const void *B;
+    long sb;
+    int ldb;
+    double alpha, beta;
+    const double *C0;
+    long sc0;
+    int ldc0;
+    double *C;
+    long sc;
+    int ldc;
+    int e0, n;                        // element range
+    int nstrip;                       // strips per element (set by the launcher)
+    unsigned long long *dmax, *cmax;  // optional per-element maxima (index e - e0)
+};
+int bgemm_num_variants();
+bool bgemm_fits(int mode, int variant, int M, int N, size_t maxlds);
+hipError_t launch_bgemm(int mode, int variant, BgemmArgs g, size_t maxlds, hipStream_t s);
 hipError_t launch_schur_vector(int na, int nb, int ldx, int ne, int fp32, const void *Dinv, const void *K,
                                const double *Ru, const double *Rh, double *X, double *Hx, hipStream_t s);
 hipError_t launch_to_f32(long n, const double *x, float *y, hipStream_t s);
@@ -137,6 +165,17 @@ struct hfem_ctx {
     // rocBLAS solution indices of the condensation GEMMs (0: rocBLAS's own choice):
     // X = inv(D)[F Ru], the Schur product, the mixed residual and update
     int gemm_sol[4] = {0, 0, 0, 0};
+    // condensation launch choices (hfem_tune_condensation): inverse variants (FP64,
+    // FP32) and, per product (0 X = inv(D)[F Ru], 1 Schur, 2 mixed X0, 3 mixed
+    // residual, 4 mixed correction), the hand-written variant (>= 0) or rocBLAS (-1)
+    int inv_var[2] = {0, 0};
+    int prod_var[5] = {0, 0, 0, 0, 0};
+    // mixed refinement: per-element maxima, previous correction, unconverged count
+    unsigned long long *rdmax = nullptr, *rcmax = nullptr, *rdprev = nullptr;
+    int *rcount = nullptr;
+    int mix_max_sweeps = 4;
+    double mix_tol = 1e-12;
+    int last_sweeps = 0, last_unconverged = 0, last_fallback = 0;
     long fstride = 0;
     int chunk = 0;
     int grid_asm = 0, grid_cond = 0;

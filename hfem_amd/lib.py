@@ -51,6 +51,10 @@ EXPORTS = {
     "hfem_assemble": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "hfem_condense": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "hfem_tune_condensation": (ctypes.c_int, [_vp, ctypes.c_int, _ip, ctypes.POINTER(ctypes.c_float)]),
+    "hfem_set_refinement": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int]),
+    "hfem_condense_report": (ctypes.c_int, [_vp, _ip, _ip, _ip]),
+    "hfem_set_condensation_choice": (ctypes.c_int, [_vp, _ip]),
+    "hfem_condense_variants": (ctypes.c_int, []),
     "hfem_read_elements": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
     "hfem_read_condensed": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _ip]),
     "hfem_assemble_linear_system": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
@@ -202,14 +206,33 @@ class HDGContext:
         _check(load().hfem_condense(self.h, e0, n, precision))
 
     def tune_condensation(self, nsample=0):
-        """Pick the rocBLAS solutions of the condensation's batched GEMMs
-        (inv(D)[F Ru], the Schur product, the mixed refinement residual and
-        update); returns (solutions[4], ms[8]) with ms = (default, picked) per
-        GEMM on the sample."""
-        sol = (ctypes.c_int * 4)()
-        ms = (ctypes.c_float * 8)()
-        _check(load().hfem_tune_condensation(self.h, nsample, sol, ms))
-        return list(sol), list(ms)
+        """Time the launch candidates of every condensation stage on a sample and
+        keep the fastest; returns (choice[7], ms[7]) for {inverse FP64, inverse
+        FP32, X = inv(D)[F Ru], Schur product, mixed X0, mixed residual, mixed
+        correction} (product choice: hand-written variant >= 0, -1 rocBLAS)."""
+        ch = (ctypes.c_int * 7)()
+        ms = (ctypes.c_float * 7)()
+        _check(load().hfem_tune_condensation(self.h, nsample, ch, ms))
+        return list(ch), list(ms)
+
+    def set_condensation_choice(self, choice, strict=True):
+        """Set the seven launch choices (the tuner's `choice` layout).  With
+        strict=False a choice that does not fit returns False instead of raising."""
+        ch = (ctypes.c_int * 7)(*choice)
+        rc = load().hfem_set_condensation_choice(self.h, ch)
+        if rc != 0 and not strict:
+            return False
+        _check(rc)
+        return True
+
+    def set_refinement(self, tol=1e-12, max_sweeps=4):
+        _check(load().hfem_set_refinement(self.h, tol, max_sweeps))
+
+    def condense_report(self):
+        """(sweeps, unconverged elements, fallback to FP64) of the last condensation."""
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(load().hfem_condense_report(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
 
     def read_elements(self, e0=0, n=None, which=("Ru", "D", "F", "K", "H", "Rh")):
         d = self.d

@@ -80,15 +80,33 @@ int hfem_assemble(hfem_ctx *ctx, int e0, int n);
 /* Static condensation of elements [e0, e0+n) on the device blocks: inv(D)F,
  * inv(D)Ru, AH (in place of H) and FH; D, Ru and Rh keep their assembled values. */
 int hfem_condense(hfem_ctx *ctx, int e0, int n, int precision);
-/* No counterpart in the reference (it calls MKL's DGEMM per element): picks, for
- * each batched GEMM of hfem_condense -- 0: inv(D)[F Ru], 1: [H Rh] - K X (both
- * precisions), 2: the mixed path's refinement residual [F Ru] - D X, 3: its
- * update X += inv32 R -- the fastest rocBLAS solution on a sample of nsample
- * elements (0: min(ne, 32768)) and keeps it in the context.  Element blocks are
- * not changed.  solutions[4] (may be NULL) receives the picks (0 = rocBLAS's own
- * choice), ms[8] (may be NULL) the default and the picked time in milliseconds
- * on the sample for each GEMM in turn. */
-int hfem_tune_condensation(hfem_ctx *ctx, int nsample, int *solutions, float *ms);
+/* Mixed path (HFEM_MIXED): inv(D) by Gauss-Jordan on the FP32 MFMA, X0 = inv32 [F Ru]
+ * on the FP32 MFMA, then sweeps R = [F Ru] - D X (FP64) and X += inv32 R until the
+ * estimated remaining error of every element is below tol max|X| (at most
+ * max_sweeps; defaults 1e-12 and 4), then the FP64 Schur product.  A range with
+ * an element that does not converge is condensed again by the FP64 path.  No
+ * counterpart in the reference (its FP32 copies are storage only,
+ * assembly/globalassembly.cpp:353-371). */
+int hfem_set_refinement(hfem_ctx *ctx, double tol, int max_sweeps);
+/* Report of the last hfem_condense: refinement sweeps run, elements left above the
+ * tolerance, 1 if the range was redone in FP64 (all 0 after an FP64 condensation). */
+int hfem_condense_report(hfem_ctx *ctx, int *sweeps, int *unconverged, int *fallback);
+/* No counterpart in the reference (it calls LAPACK/BLAS per element): times the
+ * launch candidates of every stage of hfem_condense on a sample of nsample
+ * elements (0: min(ne, 32768)) and keeps the fastest in the context -- the
+ * inverse kernels' launch shapes (FP64, FP32) and, per product (0 X = inv(D)[F Ru],
+ * 1 [H Rh] - K X, 2 mixed X0, 3 mixed residual [F Ru] - D X, 4 mixed correction
+ * X += inv32 R), the hand-written MFMA variants that fit plus, for the FP64
+ * products 0, 1 and 3, rocBLAS (its own choice and each solution it lists) as the
+ * measured comparison.  Element blocks are not changed.  choice[7] (may be NULL)
+ * receives {inverse FP64, inverse FP32, product 0..4} (product: variant >= 0, -1
+ * rocBLAS), ms[7] (may be NULL) the best time of each in milliseconds. */
+int hfem_tune_condensation(hfem_ctx *ctx, int nsample, int *choice, float *ms);
+/* Set the seven launch choices in hfem_tune_condensation's layout (a pick kept from
+ * an earlier tuning); an error if a choice cannot run this context's shapes. */
+int hfem_set_condensation_choice(hfem_ctx *ctx, const int *choice);
+/* Number of hand-written product variants (valid product choices 0 .. n-1). */
+int hfem_condense_variants(void);
 
 /* Copy the assembled element blocks of [e0, e0+n) to the host (any pointer may
  * be NULL), each element's block contiguous and column-major like elemstruct

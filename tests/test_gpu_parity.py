@@ -147,9 +147,9 @@ def test_linear_system_and_matvec(oracle_mod, kind, nc, p):
 
 
 def test_tuned_condensation(oracle_mod):
-    """The rocBLAS solutions picked by hfem_tune_condensation give the same blocks
-    (1e-13 FP64, 1e-10 mixed; a different summation order is allowed) and tuning
-    changes no element block."""
+    """Every launch candidate the tuner can pick gives the same blocks (1e-13
+    FP64, 1e-10 mixed; a different summation order is allowed), the tuner's pick
+    too, and tuning changes no element block."""
     from hfem_amd.lib import HDGContext, FP64, MIXED
     O = oracle_mod
     pr = make_problem("tet", 2, 3, seed=6)
@@ -160,20 +160,37 @@ def test_tuned_condensation(oracle_mod):
     ctx = HDGContext(pr)
     ctx.assemble_element_matrix_vector()
     before = ctx.read_elements()
-    sols, ms = ctx.tune_condensation()
-    say(f"  solutions {sols} ms {ms}")
+    choice, ms = ctx.tune_condensation()
+    say(f"  choice {choice} ms {ms}")
     after = ctx.read_elements()
     for k in before:
         assert np.array_equal(before[k], after[k]), k
-    assert len(sols) == 4 and all(ms[2 * i + 1] <= ms[2 * i] for i in range(4))
-    for prec, tol in ((FP64, RTOL_MAT), (MIXED, RTOL_MIXED)):
-        ctx.assemble_element_matrix_vector()
-        ctx.schur_element_matrix_vector(precision=prec)
-        c = ctx.read_condensed()
-        for name, want in (("DinvF", ref["F"]), ("AH", ref["H"]), ("FH", ref["Rh"])):
-            e = relerr(c[name], want)
-            say(f"  tuned precision {prec} {name}: {e:.2e}")
-            assert e < tol, name
+    assert len(choice) == 7 and all(m > 0 for m in ms)
+
+    def check(tag):
+        for prec, tol in ((FP64, RTOL_MAT), (MIXED, RTOL_MIXED)):
+            ctx.assemble_element_matrix_vector()
+            ctx.schur_element_matrix_vector(precision=prec)
+            c = ctx.read_condensed()
+            for name, want in (("DinvF", ref["F"]), ("AH", ref["H"]), ("FH", ref["Rh"])):
+                e = relerr(c[name], want)
+                say(f"  {tag} precision {prec} {name}: {e:.2e}")
+                assert e < tol, (tag, name)
+            if prec == MIXED:
+                sweeps, left, fb = ctx.condense_report()
+                say(f"  {tag} mixed sweeps {sweeps} unconverged {left} fallback {fb}")
+                assert left == 0 and fb == 0 and 1 <= sweeps <= 4
+
+    check("tuned")
+    # every hand-written variant of every product, and every inverse launch shape
+    from hfem_amd.lib import load
+    for v in range(load().hfem_condense_variants()):
+        pick = [v % 3, v % 3] + [v] * 5
+        if ctx.set_condensation_choice(pick, strict=False):
+            check(f"variant {v}")
+    # rocBLAS for the FP64 products that have it
+    if ctx.set_condensation_choice([0, 0, -1, -1, 0, -1, 0], strict=False):
+        check("rocBLAS")
     ctx.close()
 
 
