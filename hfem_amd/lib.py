@@ -55,6 +55,7 @@ EXPORTS = {
     "hfem_condense_report": (ctypes.c_int, [_vp, _ip, _ip, _ip]),
     "hfem_set_condensation_choice": (ctypes.c_int, [_vp, _ip]),
     "hfem_condense_variants": (ctypes.c_int, []),
+    "hfem_condense_noalloc": (ctypes.c_int, [ctypes.c_int] * 4 + [ctypes.c_void_p] * 7),
     "hfem_read_elements": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
     "hfem_read_condensed": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _ip]),
     "hfem_assemble_linear_system": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),

@@ -108,6 +108,21 @@ int hfem_set_condensation_choice(hfem_ctx *ctx, const int *choice);
 /* Number of hand-written product variants (valid product choices 0 .. n-1). */
 int hfem_condense_variants(void);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * The FP64 condensation of hfem_condense on caller-owned device blocks of n
+ * elements in the library's internal layouts -- D (na x na column-major), [F Ru]
+ * (na x ldx row-major, Ru in column nb), K (nb x na column-major), [H Rh]
+ * (nb x (nb+1) column-major, Rh in column nb) -- writing X = [inv(D)F inv(D)Ru]
+ * (na x ldx row-major) and [AH FH] over [H Rh], with info[n] the singular-pivot
+ * flags, on `stream` (a hipStream_t; NULL the default stream).  It allocates
+ * nothing: inv(D) lives in a fixed 32 MiB device array per GPU, so n*na*na must
+ * not exceed 4 Mi doubles (419 elements of na = 100; larger batches are refused
+ * with an error -- call it per range).  Calls that overlap on one GPU share that
+ * array and race. */
+int hfem_condense_noalloc(int na, int nb, int ldx, int n, const double *D, const double *Fx, const double *K,
+                          double *Hx, double *X, int *info, void *stream);
+
 /* Copy the assembled element blocks of [e0, e0+n) to the host (any pointer may
  * be NULL), each element's block contiguous and column-major like elemstruct
  * (Ru na, D na*na, F na*nb, K nb*na, H nb*nb, Rh nb).  H is overwritten by

@@ -42,7 +42,7 @@ def test_error_path_without_device():
 def test_tuner_rejects_a_null_context():
     from hfem_amd import lib
     L = lib.load()
-    sol = (ctypes.c_int * 4)()
-    ms = (ctypes.c_float * 8)()
+    sol = (ctypes.c_int * 7)()
+    ms = (ctypes.c_float * 7)()
     rc = L.hfem_tune_condensation(None, 0, sol, ms)
     assert rc < 0 and b"null context" in L.hfem_last_error()
