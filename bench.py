@@ -290,15 +290,21 @@ def main():
     ctx = HDGContext(prob)
     f_asm = ASSEMBLY_FLOPS[(3, 0, 3)]
     f_cond = condense_flops(na, nb)
-    # rocBLAS solutions of the FP64 condensation GEMMs, picked on a sample before timing
-    gemm_sols, gemm_ms = ctx.tune_condensation()
-    print(f"[bench] condensation GEMM solutions {gemm_sols}: ms default/picked {gemm_ms}", file=sys.stderr, flush=True)
+    # launch choices of every condensation stage, timed on a sample before the timed steps
+    choice, tune_ms = ctx.tune_condensation()
+    stages = ["inverse_fp64", "inverse_fp32", "x_fp64", "schur_fp64", "x0_mixed", "residual_fp64", "correction_fp32"]
+    tuning = {st: {"choice": ("fused" if c == 64 else "rocblas" if c < 0 else f"variant {c}"), "ms": m}
+              for st, c, m in zip(stages, choice, tune_ms)}
+    print(f"[bench] condensation tuning: {tuning}", file=sys.stderr, flush=True)
     res = {}
     precs = {"fp64": FP64, "mixed": MIXED}
     todo = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
     for name in todo:
         wall, a_ms, c_ms = run_steps(ctx, ne, precs[name], args.steps, args.warmup)
         res[name] = dict(wall=wall, asm_ms=a_ms, cond_ms=c_ms, eps=ne * args.steps / wall)
+        if name == "mixed":
+            sw, left, fb = ctx.condense_report()
+            res[name]["refinement"] = {"sweeps": sw, "unconverged_elements": left, "fp64_fallback": bool(fb)}
         print(f"[bench] {name}: {res[name]['eps']:.1f} elements/s  assemble {a_ms:.3f} ms  condense {c_ms:.3f} ms",
               file=sys.stderr, flush=True)
 
@@ -327,7 +333,7 @@ def main():
     step_s = r["wall"] / args.steps
     # dominant kernel and its roofline (achieved = algorithmic flops / average launch time)
     if r["cond_ms"] >= r["asm_ms"]:
-        kname = "dinv_kernel_fp32+dgemm_strided_batched" if head == "mixed" else "dinv_mfma_kernel+dgemm_strided_batched"
+        kname = "condensation_mixed" if head == "mixed" else "condensation_fp64"
         kms, kflops = r["cond_ms"], f_cond
     else:
         kname, kms, kflops = "assemble_kernel+flux_kernel", r["asm_ms"], f_asm
@@ -353,7 +359,11 @@ def main():
         "mixed_speedup_vs_fp64": (res["mixed"]["eps"] / res["fp64"]["eps"]) if len(res) == 2 else None,
         "mixed_max_rel_err_vs_fp64": errs or None,
         "kernel_ms": {k: {"assemble": v["asm_ms"], "condense": v["cond_ms"]} for k, v in res.items()},
-        "condense_gemm_tuning": {"solutions": gemm_sols, "sample_ms_default_picked": gemm_ms},
+        "condense_tuning": tuning,
+        "mixed_refinement": res.get("mixed", {}).get("refinement"),
+        "condense_frac_of_peak": {k: {"fp64": f_cond * ne / (v["cond_ms"] * 1e-3) / PEAK_FP64,
+                                      "fp32": f_cond * ne / (v["cond_ms"] * 1e-3) / PEAK_FP32}
+                                  for k, v in res.items()},
         "roofline": {"bound": "mfma", "kernel": kname, "achieved": achieved / 1e12, "peak": peak / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / peak, "traffic": hbm_traffic(kname),
                      "flops_per_element": kflops, "elements_per_launch": ne,

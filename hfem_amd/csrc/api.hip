@@ -430,12 +430,26 @@ static BgemmArgs prod_args(const hfem_ctx *c, int which, int e0, int n, const vo
     return g;
 }
 
+// Choice value of products 0 and 2 for the fused per-element solve (xsolve.hip):
+// product 0 -> X = inv(D)[F Ru] with inv(D) in LDS; product 2 -> X0 and every
+// refinement sweep in one kernel (products 3 and 4 then unused).
+static constexpr int kFused = 64;
+
 // rocBLAS form of products 0, 1, 3 (the measured comparison; tuned_gemm numbering)
 static int prod_rocblas_index(int which) { return which == 0 ? 0 : which == 1 ? 1 : which == 3 ? 2 : -1; }
 
 static hipError_t run_prod(hfem_ctx *c, int which, int var, int e0, int n, const void *inv, const double *Fx,
                            const double *Dm, const double *K, double *X, double *Hx, double *R, int sol = 0)
 {
+    if (var == kFused) {
+        const Dims &d = c->d;
+        if (which == 0)
+            return launch_xsolve(0, d.na, d.nb + 1, c->ldx, e0, n, Dm, inv, Fx, X, 0, 0, nullptr, nullptr, c->stream);
+        if (which == 2)   // X0 and one sweep
+            return launch_xsolve(1, d.na, d.nb + 1, c->ldx, e0, n, Dm, inv, Fx, X, 1, 0, c->rdmax + e0, c->rcmax + e0,
+                                 c->stream);
+        return hipErrorInvalidValue;
+    }
     if (var < 0) {
         const int rw = prod_rocblas_index(which);
         if (rw < 0) return hipErrorInvalidValue;
@@ -462,6 +476,8 @@ static void default_choices(hfem_ctx *c)
         for (int v = 0; v < bgemm_num_variants(); v++)
             if (bgemm_fits(prod_mode(w), v, M, N, (size_t)c->maxlds)) { c->prod_var[w] = v; break; }
     }
+    for (int w = 0; w <= 2; w += 2)
+        if (xsolve_fits(w == 0 ? 0 : 1, d.na, (size_t)c->maxlds)) c->prod_var[w] = kFused;
 }
 
 // Times the launch candidates of every stage of the condensation on a sample of
@@ -534,6 +550,7 @@ extern "C" int hfem_tune_condensation(hfem_ctx *c, int nsample, int *choice, flo
                 hipMemsetAsync(R2, 0, (size_t)ns * d.na * c->ldx * 8, c->stream) != hipSuccess ||
                 launch_refine_reset(ns, c->rdmax, c->rcmax, c->rcount, c->stream) != hipSuccess))
         rc = hfem_fail(-2, "hfem_tune_condensation: setup failed");
+    float fused_mixed_ms = -1.f;
     for (int w = 0; w < 5 && !rc; w++) {
         const int M = w == 1 ? d.nb + 1 : d.na, N = w == 1 ? d.nb : d.nb + 1;
         const void *inv = prod_mode(w) ? (const void *)It32 : (const void *)It;
@@ -543,6 +560,14 @@ extern "C" int hfem_tune_condensation(hfem_ctx *c, int nsample, int *choice, flo
             if (!bgemm_fits(prod_mode(w), v, M, N, (size_t)c->maxlds)) continue;
             const float t = time_it([&]() { return run_prod(c, w, v, 0, ns, inv, c->Fx, c->D, c->K, Xt, Ht, R2); });
             if (t >= 0 && t < bt) { bt = t; bv = v; }
+        }
+        if ((w == 0 || w == 2) && xsolve_fits(w == 0 ? 0 : 1, d.na, (size_t)c->maxlds)) {
+            // the fused solve; for the mixed path it runs X0 and one sweep, so it
+            // competes with X0 alone here and is weighed against the separate
+            // sweep products below
+            const float t = time_it([&]() { return run_prod(c, w, kFused, 0, ns, inv, c->Fx, c->D, c->K, Xt, Ht, R2); });
+            if (w == 0 && t >= 0 && t < bt) { bt = t; bv = kFused; }
+            if (w == 2) fused_mixed_ms = t;
         }
         const int rw = prod_rocblas_index(w);
         if (rw >= 0) {
@@ -561,6 +586,12 @@ extern "C" int hfem_tune_condensation(hfem_ctx *c, int nsample, int *choice, flo
         if (bv == -2) { rc = hfem_fail(-2, "hfem_tune_condensation: no launch candidate runs product %d", w); break; }
         c->prod_var[w] = bv;
         best[2 + w] = bt;
+    }
+    // the mixed solve: the fused kernel (X0 + one sweep) against X0, residual and
+    // correction as separate products
+    if (!rc && fused_mixed_ms >= 0 && fused_mixed_ms < best[4] + best[5] + best[6]) {
+        c->prod_var[2] = kFused;
+        best[4] = fused_mixed_ms;
     }
     (void)hipStreamSynchronize(c->stream);
     if (!rc) {
@@ -595,6 +626,7 @@ extern "C" int hfem_set_condensation_choice(hfem_ctx *c, const int *choice)
     for (int w = 0; w < 5; w++) {
         const int v = choice[2 + w], M = w == 1 ? d.nb + 1 : d.na, N = w == 1 ? d.nb : d.nb + 1;
         if (v == -1 && prod_rocblas_index(w) >= 0) continue;
+        if (v == kFused && (w == 0 || w == 2) && xsolve_fits(w == 0 ? 0 : 1, d.na, (size_t)c->maxlds)) continue;
         if (!bgemm_fits(prod_mode(w), v, M, N, (size_t)c->maxlds))
             return hfem_fail(-1, "hfem_set_condensation_choice: product %d cannot run variant %d", w, v);
     }
@@ -634,16 +666,23 @@ extern "C" int hfem_condense(hfem_ctx *c, int e0, int n, int precision)
         // update) until every element's estimated remaining error is below
         // mix_tol |X| (at most mix_max_sweeps), then the FP64 Schur product.
         // Elements that do not converge send the range through the FP64 path.
-        if (!c->rwork) HIPCHK(hfem_dalloc(&c->rwork, (size_t)c->hd.ne * d.na * c->ldx));
+        if (!c->rwork && c->prod_var[2] != kFused) HIPCHK(hfem_dalloc(&c->rwork, (size_t)c->hd.ne * d.na * c->ldx));
         float *inv32 = (float *)c->dinv;
         HIPCHK(launch_dinv32(d.na, e0, n, c->D, inv32, c->info, c->stream, c->inv_var[1]));
-        HIPCHK(run_prod(c, 2, c->prod_var[2], e0, n, inv32, c->Fx, c->D, c->K, c->X, c->Hx, c->rwork));
+        const bool fused = c->prod_var[2] == kFused;
+        if (!fused) HIPCHK(run_prod(c, 2, c->prod_var[2], e0, n, inv32, c->Fx, c->D, c->K, c->X, c->Hx, c->rwork));
         int left = n;
         for (int sweep = 1; sweep <= c->mix_max_sweeps && left > 0; sweep++) {
-            HIPCHK(run_prod(c, 3, c->prod_var[3], e0, n, inv32, c->Fx, c->D, c->K, c->X, c->Hx, c->rwork,
-                            c->gemm_sol[2]));
             HIPCHK(launch_refine_reset(n, c->rdmax + e0, c->rcmax + e0, c->rcount, c->stream));
-            HIPCHK(run_prod(c, 4, c->prod_var[4], e0, n, inv32, c->Fx, c->D, c->K, c->X, c->Hx, c->rwork));
+            if (fused) {
+                // X0 and the first sweep, or one more sweep from the current X
+                HIPCHK(launch_xsolve(1, d.na, d.nb + 1, c->ldx, e0, n, c->D, inv32, c->Fx, c->X, 1, sweep > 1,
+                                     c->rdmax + e0, c->rcmax + e0, c->stream));
+            } else {
+                HIPCHK(run_prod(c, 3, c->prod_var[3], e0, n, inv32, c->Fx, c->D, c->K, c->X, c->Hx, c->rwork,
+                                c->gemm_sol[2]));
+                HIPCHK(run_prod(c, 4, c->prod_var[4], e0, n, inv32, c->Fx, c->D, c->K, c->X, c->Hx, c->rwork));
+            }
             HIPCHK(launch_refine_count(n, c->rdmax + e0, c->rcmax + e0, c->rdprev + e0, sweep == 1, c->mix_tol,
                                        c->rcount, c->stream));
             HIPCHK(hipMemcpyAsync(&left, c->rcount, sizeof(int), hipMemcpyDeviceToHost, c->stream));

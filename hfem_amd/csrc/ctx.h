@@ -48,6 +48,11 @@ struct BgemmArgs {
     unsigned long long *dmax, *cmax;  // optional per-element maxima (index e - e0)
 };
 int bgemm_num_variants();
+// xsolve.hip: fused per-element solve X = inv(D)[F Ru] (mode 0 FP64, mode 1 mixed)
+bool xsolve_fits(int mode, int na, size_t maxlds);
+hipError_t launch_xsolve(int mode, int na, int N, int ldx, int e0, int n, const double *D, const void *inv,
+                         const double *Fx, double *X, int nsweep, int from_x, unsigned long long *dmax,
+                         unsigned long long *cmax, hipStream_t s);
 bool bgemm_fits(int mode, int variant, int M, int N, size_t maxlds);
 hipError_t launch_bgemm(int mode, int variant, BgemmArgs g, size_t maxlds, hipStream_t s);
 hipError_t launch_schur_vector(int na, int nb, int ldx, int ne, int fp32, const void *Dinv, const void *K,
@@ -174,7 +179,7 @@ struct hfem_ctx {
     unsigned long long *rdmax = nullptr, *rcmax = nullptr, *rdprev = nullptr;
     int *rcount = nullptr;
     int mix_max_sweeps = 4;
-    double mix_tol = 1e-12;
+    double mix_tol = 1e-11;
     int last_sweeps = 0, last_unconverged = 0, last_fallback = 0;
     long fstride = 0;
     int chunk = 0;

@@ -226,7 +226,7 @@ class HDGContext:
         _check(rc)
         return True
 
-    def set_refinement(self, tol=1e-12, max_sweeps=4):
+    def set_refinement(self, tol=1e-11, max_sweeps=4):
         _check(load().hfem_set_refinement(self.h, tol, max_sweeps))
 
     def condense_report(self):

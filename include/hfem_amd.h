@@ -83,7 +83,7 @@ int hfem_condense(hfem_ctx *ctx, int e0, int n, int precision);
 /* Mixed path (HFEM_MIXED): inv(D) by Gauss-Jordan on the FP32 MFMA, X0 = inv32 [F Ru]
  * on the FP32 MFMA, then sweeps R = [F Ru] - D X (FP64) and X += inv32 R until the
  * estimated remaining error of every element is below tol max|X| (at most
- * max_sweeps; defaults 1e-12 and 4), then the FP64 Schur product.  A range with
+ * max_sweeps; defaults 1e-11 and 4), then the FP64 Schur product.  A range with
  * an element that does not converge is condensed again by the FP64 path.  No
  * counterpart in the reference (its FP32 copies are storage only,
  * assembly/globalassembly.cpp:353-371). */

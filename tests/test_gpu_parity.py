@@ -188,6 +188,9 @@ def test_tuned_condensation(oracle_mod):
         pick = [v % 3, v % 3] + [v] * 5
         if ctx.set_condensation_choice(pick, strict=False):
             check(f"variant {v}")
+    # the fused per-element solves (FP64 X and the mixed X0 + sweeps)
+    if ctx.set_condensation_choice([0, 0, 64, 0, 64, 0, 0], strict=False):
+        check("fused solve")
     # rocBLAS for the FP64 products that have it
     if ctx.set_condensation_choice([0, 0, -1, -1, 0, -1, 0], strict=False):
         check("rocBLAS")

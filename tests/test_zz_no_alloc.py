@@ -37,35 +37,64 @@ def internal_blocks(ctx, n):
     return ldx, D, Fx.reshape(n, -1), K, Hx
 
 
+class DeviceBuffers:
+    """Device memory through the HIP runtime the library is linked against (the
+    test's own plumbing; hfem_condense_noalloc itself allocates nothing)."""
+
+    def __init__(self):
+        self.hip = ctypes.CDLL("libamdhip64.so.7")
+        self.hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self.hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        self.hip.hipFree.argtypes = [ctypes.c_void_p]
+        self.ptrs = []
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        p = ctypes.c_void_p()
+        assert self.hip.hipMalloc(ctypes.byref(p), a.nbytes) == 0
+        self.ptrs.append(p)
+        assert self.hip.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0   # host to device
+        return p
+
+    def download(self, p, like):
+        out = np.empty_like(like)
+        assert self.hip.hipDeviceSynchronize() == 0
+        assert self.hip.hipMemcpy(out.ctypes.data, p, out.nbytes, 2) == 0   # device to host
+        return out
+
+    def free(self):
+        for p in self.ptrs:
+            self.hip.hipFree(p)
+        self.ptrs = []
+
+
 def test_condense_noalloc_matches_condense():
-    import torch
     from hfem_amd.lib import HDGContext, FP64, load
     L = load()
     pr = make_problem("tet", 2, 3, seed=11)
     d = pr.dims
     na, nb = d["na"], d["nb"]
     ctx = HDGContext(pr)
+    buf = DeviceBuffers()
     for n in (pr.ne, pr.ne // 3):
         print(f"noalloc: tet p3 na={na} nb={nb} n={n}", flush=True)
         ctx.assemble_element_matrix_vector()
         ldx, D, Fx, K, Hx = internal_blocks(ctx, n)
         ctx.schur_element_matrix_vector(0, n, FP64)
         want = ctx.read_condensed(0, n)
-        dev = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in
-               dict(D=D, Fx=Fx, K=K, Hx=Hx).items()}
-        X = torch.full((n, na * ldx), float("nan"), dtype=torch.float64, device="cuda")
-        info = torch.full((n,), -7, dtype=torch.int32, device="cuda")
-        torch.cuda.synchronize()
-        rc = L.hfem_condense_noalloc(na, nb, ldx, n, dev["D"].data_ptr(), dev["Fx"].data_ptr(),
-                                     dev["K"].data_ptr(), dev["Hx"].data_ptr(), X.data_ptr(), info.data_ptr(),
-                                     ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+        dD, dF, dK, dH = (buf.upload(a) for a in (D, Fx, K, Hx))
+        X0 = np.full((n, na * ldx), np.nan)
+        dX = buf.upload(X0)
+        dI = buf.upload(np.full(n, -7, dtype=np.int32))
+        rc = L.hfem_condense_noalloc(na, nb, ldx, n, dD, dF, dK, dH, dX, dI, None)
         assert rc == 0, L.hfem_last_error()
-        torch.cuda.synchronize()
-        Xh = X.cpu().numpy().reshape(n, na, ldx)
-        Hh = dev["Hx"].cpu().numpy()
+        Xh = buf.download(dX, X0).reshape(n, na, ldx)
+        Hh = buf.download(dH, Hx)
+        info = buf.download(dI, np.zeros(n, dtype=np.int32))
+        buf.free()
         got = dict(DinvF=Xh[:, :, :nb].transpose(0, 2, 1).reshape(n, -1), DinvRu=Xh[:, :, nb],
                    AH=Hh[:, :nb * nb], FH=Hh[:, nb * nb:])
-        assert np.all(info.cpu().numpy() == 0)
+        assert np.all(info == 0)
         for k in ("DinvF", "DinvRu", "AH", "FH"):
             e = relerr(got[k], want[k])
             print(f"  {k}: {e:.2e}", flush=True)
