@@ -302,7 +302,7 @@ def app_settings(app):
     1 subdomain block Jacobi are BILU0 on one rank, 2 entity-wise block Jacobi) or
     none when flag[17] is -1, orthogonalisation flag[19], ordering flag[20] (MDF
     when nonzero), ordering reuse flag[10], quasi-Newton flag[15] with storage
-    accuracy flag[18]."""
+    accuracy flag[18], preconditioner precision flag[24] (precPrecision, :201)."""
     flag, fac, prob, sp = app["flag"], app["factor"], app["problem"], app["solversparam"]
 
     def at(a, i, default):
@@ -314,7 +314,7 @@ def app_settings(app):
                   precond=-1 if int(at(flag, 17, 0)) == -1 else (2 if prec == 2 else 0),
                   orthog=int(at(flag, 19, 1)), ordering=1 if int(at(flag, 20, 1)) != 0 else 0,
                   reuse_ordering=int(at(flag, 10, 1)), quasi_newton=int(at(flag, 15, 0)),
-                  qn_accuracy=int(at(flag, 18, 1)))
+                  qn_accuracy=int(at(flag, 18, 1)), prec_precision=1 if int(at(flag, 24, 1)) != 0 else 0)
     return dict(physicsparam=app["physicsparam"], fc_u=float(at(fac, 0, 0.0)), fc_q=float(at(fac, 1, 1.0)),
                 tdep=int(at(flag, 0, 0)), dirk_order=int(at(prob, 3, 0)), dirk_stages=int(at(prob, 4, 0)),
                 dt=app["dt"], solver_params=params)

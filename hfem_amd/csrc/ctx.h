@@ -85,6 +85,15 @@ hipError_t launch_bilu0_lsolve_level(int bsz, const int *rowpts, const int *coli
 hipError_t launch_bilu0_usolve_level(int bsz, const int *rowpts, const int *colind, const int *u2o, const int *rows,
                                      int nlev, const double *Mg, const double *y, double *v, hipStream_t s);
 hipError_t launch_block_jacobi(int nrows, int bsz, const double *Dinv, const double *x, double *y, hipStream_t s);
+// single-precision preconditioner factors (FP32 storage, FP32 MFMA factorisation)
+hipError_t launch_bilu0_factor_level(int bsz, const int *rowpts, const int *colind, const int *u2o, const int *rows,
+                                     int nlev, float *Mg, int *info, hipStream_t s);
+hipError_t launch_bilu0_lsolve_level(int bsz, const int *rowpts, const int *colind, const int *u2o, const int *rows,
+                                     int nlev, const float *Mg, const double *x, double *y, hipStream_t s);
+hipError_t launch_bilu0_usolve_level(int bsz, const int *rowpts, const int *colind, const int *u2o, const int *rows,
+                                     int nlev, const float *Mg, const double *y, double *v, hipStream_t s);
+hipError_t launch_block_jacobi(int nrows, int bsz, const float *Dinv, const double *x, double *y, hipStream_t s);
+hipError_t launch_to_float(long n, const double *x, float *y, hipStream_t s);
 int multidot_blocks(long n);
 hipError_t launch_multidot(long n, int k, const double *V, long ldv, const double *w, double *partial, double *out,
                            hipStream_t s);
@@ -120,6 +129,8 @@ struct SolverState {
     std::vector<int> lptr, uptr;        // level pointers of the L (forward) and U (backward) sweeps
     int *d_lrows = nullptr, *d_urows = nullptr;  // rows of each level, concatenated
     double *Mg = nullptr;               // BILU0 factors (nblocks blocks) or block-Jacobi inverses (nf blocks)
+    float *Mg32 = nullptr;              // the same in single precision (precPrecision 0)
+    int prec_fp32 = 0;
     double *Dinv = nullptr;             // inverted diagonal blocks of Hg (MDF weights)
     double *Cw = nullptr;               // MDF block weights ||inv(H_ii) H_ij||_F
     int *d_info = nullptr;

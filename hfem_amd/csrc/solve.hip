@@ -251,15 +251,32 @@ extern "C" int hfem_precond_factor(hfem_ctx *c, int kind)
     }
     if ((size_t)(kind == 0 ? bilu0_factor_lds(bsz) : block_inverse_lds(bsz)) > (size_t)c->maxlds)
         return hfem_fail(-1, "hfem_precond_factor: block size %d needs more LDS than the device offers", bsz);
-    if (!s.Mg) HIPCHK(hfem_dalloc(&s.Mg, (size_t)c->nblocks * b2));
+    const bool fp32 = s.prec_fp32 != 0;
+    if (!fp32 && !s.Mg) HIPCHK(hfem_dalloc(&s.Mg, (size_t)c->nblocks * b2));
+    if (fp32 && !s.Mg32) HIPCHK(hfem_dalloc(&s.Mg32, (size_t)c->nblocks * b2));
     if (kind == 2) {
-        HIPCHK(launch_block_inverse(n, bsz, c->rowstart, c->Hg, s.Mg, s.d_info, c->stream));
+        // block-Jacobi: inverses in FP64; single-precision factors are their FP32 copies
+        if (!fp32) HIPCHK(launch_block_inverse(n, bsz, c->rowstart, c->Hg, s.Mg, s.d_info, c->stream));
+        else {
+            if (!s.Mg) HIPCHK(hfem_dalloc(&s.Mg, (size_t)c->nblocks * b2));
+            HIPCHK(launch_block_inverse(n, bsz, c->rowstart, c->Hg, s.Mg, s.d_info, c->stream));
+            HIPCHK(launch_to_float((long)n * b2, s.Mg, s.Mg32, c->stream));
+        }
     } else {
         if (s.order_method < 0) RC(hfem_compute_ordering(c, 1));
-        HIPCHK(hipMemcpyAsync(s.Mg, c->Hg, sizeof(double) * c->nblocks * b2, hipMemcpyDeviceToDevice, c->stream));
-        for (size_t l = 0; l + 1 < s.lptr.size(); l++)
-            HIPCHK(launch_bilu0_factor_level(bsz, c->rowstart, c->cols, s.d_u2o, s.d_lrows + s.lptr[l],
-                                             s.lptr[l + 1] - s.lptr[l], s.Mg, s.d_info, c->stream));
+        if (!fp32) {
+            HIPCHK(hipMemcpyAsync(s.Mg, c->Hg, sizeof(double) * c->nblocks * b2, hipMemcpyDeviceToDevice, c->stream));
+            for (size_t l = 0; l + 1 < s.lptr.size(); l++)
+                HIPCHK(launch_bilu0_factor_level(bsz, c->rowstart, c->cols, s.d_u2o, s.d_lrows + s.lptr[l],
+                                                 s.lptr[l + 1] - s.lptr[l], s.Mg, s.d_info, c->stream));
+        } else {
+            // single precision (precPrecision 0): Hg rounded to FP32 and factored on
+            // the FP32 MFMA
+            HIPCHK(launch_to_float((long)c->nblocks * b2, c->Hg, s.Mg32, c->stream));
+            for (size_t l = 0; l + 1 < s.lptr.size(); l++)
+                HIPCHK(launch_bilu0_factor_level(bsz, c->rowstart, c->cols, s.d_u2o, s.d_lrows + s.lptr[l],
+                                                 s.lptr[l + 1] - s.lptr[l], s.Mg32, s.d_info, c->stream));
+        }
     }
     std::vector<int> info(n);
     HIPCHK(hipMemcpyAsync(info.data(), s.d_info, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
@@ -275,6 +292,18 @@ extern "C" int hfem_precond_factor(hfem_ctx *c, int kind)
     return 0;
 }
 
+// sys.precPrecision (preprocessing/initializestructs.cpp:476-486): 0 single-precision
+// preconditioner factors (Mg_sp), 1 double.  Takes effect at the next factorisation.
+extern "C" int hfem_set_precond_precision(hfem_ctx *c, int prec_precision)
+{
+    if (!c) return hfem_fail(-1, "hfem_set_precond_precision: null context");
+    if (prec_precision != 0 && prec_precision != 1)
+        return hfem_fail(-1, "hfem_set_precond_precision: %d (0 single, 1 double)", prec_precision);
+    if ((prec_precision == 0) != (c->sv.prec_fp32 != 0)) c->sv.prec = -2;   // factors of the other precision are stale
+    c->sv.prec_fp32 = prec_precision == 0;
+    return 0;
+}
+
 // y = M^{-1} x on device vectors (x and y distinct).
 static int precond_apply_dev(hfem_ctx *c, const double *x, double *y)
 {
@@ -284,15 +313,22 @@ static int precond_apply_dev(hfem_ctx *c, const double *x, double *y)
     if (s.prec == -1) {
         HIPCHK(hipMemcpyAsync(y, x, sizeof(double) * N, hipMemcpyDeviceToDevice, c->stream));
     } else if (s.prec == 2) {
-        HIPCHK(launch_block_jacobi(n, bsz, s.Mg, x, y, c->stream));
+        if (s.prec_fp32) HIPCHK(launch_block_jacobi(n, bsz, s.Mg32, x, y, c->stream));
+        else HIPCHK(launch_block_jacobi(n, bsz, s.Mg, x, y, c->stream));
     } else if (s.prec == 0) {
         // forward sweep into s.t, backward sweep into y
-        for (size_t l = 0; l + 1 < s.lptr.size(); l++)
-            HIPCHK(launch_bilu0_lsolve_level(bsz, c->rowstart, c->cols, s.d_u2o, s.d_lrows + s.lptr[l],
-                                             s.lptr[l + 1] - s.lptr[l], s.Mg, x, s.t, c->stream));
-        for (size_t l = 0; l + 1 < s.uptr.size(); l++)
-            HIPCHK(launch_bilu0_usolve_level(bsz, c->rowstart, c->cols, s.d_u2o, s.d_urows + s.uptr[l],
-                                             s.uptr[l + 1] - s.uptr[l], s.Mg, s.t, y, c->stream));
+        for (size_t l = 0; l + 1 < s.lptr.size(); l++) {
+            const int *rows = s.d_lrows + s.lptr[l];
+            const int nl = s.lptr[l + 1] - s.lptr[l];
+            if (s.prec_fp32) HIPCHK(launch_bilu0_lsolve_level(bsz, c->rowstart, c->cols, s.d_u2o, rows, nl, s.Mg32, x, s.t, c->stream));
+            else HIPCHK(launch_bilu0_lsolve_level(bsz, c->rowstart, c->cols, s.d_u2o, rows, nl, s.Mg, x, s.t, c->stream));
+        }
+        for (size_t l = 0; l + 1 < s.uptr.size(); l++) {
+            const int *rows = s.d_urows + s.uptr[l];
+            const int nl = s.uptr[l + 1] - s.uptr[l];
+            if (s.prec_fp32) HIPCHK(launch_bilu0_usolve_level(bsz, c->rowstart, c->cols, s.d_u2o, rows, nl, s.Mg32, s.t, y, c->stream));
+            else HIPCHK(launch_bilu0_usolve_level(bsz, c->rowstart, c->cols, s.d_u2o, rows, nl, s.Mg, s.t, y, c->stream));
+        }
     } else {
         return hfem_fail(-3, "preconditioner not factored");
     }
@@ -519,6 +555,8 @@ static int check_params(const hfem_solver_params *p)
     if (!(p->newton_tol > 0) || p->newton_maxiter < 1 || p->restart < 1 || !(p->gmres_tol > 0) || p->gmres_maxit < 1)
         return hfem_fail(-1, "solver parameters: tolerances and iteration limits must be positive");
     if (p->precond != -1 && p->precond != 0 && p->precond != 2) return hfem_fail(-1, "solver parameters: precond %d", p->precond);
+    if (p->prec_precision != 0 && p->prec_precision != 1)
+        return hfem_fail(-1, "solver parameters: prec_precision %d (0 single, 1 double)", p->prec_precision);
     if (p->ordering != 0 && p->ordering != 1) return hfem_fail(-1, "solver parameters: ordering %d", p->ordering);
     if (p->orthog != 0 && p->orthog != 1) return hfem_fail(-1, "solver parameters: orthog %d", p->orthog);
     if (p->precision != HFEM_FP64 && p->precision != HFEM_MIXED) return hfem_fail(-1, "solver parameters: precision");
@@ -567,6 +605,7 @@ extern "C" int hfem_solve_steady(hfem_ctx *c, const hfem_solver_params *p, int *
             RC(hfem_compute_ordering(c, p->ordering));
         if (!reused) {
             auto t1 = now();
+            RC(hfem_set_precond_precision(c, p->prec_precision));
             RC(hfem_precond_factor(c, p->precond));
             jac_time += secs(t1);
         }

@@ -16,41 +16,43 @@ constexpr int SV_THREADS = 256;
 // In-place Gauss-Jordan inverse of an n x n column-major block in LDS with
 // partial pivoting: rows are interchanged as pivots are chosen and the columns of
 // the result unscrambled in reverse order at the end.  All threads of the
-// workgroup call it; piv (n ints), rowk/colk (n doubles each) and red (1 double)
-// are LDS.  Returns 0, or 1 + the step of the first zero pivot (workgroup-uniform).
-__device__ int lds_inverse(double *A, int n, int *piv, double *rowk, double *colk, double *red)
+// workgroup call it; piv (n ints), rowk/colk (n values each) and red (1 value)
+// are LDS; T is the arithmetic (double, or float for FP32 preconditioner factors).
+// Returns 0, or 1 + the step of the first zero pivot (workgroup-uniform).
+template <typename T>
+__device__ int lds_inverse(T *A, int n, int *piv, T *rowk, T *colk, T *red)
 {
     const int tid = threadIdx.x, nth = blockDim.x;
     for (int k = 0; k < n; k++) {
         if (tid < 64) {
-            double best = -1.0;
+            T best = (T)-1;
             int bi = k;
             for (int i = k + tid; i < n; i += 64) {
-                const double v = fabs(A[i + n * k]);
+                const T v = fabs(A[i + n * k]);
                 if (v > best) { best = v; bi = i; }
             }
             for (int off = 32; off > 0; off >>= 1) {
-                const double ob = __shfl_down(best, off);
+                const T ob = __shfl_down(best, off);
                 const int oi = __shfl_down(bi, off);
                 if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
             }
             if (tid == 0) { piv[k] = bi; red[0] = best; }
         }
         __syncthreads();
-        if (red[0] == 0.0) return k + 1;
+        if (red[0] == (T)0) return k + 1;
         const int p = piv[k];
         for (int j = tid; j < n; j += nth) {
-            double a = A[k + n * j], b = A[p + n * j];
+            T a = A[k + n * j], b = A[p + n * j];
             rowk[j] = b;                         // pivot row after the interchange
             if (p != k) { A[k + n * j] = b; A[p + n * j] = a; }
         }
         __syncthreads();
         for (int i = tid; i < n; i += nth) colk[i] = A[i + n * k];
         __syncthreads();
-        const double pinv = 1.0 / rowk[k];
+        const T pinv = (T)1 / rowk[k];
         for (int idx = tid; idx < n * n; idx += nth) {
             const int i = idx % n, j = idx / n;
-            double v;
+            T v;
             if (i == k) v = (j == k) ? pinv : rowk[j] * pinv;
             else if (j == k) v = -colk[i] * pinv;
             else v = A[idx] - colk[i] * (rowk[j] * pinv);
@@ -62,7 +64,7 @@ __device__ int lds_inverse(double *A, int n, int *piv, double *rowk, double *col
         const int p = piv[k];
         if (p != k)
             for (int i = tid; i < n; i += nth) {
-                double t = A[i + n * k];
+                T t = A[i + n * k];
                 A[i + n * k] = A[i + n * p];
                 A[i + n * p] = t;
             }
@@ -71,14 +73,15 @@ __device__ int lds_inverse(double *A, int n, int *piv, double *rowk, double *col
     return 0;
 }
 
-struct LdsInv {
-    double *rowk, *colk, *red;
+template <typename T> struct LdsInv {
+    T *rowk, *colk, *red;
     int *piv;
 };
 
-__device__ LdsInv lds_inv_area(double *p, int n)
+template <typename T>
+__device__ LdsInv<T> lds_inv_area(T *p, int n)
 {
-    LdsInv w;
+    LdsInv<T> w;
     w.rowk = p;
     w.colk = p + n;
     w.red = p + 2 * n;
@@ -98,7 +101,7 @@ block_inverse_kernel(int nrows, int bsz, const int *__restrict__ rowpts, const d
     const int tid = threadIdx.x, nth = blockDim.x;
     const long b2 = (long)bsz * bsz;
     double *A = lds;
-    LdsInv w = lds_inv_area(A + b2, bsz);
+    LdsInv<double> w = lds_inv_area(A + b2, bsz);
     for (int row = blockIdx.x; row < nrows; row += gridDim.x) {
         const double *src = Hg + (long)rowpts[row] * b2;
         for (long i = tid; i < b2; i += nth) A[i] = src[i];
@@ -162,30 +165,50 @@ constexpr int FACTOR_MAXROW = 64;     // longest BSR row the factorisation takes
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-// The n x n product C = A B of two column-major LDS blocks on the FP64 MFMA: the
-// 16 x 16 tiles of C are spread over the waves, rows and columns past n are
-// masked to zero; out(i, j, c) receives each entry of C (i, j < n).  MFMA
-// 16x16x4 operands: lane l supplies A(l%16, l/16) and B(l/16, l%16); result lane
-// l, element v holds C(l/16 + 4v, l%16).
-template <typename OUT>
-__device__ __forceinline__ void lds_block_product(const double *A, const double *B, int n, OUT out)
+// The n x n product C = A B of two column-major LDS blocks on the MFMA (FP64, or
+// FP32 for FP32 preconditioner factors): the 16 x 16 tiles of C are spread over
+// the waves, rows and columns past n are masked to zero; out(i, j, c) receives
+// each entry of C (i, j < n).  MFMA 16x16x4 operands: lane l supplies A(l%16,
+// l/16) and B(l/16, l%16); result lane l, element v holds C(l/16 + 4v, l%16)
+// (FP64) or C(4(l/16) + v, l%16) (FP32).
+template <typename T> struct BlkMfma;
+template <> struct BlkMfma<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 run(double a, double b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int v) { return lk + 4 * v; }
+};
+template <> struct BlkMfma<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 run(float a, float b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
+};
+
+template <typename T, typename OUT>
+__device__ __forceinline__ void lds_block_product(const T *A, const T *B, int n, OUT out)
 {
+    typedef BlkMfma<T> M;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int lr = lane & 15, lk = lane >> 4, nt = (n + 15) / 16;
     for (int t = wave; t < nt * nt; t += nw) {
         const int i0 = (t % nt) * 16, j0 = (t / nt) * 16;
         const int ia = min(i0 + lr, n - 1), jb = min(j0 + lr, n - 1);
         const bool va = i0 + lr < n, vb = j0 + lr < n;
-        f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+        typename M::v4 acc = {(T)0, (T)0, (T)0, (T)0};
         for (int k0 = 0; k0 < n; k0 += 4) {
             const int k = k0 + lk, kc = min(k, n - 1);
-            const double a = (va && k < n) ? A[ia + n * kc] : 0.0;
-            const double b = (vb && k < n) ? B[kc + n * jb] : 0.0;
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+            const T a = (va && k < n) ? A[ia + n * kc] : (T)0;
+            const T b = (vb && k < n) ? B[kc + n * jb] : (T)0;
+            acc = M::run(a, b, acc);
         }
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            const int i = i0 + lk + 4 * v, j = j0 + lr;
+            const int i = i0 + M::row(lk, v), j = j0 + lr;
             if (i < n && j < n) out(i, j, acc[v]);
         }
     }
@@ -194,11 +217,12 @@ __device__ __forceinline__ void lds_block_product(const double *A, const double 
 // dst[i] = src[i] for i < n (n >= 1) by the workgroup, eight loads in flight per
 // thread (addresses clamped into the block).  Entry i is always handled by thread
 // i % blockDim.x.
-__device__ __forceinline__ void wg_load_block(double *dst, const double *__restrict__ src, long n)
+template <typename T>
+__device__ __forceinline__ void wg_load_block(T *dst, const T *__restrict__ src, long n)
 {
     const int tid = threadIdx.x, nth = blockDim.x;
     for (long b = tid; b < n; b += 8L * nth) {
-        double v[8];
+        T v[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) v[j] = src[min(b + (long)j * nth, n - 1)];
 #pragma unroll
@@ -209,11 +233,12 @@ __device__ __forceinline__ void wg_load_block(double *dst, const double *__restr
 
 // dst[i] -= d[i] for i < n, global dst, LDS d; same entry -> thread map as
 // wg_load_block.
-__device__ __forceinline__ void wg_subtract_block(double *dst, const double *d, long n)
+template <typename T>
+__device__ __forceinline__ void wg_subtract_block(T *dst, const T *d, long n)
 {
     const int tid = threadIdx.x, nth = blockDim.x;
     for (long b = tid; b < n; b += 8L * nth) {
-        double v[8];
+        T v[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) v[j] = dst[min(b + (long)j * nth, n - 1)];
 #pragma unroll
@@ -222,16 +247,21 @@ __device__ __forceinline__ void wg_subtract_block(double *dst, const double *d, 
     }
 }
 
+// T: the factors' type (double, or float: the reference's single-precision
+// preconditioner, precPrecision 0, preprocessing/initializestructs.cpp:476-486,
+// factored here on the FP32 MFMA).
+template <typename T>
 __global__ void __launch_bounds__(SV_THREADS)
 bilu0_factor_kernel(int bsz, const int *__restrict__ rowpts, const int *__restrict__ colind,
-                    const int *__restrict__ u2o, const int *__restrict__ rows, int nlev, double *__restrict__ Mg,
+                    const int *__restrict__ u2o, const int *__restrict__ rows, int nlev, T *__restrict__ Mg,
                     int *__restrict__ info)
 {
-    extern __shared__ double lds[];
+    extern __shared__ double lds_raw[];
+    T *lds = (T *)lds_raw;
     const int tid = threadIdx.x, nth = blockDim.x;
     const long b2 = (long)bsz * bsz;
-    double *X = lds, *B = lds + b2, *L = lds + 2 * b2;
-    LdsInv w = lds_inv_area(lds + 3 * b2, bsz);
+    T *X = lds, *B = lds + b2, *L = lds + 2 * b2;
+    LdsInv<T> w = lds_inv_area(lds + 3 * b2, bsz);
     // row ri: columns and their order numbers; the lower neighbours' positions by
     // increasing order number; the current neighbour row's columns
     int *rcol = w.piv + bsz + 2, *rord = rcol + FACTOR_MAXROW, *lst = rord + FACTOR_MAXROW,
@@ -272,8 +302,8 @@ bilu0_factor_kernel(int bsz, const int *__restrict__ rowpts, const int *__restri
             wg_load_block(B, Mg + (long)k0 * b2, b2);     // inv(U_kk)
             __syncthreads();
             // L_ik = A_ik inv(U_kk), kept in LDS and stored in place of A_ik
-            double *Mik = Mg + (long)ik * b2;
-            lds_block_product(X, B, bsz, [&](int i, int j, double c) {
+            T *Mik = Mg + (long)ik * b2;
+            lds_block_product(X, B, bsz, [&](int i, int j, T c) {
                 L[i + bsz * j] = c;
                 Mik[i + bsz * j] = c;
             });
@@ -289,7 +319,7 @@ bilu0_factor_kernel(int bsz, const int *__restrict__ rowpts, const int *__restri
                 if (km < 0) continue;
                 wg_load_block(B, Mg + (long)(k0 + km) * b2, b2);
                 __syncthreads();
-                lds_block_product(L, B, bsz, [&](int i, int j, double c) { X[i + bsz * j] = c; });
+                lds_block_product(L, B, bsz, [&](int i, int j, T c) { X[i + bsz * j] = c; });
                 __syncthreads();
                 wg_subtract_block(Mg + (long)(d0 + l) * b2, X, b2);
                 __syncthreads();
@@ -322,8 +352,8 @@ __device__ __forceinline__ double wave_bcast(double v, int src)
 // multiply (addresses clamped into the block; columns past bsz are multiplied by
 // zero), so a block costs ceil(bsz / CH) memory round trips instead of one per
 // column pair.
-template <int CH, bool TWO>
-__device__ __forceinline__ void wave_block_matvec_ch(int bsz, const double *__restrict__ B, double z0, double z1,
+template <int CH, bool TWO, typename TM>
+__device__ __forceinline__ void wave_block_matvec_ch(int bsz, const TM *__restrict__ B, double z0, double z1,
                                                      double &acc0, double &acc1)
 {
     const int lane = threadIdx.x & 63;
@@ -350,7 +380,8 @@ __device__ __forceinline__ void wave_block_matvec_ch(int bsz, const double *__re
     if (TWO && lane + 64 < bsz) acc1 += s1;
 }
 
-__device__ __forceinline__ void wave_block_matvec(int bsz, const double *__restrict__ B, double z0, double z1,
+template <typename TM>
+__device__ __forceinline__ void wave_block_matvec(int bsz, const TM *__restrict__ B, double z0, double z1,
                                                   double &acc0, double &acc1)
 {
     if (bsz > 64) wave_block_matvec_ch<16, true>(bsz, B, z0, z1, acc0, acc1);
@@ -364,10 +395,10 @@ __device__ __forceinline__ void wave_block_matvec(int bsz, const double *__restr
 // 256-thread workgroup per row: the four waves take the neighbour blocks in turn
 // (lane = block row, the neighbour vector broadcast from registers), their partial
 // sums meet in LDS.
-template <bool BACKWARD, bool SMALL>
+template <bool BACKWARD, bool SMALL, typename TM>
 __global__ void __launch_bounds__(SV_THREADS)
 bilu0_sweep_kernel(int bsz, const int *__restrict__ rowpts, const int *__restrict__ colind,
-                   const int *__restrict__ u2o, const int *__restrict__ rows, int nlev, const double *__restrict__ Mg,
+                   const int *__restrict__ u2o, const int *__restrict__ rows, int nlev, const TM *__restrict__ Mg,
                    const double *__restrict__ b, double *__restrict__ z)
 {
     constexpr int NW = SV_THREADS / 64;
@@ -379,7 +410,7 @@ bilu0_sweep_kernel(int bsz, const int *__restrict__ rowpts, const int *__restric
     const int r0 = lane < bsz ? lane : 0, r1 = lane + 64 < bsz ? lane + 64 : 0;
     for (int t = blockIdx.x; t < nlev; t += gridDim.x) {
         const int r = rows[t], o = u2o[r], k0 = rowpts[r], len = min(rowpts[r + 1] - k0, FACTOR_MAXROW);
-        const double *Ui = Mg + (long)k0 * b2;
+        const TM *Ui = Mg + (long)k0 * b2;
         // the loads that depend on nothing else go out together: the row's column
         // list (rows have at most FACTOR_MAXROW blocks: the factorisation rejects
         // longer ones), its right-hand side and, backward, this wave's columns
@@ -465,8 +496,9 @@ bilu0_sweep_kernel(int bsz, const int *__restrict__ rowpts, const int *__restric
 }
 
 // Block-Jacobi application y_i = inv(H_ii) x_i.  One wave per row.
+template <typename TM>
 __global__ void __launch_bounds__(SV_THREADS)
-block_jacobi_kernel(int nrows, int bsz, const double *__restrict__ Dinv, const double *__restrict__ x,
+block_jacobi_kernel(int nrows, int bsz, const TM *__restrict__ Dinv, const double *__restrict__ x,
                     double *__restrict__ y)
 {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, wpb = blockDim.x >> 6;
@@ -633,49 +665,92 @@ hipError_t launch_mdf_weights(int nrows, int bsz, const int *rowpts, const doubl
     return hipGetLastError();
 }
 
+template <typename T>
+static hipError_t factor_level(int bsz, const int *rowpts, const int *colind, const int *u2o, const int *rows, int nlev,
+                               T *Mg, int *info, hipStream_t s)
+{
+    const size_t lds = bilu0_factor_lds(bsz);
+    hipError_t e = set_lds((const void *)bilu0_factor_kernel<T>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(bilu0_factor_kernel<T>, dim3(std::min(nlev, 8192)), dim3(SV_THREADS), lds, s, bsz, rowpts,
+                       colind, u2o, rows, nlev, Mg, info);
+    return hipGetLastError();
+}
+
 hipError_t launch_bilu0_factor_level(int bsz, const int *rowpts, const int *colind, const int *u2o, const int *rows,
                                      int nlev, double *Mg, int *info, hipStream_t s)
 {
-    const size_t lds = bilu0_factor_lds(bsz);
-    hipError_t e = set_lds((const void *)bilu0_factor_kernel, lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(bilu0_factor_kernel, dim3(std::min(nlev, 8192)), dim3(SV_THREADS), lds, s, bsz, rowpts, colind,
-                       u2o, rows, nlev, Mg, info);
+    return factor_level(bsz, rowpts, colind, u2o, rows, nlev, Mg, info, s);
+}
+
+hipError_t launch_bilu0_factor_level(int bsz, const int *rowpts, const int *colind, const int *u2o, const int *rows,
+                                     int nlev, float *Mg, int *info, hipStream_t s)
+{
+    return factor_level(bsz, rowpts, colind, u2o, rows, nlev, Mg, info, s);
+}
+
+template <bool BACKWARD, typename TM>
+static hipError_t sweep_level(int bsz, const int *rowpts, const int *colind, const int *u2o, const int *rows,
+                              int nlev, const TM *Mg, const double *x, double *y, hipStream_t s)
+{
+    if (bsz > 128) return hipErrorInvalidValue;
+    if (bsz <= 64)
+        hipLaunchKernelGGL((bilu0_sweep_kernel<BACKWARD, true, TM>), dim3(std::min(nlev, 8192)), dim3(SV_THREADS), 0,
+                           s, bsz, rowpts, colind, u2o, rows, nlev, Mg, x, y);
+    else
+        hipLaunchKernelGGL((bilu0_sweep_kernel<BACKWARD, false, TM>), dim3(std::min(nlev, 8192)), dim3(SV_THREADS), 0,
+                           s, bsz, rowpts, colind, u2o, rows, nlev, Mg, x, y);
     return hipGetLastError();
 }
 
 hipError_t launch_bilu0_lsolve_level(int bsz, const int *rowpts, const int *colind, const int *u2o, const int *rows,
                                      int nlev, const double *Mg, const double *x, double *y, hipStream_t s)
 {
-    if (bsz > 128) return hipErrorInvalidValue;
-    if (bsz <= 64)
-        hipLaunchKernelGGL((bilu0_sweep_kernel<false, true>), dim3(std::min(nlev, 8192)), dim3(SV_THREADS), 0, s, bsz,
-                           rowpts, colind, u2o, rows, nlev, Mg, x, y);
-    else
-        hipLaunchKernelGGL((bilu0_sweep_kernel<false, false>), dim3(std::min(nlev, 8192)), dim3(SV_THREADS), 0, s,
-                           bsz, rowpts, colind, u2o, rows, nlev, Mg, x, y);
-    return hipGetLastError();
+    return sweep_level<false>(bsz, rowpts, colind, u2o, rows, nlev, Mg, x, y, s);
 }
-
+hipError_t launch_bilu0_lsolve_level(int bsz, const int *rowpts, const int *colind, const int *u2o, const int *rows,
+                                     int nlev, const float *Mg, const double *x, double *y, hipStream_t s)
+{
+    return sweep_level<false>(bsz, rowpts, colind, u2o, rows, nlev, Mg, x, y, s);
+}
 hipError_t launch_bilu0_usolve_level(int bsz, const int *rowpts, const int *colind, const int *u2o, const int *rows,
                                      int nlev, const double *Mg, const double *y, double *v, hipStream_t s)
 {
-    if (bsz > 128) return hipErrorInvalidValue;
-    if (bsz <= 64)
-        hipLaunchKernelGGL((bilu0_sweep_kernel<true, true>), dim3(std::min(nlev, 8192)), dim3(SV_THREADS), 0, s, bsz,
-                           rowpts, colind, u2o, rows, nlev, Mg, y, v);
-    else
-        hipLaunchKernelGGL((bilu0_sweep_kernel<true, false>), dim3(std::min(nlev, 8192)), dim3(SV_THREADS), 0, s, bsz,
-                           rowpts, colind, u2o, rows, nlev, Mg, y, v);
-    return hipGetLastError();
+    return sweep_level<true>(bsz, rowpts, colind, u2o, rows, nlev, Mg, y, v, s);
+}
+hipError_t launch_bilu0_usolve_level(int bsz, const int *rowpts, const int *colind, const int *u2o, const int *rows,
+                                     int nlev, const float *Mg, const double *y, double *v, hipStream_t s)
+{
+    return sweep_level<true>(bsz, rowpts, colind, u2o, rows, nlev, Mg, y, v, s);
 }
 
-hipError_t launch_block_jacobi(int nrows, int bsz, const double *Dinv, const double *x, double *y, hipStream_t s)
+template <typename TM>
+static hipError_t bj(int nrows, int bsz, const TM *Dinv, const double *x, double *y, hipStream_t s)
 {
     if (bsz > 128) return hipErrorInvalidValue;
     const int wpb = SV_THREADS / 64;
-    hipLaunchKernelGGL(block_jacobi_kernel, dim3(std::min((nrows + wpb - 1) / wpb, 8192)), dim3(SV_THREADS), 0, s,
-                       nrows, bsz, Dinv, x, y);
+    hipLaunchKernelGGL(block_jacobi_kernel<TM>, dim3(std::min((nrows + wpb - 1) / wpb, 8192)), dim3(SV_THREADS), 0,
+                       s, nrows, bsz, Dinv, x, y);
+    return hipGetLastError();
+}
+hipError_t launch_block_jacobi(int nrows, int bsz, const double *Dinv, const double *x, double *y, hipStream_t s)
+{
+    return bj(nrows, bsz, Dinv, x, y, s);
+}
+hipError_t launch_block_jacobi(int nrows, int bsz, const float *Dinv, const double *x, double *y, hipStream_t s)
+{
+    return bj(nrows, bsz, Dinv, x, y, s);
+}
+
+__global__ void to_float_kernel(long n, const double *__restrict__ x, float *__restrict__ y)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        y[i] = (float)x[i];
+}
+
+hipError_t launch_to_float(long n, const double *x, float *y, hipStream_t s)
+{
+    hipLaunchKernelGGL(to_float_kernel, dim3(grid_for(n)), dim3(SV_THREADS), 0, s, n, x, y);
     return hipGetLastError();
 }
 

@@ -68,6 +68,7 @@ EXPORTS = {
     "hfem_get_ordering": (ctypes.c_int, [_vp, _ip, _ip]),
     "hfem_precond_factor": (ctypes.c_int, [_vp, ctypes.c_int]),
     "hfem_precond_apply": (ctypes.c_int, [_vp, _dp, _dp]),
+    "hfem_set_precond_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
     "hfem_gmres": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _ip, _dp, _ip]),
     "hfem_read_trace_solution": (ctypes.c_int, [_vp, _dp]),
     "hfem_update_solution": (ctypes.c_int, [_vp, ctypes.c_double]),
@@ -89,14 +90,16 @@ class hfem_solver_params(ctypes.Structure):
                 ("ordering", ctypes.c_int), ("reuse_ordering", ctypes.c_int), ("restart", ctypes.c_int),
                 ("gmres_tol", ctypes.c_double), ("gmres_maxit", ctypes.c_int), ("orthog", ctypes.c_int),
                 ("precision", ctypes.c_int), ("quasi_newton", ctypes.c_int), ("qn_accuracy", ctypes.c_int),
-                ("true_newton_maxiter", ctypes.c_int), ("qn_reuse_ratio", ctypes.c_double)]
+                ("true_newton_maxiter", ctypes.c_int), ("qn_reuse_ratio", ctypes.c_double),
+                ("prec_precision", ctypes.c_int)]
 
     @classmethod
     def make(cls, newton_tol=1e-7, newton_maxiter=10, precond=0, ordering=1, reuse_ordering=1, restart=30,
              gmres_tol=1e-6, gmres_maxit=1000, orthog=1, precision=0, quasi_newton=0, qn_accuracy=1,
-             true_newton_maxiter=0, qn_reuse_ratio=3.0):
+             true_newton_maxiter=0, qn_reuse_ratio=3.0, prec_precision=1):
         return cls(newton_tol, newton_maxiter, precond, ordering, reuse_ordering, restart, gmres_tol, gmres_maxit,
-                   orthog, precision, quasi_newton, qn_accuracy, true_newton_maxiter, qn_reuse_ratio)
+                   orthog, precision, quasi_newton, qn_accuracy, true_newton_maxiter, qn_reuse_ratio,
+                   prec_precision)
 
 
 def load():
@@ -310,7 +313,10 @@ class HDGContext:
         _check(load().hfem_get_ordering(self.h, _i(o2u), _i(nlev)))
         return o2u, tuple(int(v) for v in nlev)
 
-    def compute_preconditioner(self, kind=PREC_BILU0):
+    def compute_preconditioner(self, kind=PREC_BILU0, prec_precision=1):
+        """computePreconditioner; prec_precision 0 keeps the factors in single
+        precision (sys.precPrecision)."""
+        _check(load().hfem_set_precond_precision(self.h, prec_precision))
         _check(load().hfem_precond_factor(self.h, kind))
 
     def apply_preconditioner(self, x):

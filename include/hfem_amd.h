@@ -173,6 +173,11 @@ int hfem_get_ordering(hfem_ctx *ctx, int *o2u, int *nlevels);
 /* computePreconditioner: kind -1 none, 0 BILU0 in the current ordering
  * (kernel/computeBILU0.m:1-46), 2 block-Jacobi (kernel/mkBlockJacobi.m:1-16). */
 int hfem_precond_factor(hfem_ctx *ctx, int kind);
+/* sys.precPrecision (preprocessing/initializestructs.cpp:476-486): 0 keeps the
+ * preconditioner factors in single precision (Mg_sp; BILU0 factored on the FP32
+ * MFMA, sweeps reading FP32 factors with FP64 arithmetic), 1 in double (default).
+ * Takes effect at the next hfem_precond_factor. */
+int hfem_set_precond_precision(hfem_ctx *ctx, int prec_precision);
 /* y = M^{-1} x with host vectors of length nf*bsz (kernel/applyBILU0.m:1-52). */
 int hfem_precond_apply(hfem_ctx *ctx, const double *x, double *y);
 /* gmres: left-preconditioned restarted GMRES on Hg x = Rg from x = 0
@@ -209,6 +214,8 @@ typedef struct {
     double qn_reuse_ratio;   /* reuse while the GMRES time since the last Jacobian is below this
                                 multiple of that Jacobian's assembly + preconditioner time (3 in
                                 solvers/dgsolvers.cpp:137-141) */
+    int prec_precision;      /* preconditioner factors: 0 single (FP32, factored on the FP32 MFMA),
+                                1 double (sys.precPrecision) */
 } hfem_solver_params;
 
 /* solveNonlinearProblem (solvers/dgsolvers.cpp:142-205): Newton with the

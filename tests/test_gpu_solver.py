@@ -87,6 +87,43 @@ def test_preconditioner_matches_oracle(oracle_mod, system, kind):
     assert rel(got, want) < 1e-10
 
 
+@pytest.mark.parametrize("kind", ["bilu0", "bj"])
+def test_single_precision_preconditioner(oracle_mod, system, kind):
+    """precPrecision 0: factors in FP32 (BILU0 factored on the FP32 MFMA).  The
+    application equals the oracle's FP64 preconditioner to FP32 accuracy (a
+    relative 1e-4 leaves room for the block condition numbers; a wrong block or
+    index is O(1)), and GMRES with it reaches the same FP64 tolerance and the same
+    solution as with FP64 factors."""
+    from hfem_amd.lib import PREC_BILU0, PREC_BLOCK_JACOBI
+    O = oracle_mod
+    prob, ctx, M, Rg = system
+    x = np.random.default_rng(3).standard_normal(len(Rg))
+    pk = PREC_BILU0 if kind == "bilu0" else PREC_BLOCK_JACOBI
+    if kind == "bilu0":
+        o2u, u2o = O.mdf_order(M)
+        ctx.set_ordering(o2u)
+        Mg, _ = O.bilu0_factor(M, o2u, u2o)
+        want = O.bilu0_apply(M, o2u, u2o, Mg, x)
+    else:
+        Dinv, _ = O.block_jacobi_factor(M)
+        blocks = Dinv.reshape(-1, M.bsz, M.bsz).transpose(0, 2, 1)
+        want = np.concatenate([blocks[i] @ x[i * M.bsz:(i + 1) * M.bsz] for i in range(M.nrows)])
+    ctx.compute_preconditioner(pk, prec_precision=0)
+    got = ctx.apply_preconditioner(x)
+    e32 = rel(got, want)
+    say(f"  {kind} FP32 factors apply rel err {e32:.2e}")
+    assert 1e-12 < e32 < 1e-4
+    flag32, it32, _ = ctx.gmres(restart=30, tol=1e-10, maxit=500)
+    x32 = ctx.trace_solution()
+    ctx.compute_preconditioner(pk, prec_precision=1)
+    assert rel(ctx.apply_preconditioner(x), want) < 1e-10
+    flag64, it64, _ = ctx.gmres(restart=30, tol=1e-10, maxit=500)
+    x64 = ctx.trace_solution()
+    say(f"  gmres FP32 factors {it32} it, FP64 factors {it64} it, x rel {rel(x32, x64):.2e}")
+    assert flag32 == flag64 == 0 and abs(it32 - it64) <= max(2, it64 // 10)
+    assert rel(x32, x64) < 1e-8
+
+
 @pytest.mark.parametrize("orthog", [0, 1])
 def test_gmres_matches_oracle(oracle_mod, system, orthog):
     from hfem_amd.lib import PREC_BILU0
@@ -186,6 +223,26 @@ def test_newton_line_search_retraction_matches_oracle(oracle_mod):
     U, UH = ctx.get_solution()
     assert rel(U, Ur) < 1e-10 and rel(UH, UHr) < 1e-10
     ctx.close()
+
+
+def test_single_precision_preconditioner_newton():
+    """Newton with single-precision BILU0 factors (precPrecision 0) takes the same
+    iterations as with FP64 factors, the linear solves running to the same FP64
+    tolerance; the converged state agrees to 1e-10."""
+    from hfem_amd.lib import HDGContext
+    prob = make_problem("tri", 3, 2, seed=5, angle=0.3)
+    out = {}
+    for pp in (1, 0):
+        ctx = HDGContext(prob)
+        conv, nit, hist = ctx.solve_steady(newton_tol=1e-11, newton_maxiter=8, restart=30, gmres_tol=1e-12,
+                                           ordering=1, prec_precision=pp)
+        U, UH = ctx.get_solution()
+        out[pp] = (conv, nit, hist, U, UH)
+        ctx.close()
+    (c1, n1, h1, U1, UH1), (c0, n0, h0, U0, UH0) = out[1], out[0]
+    say(f"  newton FP64 factors {h1[:, [0, 2]].tolist()} FP32 factors {h0[:, [0, 2]].tolist()}")
+    assert c0 and c1 and n0 == n1
+    assert rel(U0, U1) < 1e-10 and rel(UH0, UH1) < 1e-10
 
 
 def test_mixed_newton_same_iterations_as_fp64():
