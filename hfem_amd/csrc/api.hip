@@ -123,7 +123,7 @@ extern "C" int hfem_destroy(hfem_ctx *c)
                     c->rowstart, c->cols, c->einv, c->f2e, c->blkptr, c->blklist, c->UDG, c->UH, c->SH, c->D, c->Fx,
                     c->X, c->K, c->Hx, c->Ru, c->Rh, c->ru2, c->info, c->dinv, c->spill, c->rwork, c->fluxbuf, c->Hg, c->Rg,
                     c->r, c->partial, c->xv, c->yv, c->sv.d_o2u, c->sv.d_u2o, c->sv.d_lrows, c->sv.d_urows,
-                    c->sv.Mg, c->sv.Mg32, c->sv.Dinv, c->sv.Cw, c->sv.d_info, c->sv.x, c->sv.V, c->sv.b, c->sv.t, c->sv.dots,
+                    c->sv.Mg, c->sv.Mg32, c->sv.gy, c->sv.Dinv, c->sv.Cw, c->sv.d_info, c->sv.x, c->sv.V, c->sv.b, c->sv.t, c->sv.dots,
                     c->sv.coef, c->sv.UDGi, c->qn_dinv, c->qn_K, c->rdmax, c->rcmax, c->rdprev, c->rcount};
     for (void *p : ptrs)
         if (p) hipFree(p);
@@ -131,6 +131,7 @@ extern "C" int hfem_destroy(hfem_ctx *c)
         if (ev) hipEventDestroy(ev);
     for (auto &ev : c->sv.ev)
         if (ev) hipEventDestroy(ev);
+    if (c->sv.apply_graph) hipGraphExecDestroy(c->sv.apply_graph);
     if (c->blas) rocblas_destroy_handle(c->blas);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -291,12 +292,12 @@ static int check_range(hfem_ctx *c, int e0, int n, const char *who)
     return 0;
 }
 
-extern "C" int hfem_assemble(hfem_ctx *c, int e0, int n)
+static int assemble_range(hfem_ctx *c, int e0, int n, int res, const char *who)
 {
-    int rc = check_range(c, e0, n, "hfem_assemble");
+    int rc = check_range(c, e0, n, who);
     if (rc) return rc;
     if (!c->has_master || !c->has_phys || !c->has_mesh || !c->has_sol)
-        return hfem_fail(-3, "hfem_assemble: master, physics, mesh and solution must be set first");
+        return hfem_fail(-3, "%s: master, physics, mesh and solution must be set first", who);
     if (n == 0) return 0;
     HIPCHK(hipSetDevice(c->device));
     Master m{c->shapvt, c->shapvg, c->shapft, c->shapfg, c->perm, c->invperm};
@@ -304,9 +305,21 @@ extern "C" int hfem_assemble(hfem_ctx *c, int e0, int n)
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
     HIPCHK(launch_assemble(c->d, m, c->ph, c->dgnodes, c->UDG, c->UH, c->SH, c->elcon, c->bf, e0, n, grid, c->ldx,
                            c->fluxbuf, c->fstride, c->chunk, c->Ru, c->D, c->Fx, c->K, c->Hx, c->Rh, c->ru2,
-                           c->spill, c->stream));
+                           c->spill, c->stream, res));
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
     return 0;
+}
+
+extern "C" int hfem_assemble(hfem_ctx *c, int e0, int n) { return assemble_range(c, e0, n, 0, "hfem_assemble"); }
+
+// assembleElementVector (assembly/elementassembly.cpp:1501-1512): the residual
+// rows only.  The reference still forms every Jacobian there (:358, :785); here
+// the flux kernel evaluates the fluxes without derivatives and the assembly skips
+// the matrix phases.  Ru, Rh (and the [F Ru] / [H Rh] columns, |Ru|^2) are
+// written; D, F, K, H keep their values.
+extern "C" int hfem_assemble_vector(hfem_ctx *c, int e0, int n)
+{
+    return assemble_range(c, e0, n, 1, "hfem_assemble_vector");
 }
 
 // The two batched GEMMs of the FP64 condensation, over n elements:
@@ -875,7 +888,7 @@ extern "C" int hfem_assemble_rhs(hfem_ctx *c, double *rNorm)
                              "hfem_assemble_linear_system)");
     const int ne = c->hd.ne, nf = c->hd.nf;
     const Dims &d = c->d;
-    int rc = hfem_assemble(c, 0, ne);
+    int rc = hfem_assemble_vector(c, 0, ne);
     if (rc) return rc;
     const int bsz = d.nch * d.npf;
     HIPCHK(launch_face_gather(nf, bsz, d.nch, d.npf, d.nb, d.nb, c->f2e, c->einv, c->Rh, c->r, nullptr, nullptr,

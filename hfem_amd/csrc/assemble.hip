@@ -195,7 +195,8 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                 const double *__restrict__ UH, const double *__restrict__ SH, const int *__restrict__ elcon,
                 const int *__restrict__ bf, int e0, int n, int ldx, const double *__restrict__ fluxbuf, long fstride,
                 double *__restrict__ Ru, double *__restrict__ Dm, double *__restrict__ Fx, double *__restrict__ Km,
-                double *__restrict__ Hx, double *__restrict__ Rh, double *__restrict__ ru2, double *__restrict__ spillbuf)
+                double *__restrict__ Hx, double *__restrict__ Rh, double *__restrict__ ru2, double *__restrict__ spillbuf,
+                int res)
 {
     extern __shared__ double lds[];
     const int tid = threadIdx.x, nth = blockDim.x;
@@ -313,7 +314,9 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
         // test-function gradients (C_x(i,k) = sum_g G(i,g,x) shapvt(g,k) follows in P3)
         {
             const int nG = npv * ngv * ND, nM = npv * npv, nU = ngv * nc, nE = ND * npv * ndf;
-            for (int i = tid; i < nG + nM + nU + nE; i += nth) {
+            // residual only: udgg is all the residual rows need
+            const int i0 = res ? nG + nM : 0, i1 = res ? nG + nM + nU : nG + nM + nU + nE;
+            for (int i = i0 + tid; i < i1; i += nth) {
                 if (i < nG) {
                     const int nn = i % npv, r = i / npv, g = r % ngv, x = r / ngv;
                     double sg = 0;
@@ -348,6 +351,9 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
         }
         __syncthreads();
 
+        // P3-P8 form the element matrices; the residual-only assembly
+        // (assembleElementVector) goes from P2 straight to P9
+        if (!res) {
         // ---------------- P3: M^{-1} by Gauss-Jordan (M is SPD: no pivoting), and C ----------------
         // Wave 0 inverts M in registers, one column per lane; the other waves form
         // C_x(jj, ii) at C[x*npv*npv + ii*npv + jj] meanwhile.
@@ -580,6 +586,8 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
         }
         __syncthreads();
 
+        }   // !res (P3-P8)
+
         // ---------------- P9: Ru, Rh ----------------
         // elementint's residual rows (:418-549) and faceint's (:862-1072, boundary
         // faces with fbou): tables T[g + ngv*(j + ncu*r)] (r = 0 the time term, r > 0
@@ -657,7 +665,7 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
         // transposed so that a tile column runs along contiguous (kk, j) rows of K, H.
         // The face's point values of M^{-1}C_x and M^{-1}E_x (Q C_x, Q E_x on its
         // ngf points) are formed first, on the MFMA, next to the flux Jacobians.
-        {
+        if (!res) {
             const int ncol = ncu * npf * nch, nct = (ncol + 15) / 16;
             const int nrg = (ngf + 15) / 16;
             const long nH = (long)nb * nb;
@@ -845,7 +853,7 @@ template <int ND, int NPV, int NPF>
 __global__ void __launch_bounds__(256)
 flux_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, const double *__restrict__ UDG,
             const double *__restrict__ UH, const int *__restrict__ elcon, const int *__restrict__ bf, int e0, int n,
-            double *__restrict__ fluxbuf, long fstride)
+            double *__restrict__ fluxbuf, long fstride, int res)
 {
     extern __shared__ double lds[];
     const int tid = threadIdx.x, nth = blockDim.x;
@@ -928,7 +936,9 @@ flux_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, const
         }
         __syncthreads();
         {
-            const int nvol = ngv * NC, nfac = np * NC, nbnd = np * NCU;
+            // res: fluxes only (derivative direction 0 carries the values), for the
+            // residual-only assembly
+            const int nvol = ngv * (res ? 1 : NC), nfac = np * (res ? 1 : NC), nbnd = np * (res ? 1 : NCU);
             for (int i = tid; i < nvol; i += nth) {
                 const int g = i % ngv, k = i / ngv;
                 dual w[NC], fl[NCU * ND];
@@ -1184,7 +1194,7 @@ hipError_t launch_assemble(const Dims &d, const Master &m, const Phys &ph, const
                            const double *UH, const double *SH, const int *elcon, const int *bf, int e0, int n,
                            int grid, int ldx, double *fluxbuf, long fstride, int chunk,
                            double *Ru, double *D, double *Fx, double *K, double *Hx, double *Rh, double *ru2,
-                           double *spill, hipStream_t stream)
+                           double *spill, hipStream_t stream, int res)
 {
     const size_t lds = assemble_lds_bytes(d), flds = flux_lds_bytes(d);
     const bool sp = asm_layout(d).spill != 0;
@@ -1198,7 +1208,7 @@ hipError_t launch_assemble(const Dims &d, const Master &m, const Phys &ph, const
             hipFuncSetAttribute((const void *)flux_kernel<a, b, c>, hipFuncAttributeMaxDynamicSharedMemorySize,    \
                                 (int)flds);                                                                         \
             hipLaunchKernelGGL((flux_kernel<a, b, c>), dim3(fgrid), dim3(256), flds, stream, d, m, ph, dgnodes,     \
-                               UDG, UH, elcon, bf, c0, cn, fluxbuf, fstride);                                       \
+                               UDG, UH, elcon, bf, c0, cn, fluxbuf, fstride, res);                                  \
             done = true;                                                                                            \
         }
         HFEM_FAMILIES(FLUX)
@@ -1211,7 +1221,7 @@ hipError_t launch_assemble(const Dims &d, const Master &m, const Phys &ph, const
                                 (int)lds);                                                                          \
             hipLaunchKernelGGL((assemble_kernel<a, b, c, S>), dim3(agrid), dim3(ASM_THREADS), lds, stream, d, m, ph,\
                                dgnodes, UDG, UH, SH, elcon, bf, c0, cn, ldx, fluxbuf, fstride, Ru, D, Fx, K, Hx,   \
-                               Rh, ru2, spill);                                                                     \
+                               Rh, ru2, spill, res);                                                                \
             done = true;                                                                                            \
         }
 #define LAUNCH_LDS(a, b, c) LAUNCH(a, b, c, false)

@@ -17,7 +17,7 @@ namespace hfem {
 hipError_t launch_assemble(const Dims &, const Master &, const Phys &, const double *, const double *, const double *,
                            const double *, const int *, const int *, int, int, int, int, double *, long,
                            int, double *, double *, double *, double *, double *, double *, double *, double *,
-                           hipStream_t);
+                           hipStream_t, int res = 0);
 size_t flux_lds_bytes(const Dims &d);
 hipError_t launch_dinv(int na, int e0, int n, const double *D, double *Dinv, int *info, hipStream_t s,
                        int variant = 0);
@@ -131,6 +131,12 @@ struct SolverState {
     double *Mg = nullptr;               // BILU0 factors (nblocks blocks) or block-Jacobi inverses (nf blocks)
     float *Mg32 = nullptr;              // the same in single precision (precPrecision 0)
     int prec_fp32 = 0;
+    // optionally, the BILU0 application (every level launch of both sweeps, yv -> t
+    // -> gy) captured once per factorisation as a HIP graph and replayed
+    hipGraphExec_t apply_graph = nullptr;
+    bool graph_dirty = true;
+    int use_graph = 0;
+    double *gy = nullptr;
     double *Dinv = nullptr;             // inverted diagonal blocks of Hg (MDF weights)
     double *Cw = nullptr;               // MDF block weights ||inv(H_ii) H_ij||_F
     int *d_info = nullptr;

@@ -288,6 +288,7 @@ extern "C" int hfem_precond_factor(hfem_ctx *c, int kind)
         if (info[i]) return hfem_fail(-4, "hfem_precond_factor: singular pivot block in row %d", i);
     }
     s.prec = kind;
+    s.graph_dirty = true;
     s.ms_factor = now_ms() - t0;
     return 0;
 }
@@ -304,6 +305,36 @@ extern "C" int hfem_set_precond_precision(hfem_ctx *c, int prec_precision)
     return 0;
 }
 
+// The two BILU0 sweeps, one launch per level: forward into s.t, backward into y.
+static int bilu0_sweeps(hfem_ctx *c, const double *x, double *y)
+{
+    SolverState &s = c->sv;
+    const int bsz = trace_bsz(c);
+    for (size_t l = 0; l + 1 < s.lptr.size(); l++) {
+        const int *rows = s.d_lrows + s.lptr[l];
+        const int nl = s.lptr[l + 1] - s.lptr[l];
+        if (s.prec_fp32) HIPCHK(launch_bilu0_lsolve_level(bsz, c->rowstart, c->cols, s.d_u2o, rows, nl, s.Mg32, x, s.t, c->stream));
+        else HIPCHK(launch_bilu0_lsolve_level(bsz, c->rowstart, c->cols, s.d_u2o, rows, nl, s.Mg, x, s.t, c->stream));
+    }
+    for (size_t l = 0; l + 1 < s.uptr.size(); l++) {
+        const int *rows = s.d_urows + s.uptr[l];
+        const int nl = s.uptr[l + 1] - s.uptr[l];
+        if (s.prec_fp32) HIPCHK(launch_bilu0_usolve_level(bsz, c->rowstart, c->cols, s.d_u2o, rows, nl, s.Mg32, s.t, y, c->stream));
+        else HIPCHK(launch_bilu0_usolve_level(bsz, c->rowstart, c->cols, s.d_u2o, rows, nl, s.Mg, s.t, y, c->stream));
+    }
+    return 0;
+}
+
+// Whether the BILU0 application replays a captured graph (1, default) or launches
+// its levels one by one (0).  No counterpart in the reference.
+extern "C" int hfem_set_apply_graph(hfem_ctx *c, int use)
+{
+    if (!c) return hfem_fail(-1, "hfem_set_apply_graph: null context");
+    c->sv.use_graph = use != 0;
+    c->sv.graph_dirty = true;
+    return 0;
+}
+
 // y = M^{-1} x on device vectors (x and y distinct).
 static int precond_apply_dev(hfem_ctx *c, const double *x, double *y)
 {
@@ -316,18 +347,33 @@ static int precond_apply_dev(hfem_ctx *c, const double *x, double *y)
         if (s.prec_fp32) HIPCHK(launch_block_jacobi(n, bsz, s.Mg32, x, y, c->stream));
         else HIPCHK(launch_block_jacobi(n, bsz, s.Mg, x, y, c->stream));
     } else if (s.prec == 0) {
-        // forward sweep into s.t, backward sweep into y
-        for (size_t l = 0; l + 1 < s.lptr.size(); l++) {
-            const int *rows = s.d_lrows + s.lptr[l];
-            const int nl = s.lptr[l + 1] - s.lptr[l];
-            if (s.prec_fp32) HIPCHK(launch_bilu0_lsolve_level(bsz, c->rowstart, c->cols, s.d_u2o, rows, nl, s.Mg32, x, s.t, c->stream));
-            else HIPCHK(launch_bilu0_lsolve_level(bsz, c->rowstart, c->cols, s.d_u2o, rows, nl, s.Mg, x, s.t, c->stream));
-        }
-        for (size_t l = 0; l + 1 < s.uptr.size(); l++) {
-            const int *rows = s.d_urows + s.uptr[l];
-            const int nl = s.uptr[l + 1] - s.uptr[l];
-            if (s.prec_fp32) HIPCHK(launch_bilu0_usolve_level(bsz, c->rowstart, c->cols, s.d_u2o, rows, nl, s.Mg32, s.t, y, c->stream));
-            else HIPCHK(launch_bilu0_usolve_level(bsz, c->rowstart, c->cols, s.d_u2o, rows, nl, s.Mg, s.t, y, c->stream));
+        const long bytes = sizeof(double) * N;
+        if (s.use_graph) {
+            // the level launches of both sweeps as one graph from c->yv into s.gy
+            // (captured after each factorisation; no memset nodes in it)
+            if (s.graph_dirty || !s.apply_graph) {
+                if (s.apply_graph) HIPCHK(hipGraphExecDestroy(s.apply_graph));
+                s.apply_graph = nullptr;
+                if (!s.gy) HIPCHK(hfem_dalloc(&s.gy, N));
+                hipGraph_t g = nullptr;
+                HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                const int rc = bilu0_sweeps(c, c->yv, s.gy);
+                hipError_t e = hipStreamEndCapture(c->stream, &g);
+                if (rc) {
+                    if (g) hipGraphDestroy(g);
+                    return rc;
+                }
+                HIPCHK(e);
+                e = hipGraphInstantiate(&s.apply_graph, g, nullptr, nullptr, 0);
+                hipGraphDestroy(g);
+                HIPCHK(e);
+                s.graph_dirty = false;
+            }
+            if (x != c->yv) HIPCHK(hipMemcpyAsync(c->yv, x, bytes, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(hipGraphLaunch(s.apply_graph, c->stream));
+            HIPCHK(hipMemcpyAsync(y, s.gy, bytes, hipMemcpyDeviceToDevice, c->stream));
+        } else {
+            RC(bilu0_sweeps(c, x, y));
         }
     } else {
         return hfem_fail(-3, "preconditioner not factored");
@@ -537,7 +583,7 @@ extern "C" int hfem_residual_norm(hfem_ctx *c, double *rNorm)
     RC(require_system(c, "hfem_residual_norm"));
     const int ne = c->hd.ne, nf = c->hd.nf;
     const Dims &d = c->d;
-    RC(hfem_assemble(c, 0, ne));
+    RC(hfem_assemble_vector(c, 0, ne));   // residual rows only
     HIPCHK(launch_face_gather(nf, trace_bsz(c), d.nch, d.npf, d.nb, d.nb, c->f2e, c->einv, c->Rh, c->r, nullptr,
                               nullptr, c->stream));
     double s1 = 0, s2 = 0;

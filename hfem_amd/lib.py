@@ -49,6 +49,7 @@ EXPORTS = {
     "hfem_set_solution": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
     "hfem_get_solution": (ctypes.c_int, [_vp, _dp, _dp]),
     "hfem_assemble": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "hfem_assemble_vector": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "hfem_condense": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "hfem_tune_condensation": (ctypes.c_int, [_vp, ctypes.c_int, _ip, ctypes.POINTER(ctypes.c_float)]),
     "hfem_set_refinement": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int]),
@@ -69,6 +70,7 @@ EXPORTS = {
     "hfem_precond_factor": (ctypes.c_int, [_vp, ctypes.c_int]),
     "hfem_precond_apply": (ctypes.c_int, [_vp, _dp, _dp]),
     "hfem_set_precond_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "hfem_set_apply_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
     "hfem_gmres": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _ip, _dp, _ip]),
     "hfem_read_trace_solution": (ctypes.c_int, [_vp, _dp]),
     "hfem_update_solution": (ctypes.c_int, [_vp, ctypes.c_double]),
@@ -205,6 +207,11 @@ class HDGContext:
         n = self.prob.ne - e0 if n is None else n
         _check(load().hfem_assemble(self.h, e0, n))
 
+    def assemble_element_vector(self, e0=0, n=None):
+        """assembleElementVector: Ru and Rh only (no Jacobians)."""
+        n = self.prob.ne - e0 if n is None else n
+        _check(load().hfem_assemble_vector(self.h, e0, n))
+
     def schur_element_matrix_vector(self, e0=0, n=None, precision=FP64):
         n = self.prob.ne - e0 if n is None else n
         _check(load().hfem_condense(self.h, e0, n, precision))
@@ -318,6 +325,11 @@ class HDGContext:
         precision (sys.precPrecision)."""
         _check(load().hfem_set_precond_precision(self.h, prec_precision))
         _check(load().hfem_precond_factor(self.h, kind))
+
+    def set_apply_graph(self, use=True):
+        """Replay the BILU0 application as a captured HIP graph, or launch its
+        levels one by one (default)."""
+        _check(load().hfem_set_apply_graph(self.h, int(use)))
 
     def apply_preconditioner(self, x):
         x = _f64(x)

@@ -7,6 +7,7 @@
  * with the element data resident in device memory:
  *
  *   hfem_assemble        assembleElementMatrixVector  assembly/elementassembly.cpp:1481-1499
+ *   hfem_assemble_vector assembleElementVector        assembly/elementassembly.cpp:1501-1512
  *   hfem_condense        schurElementMatrixVector     assembly/elementschur.cpp:331-343
  *   hfem_assemble_linear_system
  *                        assembleLinearSystem         assembly/globalassembly.cpp:423-556
@@ -77,6 +78,11 @@ int hfem_get_solution(hfem_ctx *ctx, double *UDG, double *UH);
 
 /* Element assembly for elements [e0, e0+n) into the device element blocks. */
 int hfem_assemble(hfem_ctx *ctx, int e0, int n);
+/* assembleElementVector (assembly/elementassembly.cpp:1501-1512): the residual rows
+ * Ru, Rh of elements [e0, e0+n) only (fluxes without derivatives, no matrix
+ * phases; the reference still forms the Jacobians there).  D, F, K, H are left as
+ * they are.  Used by hfem_residual_norm and hfem_assemble_rhs. */
+int hfem_assemble_vector(hfem_ctx *ctx, int e0, int n);
 /* Static condensation of elements [e0, e0+n) on the device blocks: inv(D)F,
  * inv(D)Ru, AH (in place of H) and FH; D, Ru and Rh keep their assembled values. */
 int hfem_condense(hfem_ctx *ctx, int e0, int n, int precision);
@@ -178,6 +184,12 @@ int hfem_precond_factor(hfem_ctx *ctx, int kind);
  * MFMA, sweeps reading FP32 factors with FP64 arithmetic), 1 in double (default).
  * Takes effect at the next hfem_precond_factor. */
 int hfem_set_precond_precision(hfem_ctx *ctx, int prec_precision);
+/* No counterpart in the reference: 1 replays the BILU0 application (the
+ * level-by-level launches of both sweeps) as a HIP graph captured once per
+ * factorisation; 0 (default) launches the levels one by one.  Measured equal on
+ * MI355X: a sweep is bound by the latency of its chain of dependent levels, not
+ * by launch overhead. */
+int hfem_set_apply_graph(hfem_ctx *ctx, int use);
 /* y = M^{-1} x with host vectors of length nf*bsz (kernel/applyBILU0.m:1-52). */
 int hfem_precond_apply(hfem_ctx *ctx, const double *x, double *y);
 /* gmres: left-preconditioned restarted GMRES on Hg x = Rg from x = 0

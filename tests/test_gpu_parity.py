@@ -223,6 +223,30 @@ def test_condensation_by_parts(precision):
     ctx.close()
 
 
+@pytest.mark.parametrize("kind,nc,p", [("tet", 1, 3), ("tri", 2, 2), ("hex", 3, 1)])
+def test_assemble_vector_matches_full_assembly(kind, nc, p):
+    """assembleElementVector: the residual-only assembly writes the same Ru and Rh
+    as the full assembly (bit for bit: the same residual code reads the same flux
+    values) and leaves the element matrices alone."""
+    from hfem_amd.lib import HDGContext
+    pr = make_problem(kind, nc, p, seed=2)
+    ctx = HDGContext(pr)
+    ctx.assemble_element_matrix_vector()
+    full = ctx.read_elements()
+    pr2 = make_problem(kind, nc, p, seed=9)          # another state: Ru, Rh change, D.. must not
+    ctx.set_solution(pr2.UDG, pr2.UH, pr2.SH)
+    ctx.assemble_element_vector()
+    vec = ctx.read_elements()
+    for k in ("D", "F", "K", "H"):
+        assert np.array_equal(vec[k], full[k]), k
+    ctx.assemble_element_matrix_vector()
+    full2 = ctx.read_elements()
+    for k in ("Ru", "Rh"):
+        assert np.array_equal(vec[k], full2[k]), k
+        assert not np.array_equal(vec[k], full[k]), k
+    ctx.close()
+
+
 def test_errors_are_reported():
     from hfem_amd.lib import HDGContext, HfemError
     pr = make_problem("tri", 1, 1, seed=0)

@@ -87,6 +87,33 @@ def test_preconditioner_matches_oracle(oracle_mod, system, kind):
     assert rel(got, want) < 1e-10
 
 
+def test_bilu0_apply_graph_replays(oracle_mod, system):
+    """The BILU0 application replayed from its captured HIP graph equals the
+    level-by-level launches bit for bit, on every replay, with other device work
+    (a matvec, a GMRES solve, a re-factorisation) between the replays."""
+    from hfem_amd.lib import PREC_BILU0
+    O = oracle_mod
+    prob, ctx, M, Rg = system
+    rng = np.random.default_rng(5)
+    xs = [rng.standard_normal(len(Rg)) for _ in range(3)]
+    o2u, u2o = O.mdf_order(M)
+    ctx.set_ordering(o2u)
+    ctx.set_apply_graph(False)
+    ctx.compute_preconditioner(PREC_BILU0)
+    want = [ctx.apply_preconditioner(x) for x in xs]
+    ctx.set_apply_graph(True)
+    ctx.compute_preconditioner(PREC_BILU0)
+    for rep in range(3):
+        for x, w in zip(xs, want):
+            assert np.array_equal(ctx.apply_preconditioner(x), w), rep
+        ctx.bsr_matvec(xs[rep])
+        if rep == 1:
+            ctx.gmres(restart=10, tol=1e-8, maxit=50)
+        if rep == 2:
+            ctx.compute_preconditioner(PREC_BILU0)   # the graph is captured again
+    say("  graph replays equal the level launches")
+
+
 @pytest.mark.parametrize("kind", ["bilu0", "bj"])
 def test_single_precision_preconditioner(oracle_mod, system, kind):
     """precPrecision 0: factors in FP32 (BILU0 factored on the FP32 MFMA).  The
