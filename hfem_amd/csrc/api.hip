@@ -764,6 +764,13 @@ static int read_rowmajor(hfem_ctx *c, const double *src, int e0, int n, double *
 {
     const Dims &d = c->d;
     const long na = d.na, nb = d.nb, ldx = c->ldx;
+    if (!colmajor) {
+        // the last column alone: one strided copy
+        if (lastcol && n > 0)
+            HIPCHK(hipMemcpy2D(lastcol, sizeof(double), src + (long)e0 * na * ldx + nb, sizeof(double) * ldx,
+                               sizeof(double), (size_t)n * na, hipMemcpyDeviceToHost));
+        return 0;
+    }
     std::vector<double> tmp((size_t)n * na * ldx);
     HIPCHK(hipMemcpy(tmp.data(), src + (long)e0 * na * ldx, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost));
     for (long t = 0; t < n; t++)
