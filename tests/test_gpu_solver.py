@@ -5,7 +5,7 @@ pieces (oracle/solver_oracle.c) and its C++ drivers (oracle/newton_oracle.py);
 those are pinned by the properties in tests/test_solver_oracle.py.
 Tolerances: FP64 throughout; orderings and GMRES iteration counts must agree
 exactly, vectors to 1e-10 relative (different summation orders), Newton
-residual histories to 1e-8 relative above the 1e-12 level.
+residual histories to 1e-10 relative above the 1e-12 level.
 """
 import dataclasses
 
@@ -189,7 +189,7 @@ def test_update_and_residual_match_oracle(oracle_mod, system):
     r = ctx.residual_norm()
     rr = N.residual_norm(prob, Ur, UHr, prob.SH)
     say(f"  residual norm gpu {r:.6e} oracle {rr:.6e}")
-    assert abs(r - rr) <= 1e-8 * rr
+    assert abs(r - rr) <= 1e-11 * rr
 
 
 def test_newton_matches_oracle(oracle_mod):
@@ -204,7 +204,7 @@ def test_newton_matches_oracle(oracle_mod):
     say(f"  newton gpu {nit} it {hist[:, :2].tolist()}; oracle {[(h['old'], h['new']) for h in hr]}")
     assert conv and okr and nit == len(hr)
     for k, h in enumerate(hr):
-        assert abs(hist[k, 0] - h["old"]) <= 1e-8 * h["old"] + 1e-12
+        assert abs(hist[k, 0] - h["old"]) <= 1e-10 * h["old"] + 1e-12
         assert hist[k, 2] == h["gmres_iters"]
     assert rel(U, Ur) < 1e-10 and rel(UH, UHr) < 1e-10
     ctx.close()
@@ -221,7 +221,7 @@ def test_dirk_step_matches_oracle(oracle_mod):
                                      restart=30, gmres_tol=1e-12)
     say(f"  dirk: newton its gpu {nit} oracle {sum(len(h) for h in hist)}; UDG {rel(U, Ur):.2e} UH {rel(UH, UHr):.2e}")
     assert conv
-    assert rel(U, Ur) < 1e-9 and rel(UH, UHr) < 1e-9
+    assert rel(U, Ur) < 1e-10 and rel(UH, UHr) < 1e-10
     ctx.close()
 
 
@@ -246,7 +246,7 @@ def test_newton_line_search_retraction_matches_oracle(oracle_mod):
         assert hist[k, 4] == len(h["alphas"])
         assert hist[k, 3] == sum(h["alphas"])
         assert hist[k, 2] == h["gmres_iters"]
-        assert abs(hist[k, 0] - h["old"]) <= 1e-8 * h["old"] + 1e-12
+        assert abs(hist[k, 0] - h["old"]) <= 1e-10 * h["old"] + 1e-12
     U, UH = ctx.get_solution()
     assert rel(U, Ur) < 1e-10 and rel(UH, UHr) < 1e-10
     ctx.close()
