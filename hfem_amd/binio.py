@@ -11,12 +11,14 @@
   UDG block (the u part; the whole ``npv*nc`` block of every element when q is
   written, writeQflag 1), then the ``n`` values of UH.
 * a serial ``mesh.bin``, ``master.bin`` and ``app.bin`` (read only), in the
-  order readmeshstruct, readmasterstruct and readappstruct read them.
+  order readmeshstruct, readmasterstruct and readappstruct read them;
+* one rank's record of a parallel case (``mesh<k>.bin`` / ``sol<k>.bin`` holding
+  several ranks each, sliced by ``app.meshsize`` / ``app.solsize``), as the
+  parallel branches of readmeshstruct and readsolstruct read it.
 
 Array layouts are the reference's, which the library uses as they are: UDG
 element-major ``(ne, nc, npv)`` (MATLAB ``npv x nc x ne``), UH ``(nnodes, nch)``
-(MATLAB ``nch x nnodes``).  Parallel files (several ranks in one file, sliced
-by ``app.solsize``) are not read.
+(MATLAB ``nch x nnodes``).
 """
 import numpy as np
 
@@ -148,6 +150,98 @@ def read_mesh_bin(path, endian="<"):
         o += n
         out[name] = a if name == "dgnodes" else a.astype(np.int64)
     return out
+
+
+def rank_record(app, rank, kind):
+    """Where rank ``rank``'s record of a parallel case lies (the parallel branches
+    of readmeshstruct / readsolstruct, preprocessing/readbinaryfiles.cpp): the
+    file number k of ``mesh<k>.bin`` / ``sol<k>.bin``, the record's offset and
+    its length in doubles, from app.nproc, app.nfile and app.meshsize (kind
+    'mesh') or app.solsize (kind 'sol'), which hold nproc/nfile + 1 offsets per
+    file."""
+    nproc, nfile = int(app["nproc"]), int(app["nfile"])
+    if nproc < 2:
+        raise ValueError("a serial case (app.nproc = 1) has no rank records")
+    if nfile < 1 or nproc % nfile:
+        raise ValueError(f"nproc {nproc} is not a multiple of nfile {nfile}")
+    if not 0 <= rank < nproc:
+        raise ValueError(f"rank {rank} outside [0, {nproc})")
+    per = nproc // nfile
+    filenumber, file_rank = rank // per + 1, rank % per
+    sizes = np.asarray(app["meshsize" if kind == "mesh" else "solsize"], dtype=np.int64)
+    pos = (per + 1) * (filenumber - 1)
+    if sizes.size < pos + per + 1:
+        raise ValueError(f"app.{kind}size has {sizes.size} offsets, {nfile} files of {per} ranks need "
+                         f"{nfile * (per + 1)}")
+    start, end = int(sizes[pos + file_rank]), int(sizes[pos + file_rank + 1])
+    return filenumber, start, end - start
+
+
+def _record(path, start, length, endian):
+    raw = np.fromfile(path, dtype=_dt(endian)).astype(np.float64)
+    if start < 0 or length < 0 or start + length > raw.size:
+        raise ValueError(f"{path}: record [{start}, {start + length}) outside the file's {raw.size} values")
+    return raw[start:start + length]
+
+
+def read_mesh_bin_rank(prefix, app, rank, endian="<"):
+    """Rank ``rank``'s record of a parallel ``mesh<k>.bin`` (``prefix`` +
+    'mesh<k>.bin'), in the order the parallel branch of readmeshstruct reads it:
+    30 dimensions (nnbsd = ndims[20]), then elementtype, t, dgnodes, elcon, bf,
+    t2f, t2t, f, isEDGface as in a serial file, then nbsd (nnbsd), extintelem,
+    elem2cpu (ne each), extintface, face2cpu (nf each) and, for hybrid != 0,
+    extintent and ent2cpu (ndh each).  The record's length must be what the
+    dimensions give, as readmeshstruct checks."""
+    k, start, length = rank_record(app, rank, "mesh")
+    rec = _record(f"{prefix}mesh{k}.bin", start, length, endian)
+    if rec.size < MESH_HEADER:
+        raise ValueError(f"rank {rank}: record shorter than the {MESH_HEADER}-value header")
+    nd = rec[:MESH_HEADER]
+    if not np.all(nd == np.round(nd)):
+        raise ValueError(f"rank {rank}: header is not a list of integers (wrong byte order?)")
+    nd = nd.astype(np.int64)
+    names = ["dim", "ncd", "nfe", "nve", "nvf", "ne", "nf", "nv", "ndh", "npe", "npf", "nc", "ncu", "ncq", "ncp",
+             "nch", "nn", "nie", "nif", "nin", "nnbsd", "nxe", "nxf", "nxn"]
+    d = {key: int(nd[i]) for i, key in enumerate(names)}
+    d["hybrid"], d["nproc"] = int(nd[-2]), int(nd[-1])
+    if d["nproc"] != int(app["nproc"]):
+        raise ValueError(f"rank {rank}: the record is for {d['nproc']} ranks, app.bin for {app['nproc']}")
+    ne, nf, nfe, nve, nvf, npe, npf, ncd, ndh = (d[x] for x in ("ne", "nf", "nfe", "nve", "nvf", "npe", "npf", "ncd",
+                                                                 "ndh"))
+    shapes = [("elementtype", (ne,)), ("t", (ne, nve)), ("dgnodes", (ne, ncd, npe)), ("elcon", (ne, npf * nfe)),
+              ("bf", (ne, nfe)), ("t2f", (ne, nfe)), ("t2t", (ne, nfe)), ("f", (nf, nvf + 3)), ("isEDGface", (nf,)),
+              ("nbsd", (d["nnbsd"],)), ("extintelem", (ne,)), ("elem2cpu", (ne,)), ("extintface", (nf,)),
+              ("face2cpu", (nf,))]
+    if d["hybrid"] != 0:
+        shapes += [("extintent", (ndh,)), ("ent2cpu", (ndh,))]
+    need = MESH_HEADER + sum(int(np.prod(sh)) for _, sh in shapes)
+    if need != rec.size:
+        raise ValueError(f"rank {rank}: the dimensions need {need} values, the record holds {rec.size}")
+    out = {"ndims": d}
+    o = MESH_HEADER
+    for name, shp in shapes:
+        n = int(np.prod(shp))
+        a = rec[o:o + n].reshape(shp)
+        o += n
+        out[name] = a if name == "dgnodes" else a.astype(np.int64)
+    return out
+
+
+def read_sol_bin_rank(prefix, app, rank, endian="<"):
+    """Rank ``rank``'s record of a parallel ``sol<k>.bin``, as the parallel branch
+    of readsolstruct reads it: the ten-value header, then UDG, UH and PDG; the
+    counts must account for the record."""
+    k, start, length = rank_record(app, rank, "sol")
+    rec = _record(f"{prefix}sol{k}.bin", start, length, endian)
+    if rec.size < SOL_HEADER:
+        raise ValueError(f"rank {rank}: record shorter than the {SOL_HEADER}-value header")
+    nd = rec[:SOL_HEADER]
+    nudg, nuh, npdg = (int(v) for v in nd[:3])
+    if SOL_HEADER + nudg + nuh + npdg != rec.size:
+        raise ValueError(f"rank {rank}: header counts {nudg}+{nuh}+{npdg} do not match {rec.size - SOL_HEADER}")
+    o = SOL_HEADER
+    return dict(ndims=nd.astype(np.int64), UDG=rec[o:o + nudg], UH=rec[o + nudg:o + nudg + nuh],
+                PDG=rec[o + nudg + nuh:])
 
 
 def mesh_from_bin(m, nd, elemtype, porder):
