@@ -190,6 +190,9 @@ def ref_lib():
         L.ref_time.restype = ctypes.c_double
         L.ref_time.argtypes = [_ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp,
                                _dp, _dp, _ip, _ip, ctypes.c_int, ctypes.c_int]
+        if hasattr(L, "ref_time_mt"):
+            L.ref_time_mt.restype = ctypes.c_double
+            L.ref_time_mt.argtypes = L.ref_time.argtypes + [ctypes.c_int]
         _ref = L
     return _ref
 
@@ -238,6 +241,14 @@ class Reference(Oracle):
         p = self.prob
         return ref_lib().ref_time(*self._common(), p.ne, p.UH.shape[0], _d(self.dgnodes), _d(_c64(p.UDG)),
                                   _d(_c64(p.UH)), _d(_c64(p.SH)), _i(self.elcon), _i(self.bf), nelem, impl)
+
+    def time_mt(self, nelem, impl=1, threads=1):
+        """Wall-clock seconds for nelem assemble+condense calls on `threads` host
+        threads (each with its own reference structs)."""
+        p = self.prob
+        return ref_lib().ref_time_mt(*self._common(), p.ne, p.UH.shape[0], _d(self.dgnodes), _d(_c64(p.UDG)),
+                                     _d(_c64(p.UH)), _d(_c64(p.SH)), _i(self.elcon), _i(self.bf), nelem, impl,
+                                     threads)
 
 
 # --------------------------------------------------------------------------------------------

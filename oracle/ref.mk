@@ -10,7 +10,7 @@ IO := $(HERE)_ref/libhfemrefio.so
 all: $(OUT) $(IO)
 $(OUT): $(HERE)ref_harness.cpp
 	mkdir -p $(HERE)_ref
-	$(CXX) -O2 -fPIC -shared -w -I$(REF) -o $@ $(HERE)ref_harness.cpp -L$(MKLDIR) -lmkl_rt -Wl,-rpath,$(MKLDIR)
+	$(CXX) -O2 -fPIC -shared -w -fopenmp -I$(REF) -o $@ $(HERE)ref_harness.cpp -L$(MKLDIR) -lmkl_rt -Wl,-rpath,$(MKLDIR)
 # the reference's binary readers and writers (readbinaryfiles.cpp, ioutilities.cpp)
 $(IO): $(HERE)ref_io.cpp
 	mkdir -p $(HERE)_ref

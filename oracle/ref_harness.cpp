@@ -20,6 +20,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#include <omp.h>
 using namespace std;
 #include "preprocessing/digasopre.h"
 #include "preprocessing/errormsg.cpp"
@@ -170,6 +171,44 @@ extern "C" int ref_getq(const int* dims, const double* shapvt, const double* sha
     memcpy(UDG, &S->sol.UDG[0], sizeof(double) * ne * npv * nc);
     delete S;
     return 0;
+}
+
+// Timed element loop on nthreads host threads (the all-cores CPU baseline): each
+// thread has its own structs, as the reference's per-thread elems[] / temps[]
+// intend (main/digasoser.cpp:51-58), and takes every nthreads-th element of
+// [0, n) (repeating over the first ne); MKL runs sequentially inside the threads.
+// Returns the wall-clock seconds of the whole loop.
+extern "C" double ref_time_mt(const int* dims, const double* shapvt, const double* shapvg, const double* sgl,
+                              const double* shapft, const double* shapfg, const double* sfgl, const int* perm,
+                              const double* param, const double* ui, int ne, int nuh, const double* dgnodes,
+                              const double* UDG, const double* UH, const double* SH, const int* elcon,
+                              const int* bf, int n, int impl, int nthreads)
+{
+    vector<Setup*> S(nthreads);
+    for (int th = 0; th < nthreads; th++) {
+        S[th] = new Setup();
+        build(*S[th], dims, shapvt, shapvg, sgl, shapft, shapfg, sfgl, perm, param, ui, 0.0, 1.0, 0, ne, nuh,
+              dgnodes, UDG, UH, SH, elcon, bf);
+    }
+    int nb = dims[4] * dims[6] * dims[1];
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+#pragma omp parallel num_threads(nthreads)
+    {
+        const int th = omp_get_thread_num();
+        Setup* s = S[th];
+        double elemtimes[32] = {0}, schurtimes[8] = {0};
+        long long flops = 0;
+        for (int t = th; t < n; t += nthreads) {
+            int ie = t % ne;
+            assembleElementMatrixVector(s->elem, s->mesh, s->master, s->app, s->sol, s->temp, ie, elemtimes);
+            memcpy(s->elem.Rhonly, s->elem.Rh, sizeof(double) * nb);
+            schurElementMatrixVector(s->elem, s->mesh, s->app, s->temp, impl, ie, schurtimes, &flops);
+        }
+    }
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    for (Setup* s : S) delete s;
+    return (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
 }
 
 // Timed element loop (CPU baseline): assemble + condense elements [0, n) of the
