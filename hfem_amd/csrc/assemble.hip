@@ -200,7 +200,9 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
 {
     extern __shared__ double lds[];
     const int tid = threadIdx.x, nth = blockDim.x;
-    const int lane = tid & 63, wave = tid >> 6, nwave = nth >> 6, lr = lane & 15, lk = lane >> 4;
+    const int lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+    // wave-uniform: task and tile indices derived from it stay in scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), nwave = nth >> 6;
     const FluxLayout FL = flux_layout(d);
     const AsmLayout LY = asm_layout(d, SPILL);
     constexpr int NCU = ND + 2, NC = NCU * (ND + 1), ND1 = ND + 1;

@@ -135,7 +135,8 @@ dinv_mfma_kernel(int na, int e0, int n, const double *__restrict__ D, TO *__rest
     __shared__ T Z[NN * PS];              // pivot rows of the panel, Z[k*NN + c]; output staging
     __shared__ int label[NN], pof[NN], rowk[NN];
     __shared__ int sing_s;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 15, lk = lane >> 4;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lr = lane & 15,
+              lk = lane >> 4;
     // A last panel with fewer than 16 pivots leaves rows of Z unwritten that its
     // rank-16 update multiplies by the panel's zero columns: they must hold finite
     // values (LDS comes with whatever an earlier kernel left), so Z starts at zero

@@ -193,7 +193,7 @@ template <typename T, typename OUT>
 __device__ __forceinline__ void lds_block_product(const T *A, const T *B, int n, OUT out)
 {
     typedef BlkMfma<T> M;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nw = blockDim.x >> 6;
     const int lr = lane & 15, lk = lane >> 4, nt = (n + 15) / 16;
     for (int t = wave; t < nt * nt; t += nw) {
         const int i0 = (t % nt) * 16, j0 = (t / nt) * 16;
@@ -405,7 +405,7 @@ bilu0_sweep_kernel(int bsz, const int *__restrict__ rowpts, const int *__restric
     __shared__ double red[NW][128];
     __shared__ double tb[128];
     __shared__ int scol[FACTOR_MAXROW], stake[FACTOR_MAXROW];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const long b2 = (long)bsz * bsz;
     const int r0 = lane < bsz ? lane : 0, r1 = lane + 64 < bsz ? lane + 64 : 0;
     for (int t = blockIdx.x; t < nlev; t += gridDim.x) {
@@ -501,7 +501,7 @@ __global__ void __launch_bounds__(SV_THREADS)
 block_jacobi_kernel(int nrows, int bsz, const TM *__restrict__ Dinv, const double *__restrict__ x,
                     double *__restrict__ y)
 {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, wpb = blockDim.x >> 6;
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wpb = blockDim.x >> 6;
     const long b2 = (long)bsz * bsz;
     for (int r = blockIdx.x * wpb + wv; r < nrows; r += gridDim.x * wpb) {
         const double *xr = x + (long)r * bsz;

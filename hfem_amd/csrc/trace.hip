@@ -75,7 +75,73 @@ __global__ void sumsq_kernel(long n, const double *__restrict__ x, int square, d
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-// y = Hg x: one thread per output entry of a row block.
+// y = Hg x with one row block per workgroup of four waves.  Blocks are stored
+// column-major and a row's blocks are contiguous, so the row is one run of
+// nblk*bsz columns of bsz doubles each: column q starts at Hg + (k0*bsz + q)*bsz.
+// The four waves deal the columns among themselves (wave w takes q = w, w+4, ...),
+// lane r reads entry r of a column (r + 64 for the second chunk when bsz > 64)
+// and the wave-uniform x entry of that column (the wave index goes through
+// readfirstlane so that column bookkeeping, cols[] and x[] are scalar loads; as
+// vector loads they cost a third of the bandwidth); eight columns are in flight
+// per wave, read nontemporal (the matrix is streamed once per product).  The four
+// partial rows are summed through LDS.
+template <int RCH>
+__global__ __launch_bounds__(256) void bsr_matvec_rows_kernel(int bsz, const int *__restrict__ rowstart,
+                                                              const int *__restrict__ cols,
+                                                              const double *__restrict__ Hg,
+                                                              const double *__restrict__ x, double *__restrict__ y)
+{
+    __shared__ double part[4][64 * RCH];
+    const int i = blockIdx.x, lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int k0 = rowstart[i];
+    const long ncol = (long)(rowstart[i + 1] - k0) * bsz;
+    const double *base = Hg + (long)k0 * bsz * bsz;
+    bool live[RCH];
+    double acc[RCH];
+#pragma unroll
+    for (int h = 0; h < RCH; h++) {
+        live[h] = lane + 64 * h < bsz;
+        acc[h] = 0.0;
+    }
+    // (kq, cq): block within the row and column within the block of column q
+    int kq = 0, cq = w;
+    while (cq >= bsz) cq -= bsz, kq++;
+    long q = w;
+    auto step = [&]() {
+        cq += 4;
+        while (cq >= bsz) cq -= bsz, kq++;
+    };
+    for (; q + 28 < ncol; q += 32) {
+        double v[8][RCH], xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            xv[u] = x[(long)cols[k0 + kq] * bsz + cq];
+            const double *col = base + (q + 4 * u) * bsz + lane;
+#pragma unroll
+            for (int h = 0; h < RCH; h++) v[u][h] = live[h] ? __builtin_nontemporal_load(col + 64 * h) : 0.0;
+            step();
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+#pragma unroll
+            for (int h = 0; h < RCH; h++) acc[h] += v[u][h] * xv[u];
+    }
+    for (; q < ncol; q += 4) {
+        const double xq = x[(long)cols[k0 + kq] * bsz + cq];
+        const double *col = base + q * bsz + lane;
+#pragma unroll
+        for (int h = 0; h < RCH; h++)
+            if (live[h]) acc[h] += __builtin_nontemporal_load(col + 64 * h) * xq;
+        step();
+    }
+#pragma unroll
+    for (int h = 0; h < RCH; h++) part[w][lane + 64 * h] = acc[h];
+    __syncthreads();
+    for (int r = threadIdx.x; r < bsz; r += 256)
+        y[(long)i * bsz + r] = (part[0][r] + part[1][r]) + (part[2][r] + part[3][r]);
+}
+
+// y = Hg x: one thread per output entry of a row block (blocks wider than 128).
 __global__ void bsr_matvec_kernel(int nrow, int bsz, const int *__restrict__ rowstart, const int *__restrict__ cols,
                                   const double *__restrict__ Hg, const double *__restrict__ x, double *__restrict__ y)
 {
@@ -123,8 +189,14 @@ hipError_t launch_sumsq(long n, const double *x, int square, double *partial, in
 hipError_t launch_bsr_matvec(int nrow, int bsz, const int *rowstart, const int *cols, const double *Hg,
                              const double *x, double *y, hipStream_t s)
 {
-    hipLaunchKernelGGL(bsr_matvec_kernel, dim3(grid_for((long)nrow * bsz)), dim3(256), 0, s, nrow, bsz, rowstart,
-                       cols, Hg, x, y);
+    if (nrow <= 0) return hipSuccess;
+    if (bsz <= 64)
+        hipLaunchKernelGGL(bsr_matvec_rows_kernel<1>, dim3(nrow), dim3(256), 0, s, bsz, rowstart, cols, Hg, x, y);
+    else if (bsz <= 128)
+        hipLaunchKernelGGL(bsr_matvec_rows_kernel<2>, dim3(nrow), dim3(256), 0, s, bsz, rowstart, cols, Hg, x, y);
+    else
+        hipLaunchKernelGGL(bsr_matvec_kernel, dim3(grid_for((long)nrow * bsz)), dim3(256), 0, s, nrow, bsz, rowstart,
+                           cols, Hg, x, y);
     return hipGetLastError();
 }
 
