@@ -124,7 +124,7 @@ extern "C" int hfem_destroy(hfem_ctx *c)
                     c->X, c->K, c->Hx, c->Ru, c->Rh, c->ru2, c->info, c->dinv, c->spill, c->rwork, c->fluxbuf, c->Hg, c->Rg,
                     c->r, c->partial, c->xv, c->yv, c->sv.d_o2u, c->sv.d_u2o, c->sv.d_lrows, c->sv.d_urows,
                     c->sv.Mg, c->sv.Mg32, c->sv.gy, c->sv.Dinv, c->sv.Cw, c->sv.d_info, c->sv.x, c->sv.V, c->sv.b, c->sv.t, c->sv.dots,
-                    c->sv.coef, c->sv.UDGi, c->qn_dinv, c->qn_K, c->rdmax, c->rcmax, c->rdprev, c->rcount};
+                    c->sv.coef, c->sv.UDGi, c->sv.sf_flags, c->sv.sf_err, c->qn_dinv, c->qn_K, c->rdmax, c->rcmax, c->rdprev, c->rcount};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (auto &ev : c->ev)

@@ -94,6 +94,12 @@ hipError_t launch_bilu0_usolve_level(int bsz, const int *rowpts, const int *coli
                                      int nlev, const float *Mg, const double *y, double *v, hipStream_t s);
 hipError_t launch_block_jacobi(int nrows, int bsz, const float *Dinv, const double *x, double *y, hipStream_t s);
 hipError_t launch_to_float(long n, const double *x, float *y, hipStream_t s);
+hipError_t launch_bilu0_syncfree(int bsz, const int *rowpts, const int *colind, const int *u2o, const int *lorder,
+                                 const int *uorder, int n, const double *Mg, const double *x, double *t, double *y,
+                                 unsigned *flags, unsigned *sync, int nwg, int spin_limit, int *err, hipStream_t s);
+hipError_t launch_bilu0_syncfree(int bsz, const int *rowpts, const int *colind, const int *u2o, const int *lorder,
+                                 const int *uorder, int n, const float *Mg, const double *x, double *t, double *y,
+                                 unsigned *flags, unsigned *sync, int nwg, int spin_limit, int *err, hipStream_t s);
 int multidot_blocks(long n);
 hipError_t launch_multidot(long n, int k, const double *V, long ldv, const double *w, double *partial, double *out,
                            hipStream_t s);
@@ -137,6 +143,12 @@ struct SolverState {
     bool graph_dirty = true;
     int use_graph = 0;
     double *gy = nullptr;
+    // the BILU0 application without level launches (sweep_mode 1, default): 2 nf ready
+    // flags, then the epoch and two ticket counters; a flag set when a bounded wait
+    // gave up
+    int sweep_mode = 1, spin_limit = 1 << 20, sweep_wgs = 1024;
+    unsigned *sf_flags = nullptr;
+    int *sf_err = nullptr;
     double *Dinv = nullptr;             // inverted diagonal blocks of Hg (MDF weights)
     double *Cw = nullptr;               // MDF block weights ||inv(H_ii) H_ij||_F
     int *d_info = nullptr;

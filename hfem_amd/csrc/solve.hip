@@ -305,11 +305,58 @@ extern "C" int hfem_set_precond_precision(hfem_ctx *c, int prec_precision)
     return 0;
 }
 
-// The two BILU0 sweeps, one launch per level: forward into s.t, backward into y.
+// Buffers of the sync-free sweeps, allocated and cleared at their first use (before
+// any graph capture: the clearing is a memset).
+static int ensure_syncfree(hfem_ctx *c)
+{
+    SolverState &s = c->sv;
+    const int n = c->hd.nf;
+    if (s.lptr.empty() || s.lptr.back() != n || s.uptr.empty() || s.uptr.back() != n)
+        return hfem_fail(-3, "sync-free BILU0 sweeps: the level lists do not cover the %d rows", n);
+    if (!s.sf_flags) {
+        HIPCHK(hfem_dalloc(&s.sf_flags, (size_t)2 * n + 3));
+        HIPCHK(hipMemset(s.sf_flags, 0, sizeof(unsigned) * ((size_t)2 * n + 3)));
+    }
+    if (!s.sf_err) {
+        HIPCHK(hfem_dalloc(&s.sf_err, 1));
+        HIPCHK(hipMemset(s.sf_err, 0, sizeof(int)));
+    }
+    return 0;
+}
+
+// Whether a bounded wait of the sync-free sweeps gave up since the last check (the
+// applications since then are invalid); clears the flag.  Synchronises the stream.
+static int check_syncfree(hfem_ctx *c, const char *who)
+{
+    SolverState &s = c->sv;
+    if (!s.sf_err) return 0;
+    int e = 0;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(&e, s.sf_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (!e) return 0;
+    HIPCHK(hipMemset(s.sf_err, 0, sizeof(int)));
+    return hfem_fail(-5, "%s: a wait of the sync-free BILU0 sweeps gave up after %d polls; the preconditioner "
+                     "applications are invalid", who, s.spin_limit);
+}
+
+// The two BILU0 sweeps: forward into s.t, backward into y; one launch per level
+// (sweep_mode 0) or one persistent launch per sweep (sweep_mode 1).
 static int bilu0_sweeps(hfem_ctx *c, const double *x, double *y)
 {
     SolverState &s = c->sv;
     const int bsz = trace_bsz(c);
+    if (s.sweep_mode == 1) {
+        const int n = c->hd.nf;
+        if (s.prec_fp32)
+            HIPCHK(launch_bilu0_syncfree(bsz, c->rowstart, c->cols, s.d_u2o, s.d_lrows, s.d_urows, n, s.Mg32, x, s.t, y,
+                                         s.sf_flags, s.sf_flags + 2 * (size_t)n, s.sweep_wgs, s.spin_limit, s.sf_err,
+                                         c->stream));
+        else
+            HIPCHK(launch_bilu0_syncfree(bsz, c->rowstart, c->cols, s.d_u2o, s.d_lrows, s.d_urows, n, s.Mg, x, s.t, y,
+                                         s.sf_flags, s.sf_flags + 2 * (size_t)n, s.sweep_wgs, s.spin_limit, s.sf_err,
+                                         c->stream));
+        return 0;
+    }
     for (size_t l = 0; l + 1 < s.lptr.size(); l++) {
         const int *rows = s.d_lrows + s.lptr[l];
         const int nl = s.lptr[l + 1] - s.lptr[l];
@@ -322,6 +369,22 @@ static int bilu0_sweeps(hfem_ctx *c, const double *x, double *y)
         if (s.prec_fp32) HIPCHK(launch_bilu0_usolve_level(bsz, c->rowstart, c->cols, s.d_u2o, rows, nl, s.Mg32, s.t, y, c->stream));
         else HIPCHK(launch_bilu0_usolve_level(bsz, c->rowstart, c->cols, s.d_u2o, rows, nl, s.Mg, s.t, y, c->stream));
     }
+    return 0;
+}
+
+// How the BILU0 sweeps run: mode 1 (default) one persistent launch per sweep whose
+// workgroups wait on per-row ready flags, each wait bounded by spin_limit polls (0:
+// the default 2^20), a wait that gives up making the next hfem_gmres /
+// hfem_precond_apply fail; mode 0 one launch per level.  Same bits either way.  No
+// counterpart in the reference.
+extern "C" int hfem_set_bilu0_sweep(hfem_ctx *c, int mode, int spin_limit)
+{
+    if (!c) return hfem_fail(-1, "hfem_set_bilu0_sweep: null context");
+    if (mode != 0 && mode != 1) return hfem_fail(-1, "hfem_set_bilu0_sweep: mode %d (0 level launches, 1 sync-free)", mode);
+    if (spin_limit < 0) return hfem_fail(-1, "hfem_set_bilu0_sweep: spin_limit %d", spin_limit);
+    c->sv.sweep_mode = mode;
+    c->sv.spin_limit = spin_limit ? spin_limit : (1 << 20);
+    c->sv.graph_dirty = true;
     return 0;
 }
 
@@ -348,6 +411,7 @@ static int precond_apply_dev(hfem_ctx *c, const double *x, double *y)
         else HIPCHK(launch_block_jacobi(n, bsz, s.Mg, x, y, c->stream));
     } else if (s.prec == 0) {
         const long bytes = sizeof(double) * N;
+        if (s.sweep_mode == 1) RC(ensure_syncfree(c));
         if (s.use_graph) {
             // the level launches of both sweeps as one graph from c->yv into s.gy
             // (captured after each factorisation; no memset nodes in it)
@@ -391,6 +455,7 @@ extern "C" int hfem_precond_apply(hfem_ctx *c, const double *x, double *y)
     RC(precond_apply_dev(c, c->xv, c->yv));
     HIPCHK(hipMemcpyAsync(y, c->yv, sizeof(double) * N, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    RC(check_syncfree(c, "hfem_precond_apply"));
     return 0;
 }
 
@@ -521,6 +586,7 @@ extern "C" int hfem_gmres(hfem_ctx *c, int restart, double tol, int maxit, int o
         if (fl >= 0) break;
     }
     HIPCHK(hipStreamSynchronize(c->stream));
+    RC(check_syncfree(c, "hfem_gmres"));
     if (iters) *iters = it;
     if (flag) *flag = fl;
     s.ms_gmres = now_ms() - t0;

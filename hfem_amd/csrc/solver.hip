@@ -495,6 +495,202 @@ bilu0_sweep_kernel(int bsz, const int *__restrict__ rowpts, const int *__restric
     }
 }
 
+// Wait until *f holds `epoch`.  The flags and the vector entries they guard are
+// read and written only by agent-scope atomics (coherent across the XCDs without
+// whole-L2 write-backs or invalidations, which an acquire / release pair costs
+// here per row).  At most spin_limit polls; a wait that runs out, or any wait once
+// another has (*err set), sets *err and returns.
+__device__ __forceinline__ void wait_ready(const unsigned *f, unsigned epoch, int spin_limit, int *err)
+{
+    // (the polled values go through readfirstlane: a uniform loop)
+    int spins = 0;
+    while ((unsigned)__builtin_amdgcn_readfirstlane(
+               __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != epoch) {
+        if (++spins > spin_limit ||
+            __builtin_amdgcn_readfirstlane(__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+// One BILU0 sweep without level launches (kernel/applyBILU0.m), with the per-row
+// arithmetic of bilu0_sweep_kernel (the same bits).  The rows go in `order` -- the
+// level lists concatenated, so every row comes after the rows it reads -- to
+// persistent workgroups through an atomic ticket.  A wave issues the loads of a
+// neighbour block (bsz <= 64: all 64 columns into registers) before it waits for
+// that neighbour's ready flag, so the block's latency overlaps the wait.  Vector
+// entries written in the sweep are stored and read by agent-scope atomics; the row's
+// flag is stored once those stores have completed (workgroup fence, then barrier).
+// Flags hold the
+// epoch of the current application (sweep_reset_kernel bumps it), so they are never
+// cleared.  A wait that gives up (wait_ready) leaves a wrong result and *err set,
+// which the host reports; every workgroup still runs to the end of the tickets.
+// No deadlock: a workgroup waits only on rows of smaller tickets, which were drawn
+// by workgroups already running.
+template <bool BACKWARD, bool SMALL, typename TM>
+__global__ void __launch_bounds__(SV_THREADS)
+bilu0_sweep_syncfree_kernel(int bsz, const int *__restrict__ rowpts, const int *__restrict__ colind,
+                            const int *__restrict__ u2o, const int *__restrict__ order, int n,
+                            const TM *__restrict__ Mg, const double *__restrict__ b, double *z, unsigned *flags,
+                            const unsigned *epoch_p, unsigned *counter, int spin_limit, int *err)
+{
+    constexpr int NW = SV_THREADS / 64;
+    __shared__ double red[NW][128];
+    __shared__ double tb[128];
+    __shared__ int scol[FACTOR_MAXROW], stake[FACTOR_MAXROW];
+    __shared__ int sticket;
+    const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const long b2 = (long)bsz * bsz;
+    const int r0 = lane < bsz ? lane : 0, r1 = lane + 64 < bsz ? lane + 64 : 0;
+    const unsigned epoch = __hip_atomic_load(epoch_p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // No lane-conditional code may sit between the last barrier of one row and the
+    // first of the next: with the flag store and the ticket draw both under
+    // "tid == 0", the structurizer merged them across the back edge into an inner
+    // loop that only lane 0 leaves, so the other lanes re-ran the old row forever.
+    // Both are therefore done by the whole of wave 0 (a wave-uniform condition):
+    // the draw adds 1 from lane 0 and 0 from the others, and lane 0's old value is
+    // the ticket; every lane stores the same flag.
+    while (true) {
+        if (wv == 0) sticket = __builtin_amdgcn_readfirstlane(atomicAdd(counter, lane == 0 ? 1u : 0u));
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(sticket);
+        if (t >= n) break;
+        const int r = order[t], o = u2o[r], k0 = rowpts[r], len = min(rowpts[r + 1] - k0, FACTOR_MAXROW);
+        const TM *Ui = Mg + (long)k0 * b2;
+        if (tid < len) {
+            const int c = colind[k0 + tid];
+            scol[tid] = c;
+            stake[tid] = BACKWARD ? (u2o[c] > o) : (u2o[c] < o);
+        }
+        // the right-hand side of entries lane and lane + 64, for wave 0, which
+        // stores the row (so that its own wait covers every store of the row)
+        const double bv0 = wv == 0 && lane < bsz ? b[(long)r * bsz + lane] : 0.0;
+        const double bv1 = wv == 0 && lane + 64 < bsz ? b[(long)r * bsz + lane + 64] : 0.0;
+        constexpr int ND = (SMALL && BACKWARD) ? 64 / NW : 1;
+        double dv[ND];
+        if (SMALL && BACKWARD) {
+#pragma unroll
+            for (int j = 0; j < ND; j++) dv[j] = Ui[r0 + (long)bsz * min(wv + NW * j, bsz - 1)];
+        }
+        __syncthreads();
+        double acc0 = 0.0, acc1 = 0.0;
+        int sel = 0;
+        for (int q = 1; q < len; q++) {
+            if (!__builtin_amdgcn_readfirstlane(stake[q])) continue;
+            if ((sel++ % NW) != wv) continue;
+            const int c = __builtin_amdgcn_readfirstlane(scol[q]);
+            const TM *B = Mg + (long)(k0 + q) * b2;
+            const double *zc = z + (long)c * bsz;
+            if (SMALL) {
+                double v0[64];
+#pragma unroll
+                for (int j = 0; j < 64; j++) v0[j] = B[r0 + (long)bsz * min(j, bsz - 1)];
+                wait_ready(flags + c, epoch, spin_limit, err);
+                const double z0 = lane < bsz ? __hip_atomic_load(zc + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                             : 0.0;
+                double s0 = 0.0;
+#pragma unroll
+                for (int j = 0; j < 64; j++) {
+                    double zj = wave_bcast(z0, j);
+                    zj = j < bsz ? zj : 0.0;
+                    s0 += v0[j] * zj;
+                }
+                if (lane < bsz) acc0 += s0;
+            } else {
+                wait_ready(flags + c, epoch, spin_limit, err);
+                const double z0 =
+                    lane < bsz ? __hip_atomic_load(zc + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+                const double z1 =
+                    lane + 64 < bsz ? __hip_atomic_load(zc + lane + 64, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                    : 0.0;
+                wave_block_matvec_ch<16, true>(bsz, B, z0, z1, acc0, acc1);
+            }
+        }
+        red[wv][lane] = acc0;
+        red[wv][lane + 64] = acc1;
+        __syncthreads();
+        if (wv == 0) {
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int i = lane + 64 * h;
+                if (i < bsz) {
+                    double sum = 0.0;
+#pragma unroll
+                    for (int w = 0; w < NW; w++) sum += red[w][i];
+                    const double v = (h ? bv1 : bv0) - sum;
+                    if (BACKWARD) tb[i] = v;
+                    else __hip_atomic_store(z + (long)r * bsz + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+        }
+        if (BACKWARD) {
+            __syncthreads();
+            double a0 = 0.0, a1 = 0.0;
+            if (SMALL) {
+#pragma unroll
+                for (int j = 0; j < ND; j++) {
+                    const int cc = wv + NW * j;
+                    a0 += dv[j] * (cc < bsz ? tb[min(cc, bsz - 1)] : 0.0);
+                }
+            } else {
+                for (int c0 = wv; c0 < bsz; c0 += 8 * NW) {
+                    double v0[8], v1[8];
+#pragma unroll
+                    for (int j = 0; j < 8; j++) {
+                        const long col = (long)bsz * min(c0 + j * NW, bsz - 1);
+                        v0[j] = Ui[r0 + col];
+                        v1[j] = Ui[r1 + col];
+                    }
+#pragma unroll
+                    for (int j = 0; j < 8; j++) {
+                        const int cc = c0 + j * NW;
+                        const double tc = cc < bsz ? tb[min(cc, bsz - 1)] : 0.0;
+                        a0 += v0[j] * tc;
+                        a1 += v1[j] * tc;
+                    }
+                }
+            }
+            if (lane >= bsz) a0 = 0.0;
+            if (lane + 64 >= bsz) a1 = 0.0;
+            red[wv][lane] = a0;
+            red[wv][lane + 64] = a1;
+            __syncthreads();
+            if (wv == 0) {
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int i = lane + 64 * h;
+                    if (i < bsz) {
+                        double sum = 0.0;
+#pragma unroll
+                        for (int w = 0; w < NW; w++) sum += red[w][i];
+                        __hip_atomic_store(z + (long)r * bsz + i, sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                }
+            }
+        }
+        // publish row r: wave 0 stored it; once its stores have completed (vmcnt 0:
+        // device-scope stores are acknowledged at the coherence point) the flag goes
+        // out.  (The barrier also frees red, tb and the column lists for the next row.)
+        if (wv == 0) __builtin_amdgcn_s_waitcnt(0x0F70);   // s_waitcnt vmcnt(0)
+        __syncthreads();
+        if (wv == 0) __hip_atomic_store(flags + r, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// A new epoch for the flags of the next application and fresh tickets for its two
+// sweeps (a kernel, not a memset, so that the application can be captured in a graph).
+__global__ void sweep_reset_kernel(unsigned *sync)
+{
+    if (threadIdx.x == 0) {
+        unsigned e = sync[0] + 1;
+        sync[0] = e == 0 ? 1 : e;
+        sync[1] = 0;
+        sync[2] = 0;
+    }
+}
+
 // Block-Jacobi application y_i = inv(H_ii) x_i.  One wave per row.
 template <typename TM>
 __global__ void __launch_bounds__(SV_THREADS)
@@ -722,6 +918,46 @@ hipError_t launch_bilu0_usolve_level(int bsz, const int *rowpts, const int *coli
                                      int nlev, const float *Mg, const double *y, double *v, hipStream_t s)
 {
     return sweep_level<true>(bsz, rowpts, colind, u2o, rows, nlev, Mg, y, v, s);
+}
+
+// Both sweeps of one BILU0 application without level launches: x -> t (forward,
+// rows in lorder) -> y (backward, rows in uorder).  sync: [0] epoch, [1], [2] the
+// tickets of the two sweeps; flags: n forward then n backward ready flags.
+template <typename TM>
+static hipError_t sweeps_syncfree(int bsz, const int *rowpts, const int *colind, const int *u2o, const int *lorder,
+                                  const int *uorder, int n, const TM *Mg, const double *x, double *t, double *y,
+                                  unsigned *flags, unsigned *sync, int nwg, int spin_limit, int *err, hipStream_t s)
+{
+    if (bsz > 128 || n <= 0 || nwg <= 0) return hipErrorInvalidValue;
+    const int g = std::min(n, nwg);
+    hipLaunchKernelGGL(sweep_reset_kernel, dim3(1), dim3(64), 0, s, sync);
+    if (bsz <= 64) {
+        hipLaunchKernelGGL((bilu0_sweep_syncfree_kernel<false, true, TM>), dim3(g), dim3(SV_THREADS), 0, s, bsz, rowpts,
+                           colind, u2o, lorder, n, Mg, x, t, flags, sync, sync + 1, spin_limit, err);
+        hipLaunchKernelGGL((bilu0_sweep_syncfree_kernel<true, true, TM>), dim3(g), dim3(SV_THREADS), 0, s, bsz, rowpts,
+                           colind, u2o, uorder, n, Mg, t, y, flags + n, sync, sync + 2, spin_limit, err);
+    } else {
+        hipLaunchKernelGGL((bilu0_sweep_syncfree_kernel<false, false, TM>), dim3(g), dim3(SV_THREADS), 0, s, bsz,
+                           rowpts, colind, u2o, lorder, n, Mg, x, t, flags, sync, sync + 1, spin_limit, err);
+        hipLaunchKernelGGL((bilu0_sweep_syncfree_kernel<true, false, TM>), dim3(g), dim3(SV_THREADS), 0, s, bsz,
+                           rowpts, colind, u2o, uorder, n, Mg, t, y, flags + n, sync, sync + 2, spin_limit, err);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_bilu0_syncfree(int bsz, const int *rowpts, const int *colind, const int *u2o, const int *lorder,
+                                 const int *uorder, int n, const double *Mg, const double *x, double *t, double *y,
+                                 unsigned *flags, unsigned *sync, int nwg, int spin_limit, int *err, hipStream_t s)
+{
+    return sweeps_syncfree(bsz, rowpts, colind, u2o, lorder, uorder, n, Mg, x, t, y, flags, sync, nwg, spin_limit,
+                           err, s);
+}
+hipError_t launch_bilu0_syncfree(int bsz, const int *rowpts, const int *colind, const int *u2o, const int *lorder,
+                                 const int *uorder, int n, const float *Mg, const double *x, double *t, double *y,
+                                 unsigned *flags, unsigned *sync, int nwg, int spin_limit, int *err, hipStream_t s)
+{
+    return sweeps_syncfree(bsz, rowpts, colind, u2o, lorder, uorder, n, Mg, x, t, y, flags, sync, nwg, spin_limit,
+                           err, s);
 }
 
 template <typename TM>

@@ -71,6 +71,7 @@ EXPORTS = {
     "hfem_precond_apply": (ctypes.c_int, [_vp, _dp, _dp]),
     "hfem_set_precond_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
     "hfem_set_apply_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "hfem_set_bilu0_sweep": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "hfem_gmres": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _ip, _dp, _ip]),
     "hfem_read_trace_solution": (ctypes.c_int, [_vp, _dp]),
     "hfem_update_solution": (ctypes.c_int, [_vp, ctypes.c_double]),
@@ -330,6 +331,11 @@ class HDGContext:
         """Replay the BILU0 application as a captured HIP graph, or launch its
         levels one by one (default)."""
         _check(load().hfem_set_apply_graph(self.h, int(use)))
+
+    def set_bilu0_sweep(self, mode, spin_limit=0):
+        """BILU0 sweeps as one persistent launch per sweep with bounded per-row
+        waits (mode 1, default) or level by level (mode 0)."""
+        _check(load().hfem_set_bilu0_sweep(self.h, int(mode), int(spin_limit)))
 
     def apply_preconditioner(self, x):
         x = _f64(x)

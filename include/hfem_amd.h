@@ -190,6 +190,13 @@ int hfem_set_precond_precision(hfem_ctx *ctx, int prec_precision);
  * MI355X: a sweep is bound by the latency of its chain of dependent levels, not
  * by launch overhead. */
 int hfem_set_apply_graph(hfem_ctx *ctx, int use);
+/* No counterpart in the reference (kernel/applyBILU0.m runs its sweeps row by
+ * row): mode 1 (default) runs each BILU0 sweep as one persistent launch whose
+ * workgroups take the rows in level order and wait on per-row ready flags, each
+ * wait bounded by spin_limit polls (0: 2^20); mode 0 launches the sweeps level by
+ * level.  Same results bit for bit.  A wait that gives up makes the next
+ * hfem_gmres / hfem_precond_apply return an error. */
+int hfem_set_bilu0_sweep(hfem_ctx *ctx, int mode, int spin_limit);
 /* y = M^{-1} x with host vectors of length nf*bsz (kernel/applyBILU0.m:1-52). */
 int hfem_precond_apply(hfem_ctx *ctx, const double *x, double *y);
 /* gmres: left-preconditioned restarted GMRES on Hg x = Rg from x = 0

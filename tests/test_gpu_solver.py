@@ -170,6 +170,113 @@ def test_gmres_matches_oracle(oracle_mod, system, orthog):
     assert rel(M.matvec(x), Rg) < 1e-6
 
 
+@pytest.mark.parametrize("oblique", [False, True], ids=["axis_flow", "oblique_flow"])
+def test_restarted_gmres_stall_belongs_to_the_system(oracle_mod, oblique):
+    """GMRES(30) with BILU0 and approximate MDF on 162 p=3 tets (the solve that
+    follows the element-path bench case).  With the freestream along the box
+    axes the flow is tangential to four box faces, where the normal eigenvalue
+    of the far-field flux crosses zero: the trace system is nearly singular and
+    indefinite, and restarted GMRES stagnates -- on the CPU oracle exactly as on
+    the GPU, with the same residual history.  With the flow turned off the axes
+    the same mesh converges, in the same number of iterations on both."""
+    from hfem_amd.lib import HDGContext, PREC_BILU0
+    O = oracle_mod
+    kw = dict(angle=0.3, elevation=0.2) if oblique else {}
+    prob = make_problem("tet", 3, 3, seed=0, **kw)
+    ctx = HDGContext(prob)
+    ctx.assemble_linear_system()
+    Hg, Rg = ctx.read_system()
+    M = O.BSR.from_problem(prob, Hg)
+    o2u, u2o = O.mdf_order(M)
+    ctx.set_ordering(o2u)
+    ctx.compute_preconditioner(PREC_BILU0)
+    flag, it, rev = ctx.gmres(restart=30, tol=1e-8, maxit=90)
+    ctx.close()
+    Mg, _ = O.bilu0_factor(M, o2u, u2o)
+    _, fr, itr, revr = O.gmres(M, Rg, O.PREC_BILU0, Mg, o2u, u2o, restart=30, tol=1e-8, maxit=90)
+    say(f"  {'oblique' if oblique else 'axis'} flow: gpu {it} it flag {flag} final {rev[-1]:.3e}; "
+        f"oracle {itr} it flag {fr} final {revr[-1]:.3e}")
+    if oblique:
+        assert flag == fr == 0 and it == itr
+    else:
+        assert flag == fr == 1 and it == itr == 90 and rev[-1] > 1e-5 and revr[-1] > 1e-5
+    # (the nearly singular axis-flow system amplifies the rounding of two summation
+    # orders to a few 1e-6 within the first cycle)
+    np.testing.assert_allclose(rev[:30], revr[:30], rtol=1e-6 if oblique else 1e-4)
+    np.testing.assert_allclose(rev, revr, rtol=1e-3)
+
+
+@pytest.mark.parametrize("prec_precision", [1, 0], ids=["fp64_factors", "fp32_factors"])
+def test_syncfree_sweeps_match_level_sweeps(oracle_mod, system, prec_precision):
+    """The BILU0 application with one persistent launch per sweep (rows taken in
+    level order, per-row ready flags, bounded waits) returns the level-by-level
+    result bit for bit, over repeated applications (the flags carry each
+    application's epoch) and inside a replayed graph; GMRES gives the same history."""
+    from hfem_amd.lib import PREC_BILU0
+    O = oracle_mod
+    prob, ctx, M, Rg = system
+    o2u, u2o = O.mdf_order(M)
+    ctx.set_ordering(o2u)
+    ctx.compute_preconditioner(PREC_BILU0, prec_precision=prec_precision)
+    rng = np.random.default_rng(3)
+    xs = [rng.standard_normal(Rg.size) for _ in range(3)]
+    ctx.set_bilu0_sweep(0)
+    ref = [ctx.apply_preconditioner(x) for x in xs]
+    flag0, it0, rev0 = ctx.gmres(restart=10, tol=1e-10, maxit=500)
+    try:
+        ctx.set_bilu0_sweep(1)
+        for use_graph in (False, True):
+            ctx.set_apply_graph(use_graph)
+            for rep in range(2):
+                for x, r in zip(xs, ref):
+                    assert np.array_equal(ctx.apply_preconditioner(x), r), (use_graph, rep)
+        ctx.set_apply_graph(False)
+        flag1, it1, rev1 = ctx.gmres(restart=10, tol=1e-10, maxit=500)
+    finally:
+        ctx.set_bilu0_sweep(1)
+        ctx.set_apply_graph(False)
+    say(f"  sync-free sweeps: gmres {it1} it (level sweeps {it0})")
+    assert flag0 == flag1 == 0 and it0 == it1 and np.array_equal(rev0, rev1)
+
+
+@pytest.mark.parametrize("case", [("tet", 4, 3), ("hex", 3, 2)], ids=lambda c: f"{c[0]}{c[1]}p{c[2]}")
+def test_syncfree_sweeps_many_levels(case):
+    """Sync-free against level sweeps where the level chain is long (384 p=3 tets,
+    block 50; periodic p=2 hexes, block 45): bit-identical applications and GMRES
+    histories.  Then the give-up path, forced from the host with a one-poll wait
+    limit: the kernel still ends, the application reports the failure, and the
+    next one with the default limit is exact again."""
+    from hfem_amd.lib import HDGContext, PREC_BILU0, HfemError
+    kind, nc, p = case
+    kw = dict(angle=0.3, elevation=0.2) if kind == "tet" else dict(state="taylor_green", amp=1e-3)
+    prob = make_problem(kind, nc, p, seed=2, **kw)
+    ctx = HDGContext(prob)
+    try:
+        ctx.assemble_linear_system()
+        ctx.compute_ordering(1)
+        _, nlev = ctx.get_ordering()
+        ctx.compute_preconditioner(PREC_BILU0)
+        n = prob.mesh.nf * prob.nch * prob.master.npf
+        xs = [np.random.default_rng(s).standard_normal(n) for s in range(2)]
+        ctx.set_bilu0_sweep(0)
+        ref = [ctx.apply_preconditioner(x) for x in xs]
+        f0, it0, rev0 = ctx.gmres(restart=30, tol=1e-8, maxit=200)
+        ctx.set_bilu0_sweep(1)
+        for rep in range(3):
+            for x, r in zip(xs, ref):
+                assert np.array_equal(ctx.apply_preconditioner(x), r), rep
+        f1, it1, rev1 = ctx.gmres(restart=30, tol=1e-8, maxit=200)
+        say(f"  {kind}{nc} p{p}: levels {nlev}, gmres {it1} it (level sweeps {it0})")
+        assert f0 == f1 and it0 == it1 and np.array_equal(rev0, rev1)
+        ctx.set_bilu0_sweep(1, spin_limit=1)
+        with pytest.raises(HfemError, match="gave up"):
+            ctx.apply_preconditioner(xs[0])
+        ctx.set_bilu0_sweep(1)
+        assert np.array_equal(ctx.apply_preconditioner(xs[0]), ref[0])
+    finally:
+        ctx.close()
+
+
 def test_update_and_residual_match_oracle(oracle_mod, system):
     import newton_oracle as N
     from hfem_amd.lib import PREC_BILU0
