@@ -94,6 +94,12 @@ hipError_t launch_bilu0_usolve_level(int bsz, const int *rowpts, const int *coli
                                      int nlev, const float *Mg, const double *y, double *v, hipStream_t s);
 hipError_t launch_block_jacobi(int nrows, int bsz, const float *Dinv, const double *x, double *y, hipStream_t s);
 hipError_t launch_to_float(long n, const double *x, float *y, hipStream_t s);
+hipError_t launch_bilu0_factor_syncfree(int bsz, const int *rowpts, const int *colind, const int *u2o,
+                                        const int *order, int n, double *Mg, int *info, unsigned *flags,
+                                        unsigned *sync, int nwg, int spin_limit, int *err, hipStream_t s);
+hipError_t launch_bilu0_factor_syncfree(int bsz, const int *rowpts, const int *colind, const int *u2o,
+                                        const int *order, int n, float *Mg, int *info, unsigned *flags,
+                                        unsigned *sync, int nwg, int spin_limit, int *err, hipStream_t s);
 hipError_t launch_bilu0_syncfree(int bsz, const int *rowpts, const int *colind, const int *u2o, const int *lorder,
                                  const int *uorder, int n, const double *Mg, const double *x, double *t, double *y,
                                  unsigned *flags, unsigned *sync, int nwg, int spin_limit, int *err, hipStream_t s);

@@ -234,6 +234,9 @@ extern "C" int hfem_get_ordering(hfem_ctx *c, int *o2u, int *nlevels)
 
 // ---------------------------------------------------------------- preconditioner
 
+static int ensure_syncfree(hfem_ctx *c);
+static int check_syncfree(hfem_ctx *c, const char *who);
+
 extern "C" int hfem_precond_factor(hfem_ctx *c, int kind)
 {
     RC(require_system(c, "hfem_precond_factor"));
@@ -249,9 +252,10 @@ extern "C" int hfem_precond_factor(hfem_ctx *c, int kind)
         s.prec = -1;
         return 0;
     }
-    if ((size_t)(kind == 0 ? bilu0_factor_lds(bsz) : block_inverse_lds(bsz)) > (size_t)c->maxlds)
+    if ((size_t)(kind == 0 ? bilu0_factor_lds(bsz) + 16 : block_inverse_lds(bsz)) > (size_t)c->maxlds)
         return hfem_fail(-1, "hfem_precond_factor: block size %d needs more LDS than the device offers", bsz);
     const bool fp32 = s.prec_fp32 != 0;
+    s.prec = -2;   // until this factorisation has succeeded
     if (!fp32 && !s.Mg) HIPCHK(hfem_dalloc(&s.Mg, (size_t)c->nblocks * b2));
     if (fp32 && !s.Mg32) HIPCHK(hfem_dalloc(&s.Mg32, (size_t)c->nblocks * b2));
     if (kind == 2) {
@@ -264,23 +268,37 @@ extern "C" int hfem_precond_factor(hfem_ctx *c, int kind)
         }
     } else {
         if (s.order_method < 0) RC(hfem_compute_ordering(c, 1));
+        // (sync-free: one persistent launch over the rows in forward-level order;
+        // else one launch per level)
+        if (s.sweep_mode == 1) RC(ensure_syncfree(c));
+        unsigned *sync = s.sweep_mode == 1 ? s.sf_flags + 2 * (size_t)n : nullptr;
         if (!fp32) {
             HIPCHK(hipMemcpyAsync(s.Mg, c->Hg, sizeof(double) * c->nblocks * b2, hipMemcpyDeviceToDevice, c->stream));
-            for (size_t l = 0; l + 1 < s.lptr.size(); l++)
-                HIPCHK(launch_bilu0_factor_level(bsz, c->rowstart, c->cols, s.d_u2o, s.d_lrows + s.lptr[l],
-                                                 s.lptr[l + 1] - s.lptr[l], s.Mg, s.d_info, c->stream));
+            if (s.sweep_mode == 1)
+                HIPCHK(launch_bilu0_factor_syncfree(bsz, c->rowstart, c->cols, s.d_u2o, s.d_lrows, n, s.Mg, s.d_info,
+                                                    s.sf_flags, sync, s.sweep_wgs, s.spin_limit, s.sf_err, c->stream));
+            else
+                for (size_t l = 0; l + 1 < s.lptr.size(); l++)
+                    HIPCHK(launch_bilu0_factor_level(bsz, c->rowstart, c->cols, s.d_u2o, s.d_lrows + s.lptr[l],
+                                                     s.lptr[l + 1] - s.lptr[l], s.Mg, s.d_info, c->stream));
         } else {
             // single precision (precPrecision 0): Hg rounded to FP32 and factored on
             // the FP32 MFMA
             HIPCHK(launch_to_float((long)c->nblocks * b2, c->Hg, s.Mg32, c->stream));
-            for (size_t l = 0; l + 1 < s.lptr.size(); l++)
-                HIPCHK(launch_bilu0_factor_level(bsz, c->rowstart, c->cols, s.d_u2o, s.d_lrows + s.lptr[l],
-                                                 s.lptr[l + 1] - s.lptr[l], s.Mg32, s.d_info, c->stream));
+            if (s.sweep_mode == 1)
+                HIPCHK(launch_bilu0_factor_syncfree(bsz, c->rowstart, c->cols, s.d_u2o, s.d_lrows, n, s.Mg32,
+                                                    s.d_info, s.sf_flags, sync, s.sweep_wgs, s.spin_limit, s.sf_err,
+                                                    c->stream));
+            else
+                for (size_t l = 0; l + 1 < s.lptr.size(); l++)
+                    HIPCHK(launch_bilu0_factor_level(bsz, c->rowstart, c->cols, s.d_u2o, s.d_lrows + s.lptr[l],
+                                                     s.lptr[l + 1] - s.lptr[l], s.Mg32, s.d_info, c->stream));
         }
     }
     std::vector<int> info(n);
     HIPCHK(hipMemcpyAsync(info.data(), s.d_info, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    RC(check_syncfree(c, "hfem_precond_factor"));
     for (int i = 0; i < n; i++) {
         if (info[i] < 0)
             return hfem_fail(-4, "hfem_precond_factor: row %d has %d blocks, more than the factorisation takes", i,

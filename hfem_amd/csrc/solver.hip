@@ -691,6 +691,146 @@ __global__ void sweep_reset_kernel(unsigned *sync)
     }
 }
 
+// Block copy and subtraction through agent-scope atomics (coherent across the
+// XCDs), with the entry -> thread map of wg_load_block / wg_subtract_block: the
+// sync-free factorisation reads and writes the factor blocks only this way, since
+// rows are factored by workgroups on any XCD.
+template <typename T>
+__device__ __forceinline__ void wg_load_block_coh(T *dst, const T *src, long n)
+{
+    const int tid = threadIdx.x, nth = blockDim.x;
+    for (long b = tid; b < n; b += 8L * nth) {
+        T v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            v[j] = __hip_atomic_load(src + min(b + (long)j * nth, n - 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if (b + (long)j * nth < n) dst[b + (long)j * nth] = v[j];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void wg_subtract_block_coh(T *dst, const T *d, long n)
+{
+    const int tid = threadIdx.x, nth = blockDim.x;
+    for (long b = tid; b < n; b += 8L * nth) {
+        T v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            v[j] = __hip_atomic_load(dst + min(b + (long)j * nth, n - 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if (b + (long)j * nth < n)
+                __hip_atomic_store(dst + b + (long)j * nth, v[j] - d[b + (long)j * nth], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// The BILU0 factorisation without level launches: bilu0_factor_kernel's per-row
+// arithmetic (the same bits) with the rows taken in `order` (the level lists
+// concatenated) by persistent workgroups through an atomic ticket, as in
+// bilu0_sweep_syncfree_kernel.  Before reading row k's blocks (inv(U_kk) and U_kl)
+// every wave waits for k's ready flag, set once all of k's blocks are final; the
+// blocks go through agent-scope atomics (wg_load_block_coh / wg_subtract_block_coh)
+// and the flag is stored after every wave's stores have completed.  A row too long
+// for the factorisation is reported through info and still published.
+template <typename T>
+__global__ void __launch_bounds__(SV_THREADS)
+bilu0_factor_syncfree_kernel(int bsz, const int *__restrict__ rowpts, const int *__restrict__ colind,
+                             const int *__restrict__ u2o, const int *__restrict__ order, int n, T *Mg,
+                             int *__restrict__ info, unsigned *flags, const unsigned *epoch_p, unsigned *counter,
+                             int spin_limit, int *err)
+{
+    extern __shared__ double lds_raw[];
+    T *lds = (T *)lds_raw;
+    const int tid = threadIdx.x, nth = blockDim.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const long b2 = (long)bsz * bsz;
+    T *X = lds, *B = lds + b2, *L = lds + 2 * b2;
+    LdsInv<T> w = lds_inv_area(lds + 3 * b2, bsz);
+    int *rcol = w.piv + bsz + 2, *rord = rcol + FACTOR_MAXROW, *lst = rord + FACTOR_MAXROW,
+        *ncol = lst + FACTOR_MAXROW + 1, *sticket = ncol + FACTOR_MAXROW;
+    const unsigned epoch = __hip_atomic_load(epoch_p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // (loop control wave-uniform throughout: see bilu0_sweep_syncfree_kernel)
+    while (true) {
+        if (wv == 0) *sticket = __builtin_amdgcn_readfirstlane(atomicAdd(counter, lane == 0 ? 1u : 0u));
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(*sticket);
+        if (t >= n) break;
+        const int ri = order[t], oi = u2o[ri], d0 = rowpts[ri], len = rowpts[ri + 1] - d0;
+        if (len <= FACTOR_MAXROW) {
+            if (tid < len) {
+                const int c = colind[d0 + tid];
+                rcol[tid] = c;
+                rord[tid] = u2o[c];
+            }
+            __syncthreads();
+            if (tid == 0) {
+                int cnt = 0;
+                for (int q = 1; q < len; q++) {
+                    if (rord[q] >= oi) continue;
+                    int m = cnt++;
+                    while (m > 0 && rord[lst[1 + m - 1]] > rord[q]) {
+                        lst[1 + m] = lst[1 + m - 1];
+                        m--;
+                    }
+                    lst[1 + m] = q;
+                }
+                lst[0] = cnt;
+            }
+            __syncthreads();
+            const int cnt = __builtin_amdgcn_readfirstlane(lst[0]);
+            for (int qq = 0; qq < cnt; qq++) {
+                const int q = __builtin_amdgcn_readfirstlane(lst[1 + qq]);
+                const int ik = d0 + q, rk = __builtin_amdgcn_readfirstlane(rcol[q]);
+                const int ork = __builtin_amdgcn_readfirstlane(rord[q]);
+                const int k0 = rowpts[rk], klen = rowpts[rk + 1] - k0;
+                if (tid < klen && tid < FACTOR_MAXROW) ncol[tid] = colind[k0 + tid];
+                wait_ready(flags + rk, epoch, spin_limit, err);
+                wg_load_block_coh(X, Mg + (long)ik * b2, b2);
+                wg_load_block_coh(B, Mg + (long)k0 * b2, b2);     // inv(U_kk)
+                __syncthreads();
+                T *Mik = Mg + (long)ik * b2;
+                lds_block_product(X, B, bsz, [&](int i, int j, T c) {
+                    L[i + bsz * j] = c;
+                    __hip_atomic_store(Mik + i + bsz * j, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                });
+                __syncthreads();
+                for (int l = 0; l < len; l++) {
+                    if (l != 0 && __builtin_amdgcn_readfirstlane(rord[l]) <= ork) continue;
+                    const int rl = __builtin_amdgcn_readfirstlane(rcol[l]);
+                    int km = -1;
+                    for (int m = 1; m < klen && m < FACTOR_MAXROW; m++)
+                        if (__builtin_amdgcn_readfirstlane(ncol[m]) == rl) {
+                            km = m;
+                            break;
+                        }
+                    if (km < 0) continue;
+                    wg_load_block_coh(B, Mg + (long)(k0 + km) * b2, b2);
+                    __syncthreads();
+                    lds_block_product(L, B, bsz, [&](int i, int j, T c) { X[i + bsz * j] = c; });
+                    __syncthreads();
+                    wg_subtract_block_coh(Mg + (long)(d0 + l) * b2, X, b2);
+                    __syncthreads();
+                }
+                __syncthreads();
+            }
+            wg_load_block_coh(X, Mg + (long)d0 * b2, b2);
+            __syncthreads();
+            const int r = lds_inverse(X, bsz, w.piv, w.rowk, w.colk, w.red);
+            for (long i = tid; i < b2; i += nth)
+                __hip_atomic_store(Mg + (long)d0 * b2 + i, X[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (tid == 0) info[ri] = r;
+        } else {
+            if (tid == 0) info[ri] = -len;
+        }
+        // publish row ri once every wave's stores of it have completed
+        __builtin_amdgcn_s_waitcnt(0x0F70);   // s_waitcnt vmcnt(0)
+        __syncthreads();
+        if (wv == 0) __hip_atomic_store(flags + ri, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // Block-Jacobi application y_i = inv(H_ii) x_i.  One wave per row.
 template <typename TM>
 __global__ void __launch_bounds__(SV_THREADS)
@@ -871,6 +1011,34 @@ static hipError_t factor_level(int bsz, const int *rowpts, const int *colind, co
     hipLaunchKernelGGL(bilu0_factor_kernel<T>, dim3(std::min(nlev, 8192)), dim3(SV_THREADS), lds, s, bsz, rowpts,
                        colind, u2o, rows, nlev, Mg, info);
     return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t factor_syncfree(int bsz, const int *rowpts, const int *colind, const int *u2o, const int *order, int n,
+                                  T *Mg, int *info, unsigned *flags, unsigned *sync, int nwg, int spin_limit, int *err,
+                                  hipStream_t s)
+{
+    if (n <= 0 || nwg <= 0) return hipErrorInvalidValue;
+    const size_t lds = bilu0_factor_lds(bsz) + 16;   // and the ticket
+    hipError_t e = set_lds((const void *)bilu0_factor_syncfree_kernel<T>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(sweep_reset_kernel, dim3(1), dim3(64), 0, s, sync);
+    hipLaunchKernelGGL(bilu0_factor_syncfree_kernel<T>, dim3(std::min(n, nwg)), dim3(SV_THREADS), lds, s, bsz, rowpts,
+                       colind, u2o, order, n, Mg, info, flags, sync, sync + 1, spin_limit, err);
+    return hipGetLastError();
+}
+
+hipError_t launch_bilu0_factor_syncfree(int bsz, const int *rowpts, const int *colind, const int *u2o,
+                                        const int *order, int n, double *Mg, int *info, unsigned *flags,
+                                        unsigned *sync, int nwg, int spin_limit, int *err, hipStream_t s)
+{
+    return factor_syncfree(bsz, rowpts, colind, u2o, order, n, Mg, info, flags, sync, nwg, spin_limit, err, s);
+}
+hipError_t launch_bilu0_factor_syncfree(int bsz, const int *rowpts, const int *colind, const int *u2o,
+                                        const int *order, int n, float *Mg, int *info, unsigned *flags,
+                                        unsigned *sync, int nwg, int spin_limit, int *err, hipStream_t s)
+{
+    return factor_syncfree(bsz, rowpts, colind, u2o, order, n, Mg, info, flags, sync, nwg, spin_limit, err, s);
 }
 
 hipError_t launch_bilu0_factor_level(int bsz, const int *rowpts, const int *colind, const int *u2o, const int *rows,

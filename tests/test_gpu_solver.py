@@ -208,23 +208,24 @@ def test_restarted_gmres_stall_belongs_to_the_system(oracle_mod, oblique):
 
 @pytest.mark.parametrize("prec_precision", [1, 0], ids=["fp64_factors", "fp32_factors"])
 def test_syncfree_sweeps_match_level_sweeps(oracle_mod, system, prec_precision):
-    """The BILU0 application with one persistent launch per sweep (rows taken in
-    level order, per-row ready flags, bounded waits) returns the level-by-level
-    result bit for bit, over repeated applications (the flags carry each
-    application's epoch) and inside a replayed graph; GMRES gives the same history."""
+    """The BILU0 factorisation and application with one persistent launch each
+    (rows taken in level order, per-row ready flags, bounded waits) return the
+    level-by-level results bit for bit, over repeated applications (the flags carry
+    each launch's epoch) and inside a replayed graph; GMRES gives the same history."""
     from hfem_amd.lib import PREC_BILU0
     O = oracle_mod
     prob, ctx, M, Rg = system
     o2u, u2o = O.mdf_order(M)
     ctx.set_ordering(o2u)
-    ctx.compute_preconditioner(PREC_BILU0, prec_precision=prec_precision)
     rng = np.random.default_rng(3)
     xs = [rng.standard_normal(Rg.size) for _ in range(3)]
     ctx.set_bilu0_sweep(0)
+    ctx.compute_preconditioner(PREC_BILU0, prec_precision=prec_precision)
     ref = [ctx.apply_preconditioner(x) for x in xs]
     flag0, it0, rev0 = ctx.gmres(restart=10, tol=1e-10, maxit=500)
     try:
         ctx.set_bilu0_sweep(1)
+        ctx.compute_preconditioner(PREC_BILU0, prec_precision=prec_precision)
         for use_graph in (False, True):
             ctx.set_apply_graph(use_graph)
             for rep in range(2):
@@ -241,11 +242,12 @@ def test_syncfree_sweeps_match_level_sweeps(oracle_mod, system, prec_precision):
 
 @pytest.mark.parametrize("case", [("tet", 4, 3), ("hex", 3, 2)], ids=lambda c: f"{c[0]}{c[1]}p{c[2]}")
 def test_syncfree_sweeps_many_levels(case):
-    """Sync-free against level sweeps where the level chain is long (384 p=3 tets,
-    block 50; periodic p=2 hexes, block 45): bit-identical applications and GMRES
-    histories.  Then the give-up path, forced from the host with a one-poll wait
-    limit: the kernel still ends, the application reports the failure, and the
-    next one with the default limit is exact again."""
+    """Sync-free against level-launched factorisation and sweeps where the level
+    chain is long (384 p=3 tets, block 50; periodic p=2 hexes, block 45):
+    bit-identical factors, applications and GMRES histories.  Then the give-up
+    path, forced from the host with a one-poll wait limit, in the application and
+    in the factorisation: the kernels still end, the call reports the failure, and
+    the next one with the default limit is exact again."""
     from hfem_amd.lib import HDGContext, PREC_BILU0, HfemError
     kind, nc, p = case
     kw = dict(angle=0.3, elevation=0.2) if kind == "tet" else dict(state="taylor_green", amp=1e-3)
@@ -255,13 +257,16 @@ def test_syncfree_sweeps_many_levels(case):
         ctx.assemble_linear_system()
         ctx.compute_ordering(1)
         _, nlev = ctx.get_ordering()
-        ctx.compute_preconditioner(PREC_BILU0)
         n = prob.mesh.nf * prob.nch * prob.master.npf
         xs = [np.random.default_rng(s).standard_normal(n) for s in range(2)]
         ctx.set_bilu0_sweep(0)
+        ctx.compute_preconditioner(PREC_BILU0)
+        t_level = ctx.solver_stats()["factor_ms"]
         ref = [ctx.apply_preconditioner(x) for x in xs]
         f0, it0, rev0 = ctx.gmres(restart=30, tol=1e-8, maxit=200)
         ctx.set_bilu0_sweep(1)
+        ctx.compute_preconditioner(PREC_BILU0)
+        say(f"  factorisation: level launches {t_level:.1f} ms, sync-free {ctx.solver_stats()['factor_ms']:.1f} ms")
         for rep in range(3):
             for x, r in zip(xs, ref):
                 assert np.array_equal(ctx.apply_preconditioner(x), r), rep
@@ -271,7 +276,10 @@ def test_syncfree_sweeps_many_levels(case):
         ctx.set_bilu0_sweep(1, spin_limit=1)
         with pytest.raises(HfemError, match="gave up"):
             ctx.apply_preconditioner(xs[0])
+        with pytest.raises(HfemError, match="gave up"):
+            ctx.compute_preconditioner(PREC_BILU0)
         ctx.set_bilu0_sweep(1)
+        ctx.compute_preconditioner(PREC_BILU0)
         assert np.array_equal(ctx.apply_preconditioner(xs[0]), ref[0])
     finally:
         ctx.close()
