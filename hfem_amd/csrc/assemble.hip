@@ -79,9 +79,13 @@ __host__ __device__ inline AsmLayout asm_layout(const Dims &d, int spill)
     L.jacf = o; o += np;
     L.nl = o; o += np * nd;
     L.M = o; o += npv * npv;
-    L.Qm = big((ngv + np) * npv);
-    L.C = big(nd * npv * npv);
-    L.E = big(nd * npv * ndf);
+    L.Qm = big((ngv + np) * (npv | 1));
+    // Q rows and C_x, E_x columns at an odd stride (npv | 1): MFMA operand reads
+    // walk them with the lane, and an even stride of doubles puts lanes 8 apart on
+    // the same LDS bank
+    const int cs = npv | 1;
+    L.C = big(nd * cs * npv);
+    L.E = big(nd * cs * ndf);
     L.udgg = o; o += ngv * nc;
     // fhu, fhuh of one row block (P8); the Ru/Rh tables (P9)
     const int fjn = np * (nc + nch), tn = ngv * ncu * nd1 + 2 * np * nch;
@@ -212,6 +216,7 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
     // 16-wide MFMA tiles: TD over the npv columns of D and K, TF over the ndf columns
     // of F and H, KT over the npv nodes contracted in the W products of P8
     constexpr int TD = (NPV + 15) / 16, TF = (NDF + 15) / 16, NT = TD + TF, KT = (NPV + 15) / 16;
+    constexpr int CS = NPV | 1;           // row stride of Q, column stride of C_x and E_x (asm_layout)
     const int npv = NPV, ngv = d.ngv, npf = NPF, ngf = d.ngf, nfe = NFE, ndf = NDF;
     const int ncu = NCU, nch = NCU, nc = NC, na = d.na, nb = d.nb, nd1 = ND1, np = d.np;
 
@@ -347,7 +352,7 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                             const int pp = g + ngf * a;
                             s += (SFG(k, g, 0) * SFT(g, jf, 0)) * nl[pp + np * x] * jacf[pp];
                         }
-                    E[r] = s;
+                    E[x * CS * ndf + l * CS + nn] = s;
                 }
             }
         }
@@ -366,7 +371,7 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                 const int x = r / (npv * npv), rr = r % (npv * npv), jj = rr % npv, ii = rr / npv;
                 double s = 0;
                 for (int g = 0; g < ngv; g++) s += G[jj + npv * (g + ngv * x)] * SVT(g, ii, 0);
-                C[r] = s;
+                C[x * CS * npv + ii * CS + jj] = s;
             }
         }
         __syncthreads();
@@ -381,7 +386,7 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                 const int pp = r - ngv, g = pp % ngf, a = pp / ngf;
                 for (int q = 0; q < npf; q++) s += SFT(g, q, 0) * M[PERM(q, a) + npv * k];
             }
-            Qm[k + npv * r] = s;
+            Qm[k + CS * r] = s;
         }
         __syncthreads();
 
@@ -435,20 +440,26 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
             // others contract row block j; without it all waves stage between blocks.
             const bool dbuf = LY.dbuf;
             const int ntw = dbuf ? nwave - 1 : nwave;
-            double *uXb[2] = {fuX, lds + LY.fuX2}, *hjb[2] = {fj, lds + LY.fj2};
-            stage(std::integral_constant<int, 8>(), 0, tid, nth, uXb[0], hjb[0]);
+            // the two staging buffers, formed from the LDS base where they are both in
+            // LDS (picked from an array of pointers they lose their address space, and
+            // every read in the k-loops becomes a flat load followed by a full wait)
+            auto uXb = [&](int b) -> double * {
+                return SPILL ? (b ? lds + LY.fuX2 : fuX) : lds + (b ? LY.fuX2 : LY.svt + npv * ngv);
+            };
+            auto hjb = [&](int b) -> double * { return lds + (b ? LY.fj2 : LY.fj); };
+            stage(std::integral_constant<int, 8>(), 0, tid, nth, uXb(0), hjb(0));
             for (int j = 0; j < ncu; j++) {
                 __syncthreads();
                 const int cb = dbuf ? (j & 1) : 0;
                 if (!dbuf && j > 0) {
-                    stage(std::integral_constant<int, 8>(), j, tid, nth, uXb[0], hjb[0]);
+                    stage(std::integral_constant<int, 8>(), j, tid, nth, uXb(0), hjb(0));
                     __syncthreads();
                 }
                 if (dbuf && wave == nwave - 1) {
-                    if (j + 1 < ncu) stage(std::integral_constant<int, 24>(), j + 1, lane, 64, uXb[cb ^ 1], hjb[cb ^ 1]);
+                    if (j + 1 < ncu) stage(std::integral_constant<int, 24>(), j + 1, lane, 64, uXb(cb ^ 1), hjb(cb ^ 1));
                     continue;
                 }
-                const double *uX = uXb[cb], *hj = hjb[cb], *hhj = hj + np * NC;
+                const double *uX = uXb(cb), *hj = hjb(cb), *hhj = hj + np * NC;
                 for (int rt = wave; rt < nrt; rt += ntw) {
                     // this lane's A row (kc, i) and its accumulator rows
                     const int ra = rt * 16 + lr;
@@ -473,33 +484,41 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                     for (int x = 0; x < ND; x++)
 #pragma unroll
                         for (int kt = 0; kt < KT; kt++) w[x][kt] = (f64x4){0.0, 0.0, 0.0, 0.0};
-                    // volume points
+                    // volume points.  Every operand load of a k-step is issued before any
+                    // of them is used (the sched_barrier): left to itself, the scheduler
+                    // interleaves them with the arithmetic, one LDS round trip at a time.
                     for (int g0 = 0; g0 < ngv; g0 += 4) {
                         const int g = min(g0 + lk, ngv - 1);
                         const bool vg = g0 + lk < ngv;
-                        double s[ND1];
+                        double s[ND1], u0[ND], u1[ND][ND], b[TD], q[KT];
 #pragma unroll
                         for (int r = 0; r < ND1; r++) s[r] = SVG(ia, g, r);
-                        double a0 = tdj * jac[g] * s[0];
+                        const double jg = jac[g];
 #pragma unroll
-                        for (int r = 0; r < ND; r++) a0 -= uX[g + ngv * (r + ND * kca)] * s[r + 1];
+                        for (int r = 0; r < ND; r++) u0[r] = uX[g + ngv * (r + ND * kca)];
+#pragma unroll
+                        for (int x = 0; x < ND; x++)
+#pragma unroll
+                            for (int r = 0; r < ND; r++) u1[x][r] = uX[g + ngv * (r + ND * ((x + 1) * ncu + kca))];
+#pragma unroll
+                        for (int t = 0; t < TD; t++) b[t] = SVT(g, ccl[t], 0);
+#pragma unroll
+                        for (int kt = 0; kt < KT; kt++) q[kt] = Qm[kq[kt] + CS * g];
+                        __builtin_amdgcn_sched_barrier(0);
+                        double a0 = tdj * jg * s[0];
+#pragma unroll
+                        for (int r = 0; r < ND; r++) a0 -= u0[r] * s[r + 1];
                         a0 = vg ? a0 : 0.0;
                         double ax[ND];
 #pragma unroll
                         for (int x = 0; x < ND; x++) {
                             double v = 0.0;
 #pragma unroll
-                            for (int r = 0; r < ND; r++) v += uX[g + ngv * (r + ND * ((x + 1) * ncu + kca))] * s[r + 1];
+                            for (int r = 0; r < ND; r++) v += u1[x][r] * s[r + 1];
                             ax[x] = vg ? -v : 0.0;
                         }
-                        double b[TD], q[KT];
 #pragma unroll
-                        for (int t = 0; t < TD; t++) b[t] = SVT(g, ccl[t], 0);
-#pragma unroll
-                        for (int kt = 0; kt < KT; kt++) {
-                            const double qv = Qm[kq[kt] + npv * g];
-                            q[kt] = kqv[kt] ? qv : 0.0;
-                        }
+                        for (int kt = 0; kt < KT; kt++) q[kt] = kqv[kt] ? q[kt] : 0.0;
 #pragma unroll
                         for (int t = 0; t < TD; t++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b[t], acc[t], 0, 0, 0);
 #pragma unroll
@@ -508,39 +527,58 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                             for (int kt = 0; kt < KT; kt++)
                                 w[x][kt] = __builtin_amdgcn_mfma_f64_16x16x4f64(q[kt], ax[x], w[x][kt], 0, 0, 0);
                     }
-                    // face points
+                    // face points: the face-node positions of this lane's row and D
+                    // columns on each face first (no dependent index loads in the loop),
+                    // then the operand loads of a k-step issued together, as above
+                    int kia[NFE], kcc[TD][NFE];
+#pragma unroll
+                    for (int a = 0; a < NFE; a++) {
+                        kia[a] = INVP(ia, a);
+#pragma unroll
+                        for (int t = 0; t < TD; t++) kcc[t][a] = INVP(ccl[t], a);
+                    }
                     for (int p0 = 0; p0 < np; p0 += 4) {
                         const int pp = min(p0 + lk, np - 1);
                         const int a = pp / ngf, gf = pp % ngf;
-                        const int ki = INVP(ia, a);
-                        const double sv = SFG(max(ki, 0), gf, 0) * jacf[pp];
-                        const double af = (p0 + lk < np && ki >= 0) ? sv : 0.0;
-                        double bu[NT], q[KT];
+                        int ki = kia[0], kc[TD];
 #pragma unroll
-                        for (int t = 0; t < TD; t++) {
-                            const int k = INVP(ccl[t], a);
-                            const double v = SFT(gf, max(k, 0), 0);
-                            bu[t] = k >= 0 ? v : 0.0;
-                        }
+                        for (int t = 0; t < TD; t++) kc[t] = kcc[t][0];
 #pragma unroll
-                        for (int t = 0; t < TF; t++) {
-                            const double v = SFT(gf, lcl[t] % npf, 0);
-                            bu[TD + t] = lcl[t] / npf == a ? v : 0.0;
-                        }
+                        for (int a2 = 1; a2 < NFE; a2++)
+                            if (a == a2) {
+                                ki = kia[a2];
 #pragma unroll
-                        for (int kt = 0; kt < KT; kt++) {
-                            const double qv = Qm[kq[kt] + npv * (ngv + pp)];
-                            q[kt] = kqv[kt] ? qv : 0.0;
-                        }
-                        const double ad = af * hj[pp + np * kca];
-                        const double ah = af * hhj[pp + np * kca];
+                                for (int t = 0; t < TD; t++) kc[t] = kcc[t][a2];
+                            }
+                        double bu[NT], q[KT], hv[ND + 2];
+                        const double sg = SFG(max(ki, 0), gf, 0), jf = jacf[pp];
+#pragma unroll
+                        for (int t = 0; t < TD; t++) bu[t] = SFT(gf, max(kc[t], 0), 0);
+#pragma unroll
+                        for (int t = 0; t < TF; t++) bu[TD + t] = SFT(gf, lcl[t] % npf, 0);
+#pragma unroll
+                        for (int kt = 0; kt < KT; kt++) q[kt] = Qm[kq[kt] + CS * (ngv + pp)];
+                        hv[0] = hj[pp + np * kca];
+                        hv[1] = hhj[pp + np * kca];
+#pragma unroll
+                        for (int x = 0; x < ND; x++) hv[2 + x] = hj[pp + np * ((x + 1) * ncu + kca)];
+                        __builtin_amdgcn_sched_barrier(0);
+                        const double af = (p0 + lk < np && ki >= 0) ? sg * jf : 0.0;
+#pragma unroll
+                        for (int t = 0; t < TD; t++) bu[t] = kc[t] >= 0 ? bu[t] : 0.0;
+#pragma unroll
+                        for (int t = 0; t < TF; t++) bu[TD + t] = lcl[t] / npf == a ? bu[TD + t] : 0.0;
+#pragma unroll
+                        for (int kt = 0; kt < KT; kt++) q[kt] = kqv[kt] ? q[kt] : 0.0;
+                        const double ad = af * hv[0];
+                        const double ah = af * hv[1];
 #pragma unroll
                         for (int t = 0; t < TD; t++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, bu[t], acc[t], 0, 0, 0);
 #pragma unroll
                         for (int t = TD; t < NT; t++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ah, bu[t], acc[t], 0, 0, 0);
 #pragma unroll
                         for (int x = 0; x < ND; x++) {
-                            const double aq = af * hj[pp + np * ((x + 1) * ncu + kca)];
+                            const double aq = af * hv[2 + x];
 #pragma unroll
                             for (int kt = 0; kt < KT; kt++)
                                 w[x][kt] = __builtin_amdgcn_mfma_f64_16x16x4f64(q[kt], aq, w[x][kt], 0, 0, 0);
@@ -558,9 +596,9 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                                 const int k = min(kt * 16 + 4 * v + lk, npv - 1);
                                 double b[NT];
 #pragma unroll
-                                for (int t = 0; t < TD; t++) b[t] = C[x * npv * npv + ccl[t] * npv + k];
+                                for (int t = 0; t < TD; t++) b[t] = C[x * CS * npv + ccl[t] * CS + k];
 #pragma unroll
-                                for (int t = 0; t < TF; t++) b[TD + t] = -E[x * npv * ndf + lcl[t] * npv + k];
+                                for (int t = 0; t < TF; t++) b[TD + t] = -E[x * CS * ndf + lcl[t] * CS + k];
 #pragma unroll
                                 for (int t = 0; t < NT; t++)
                                     acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(w[x][kt][v], b[t], acc[t], 0, 0, 0);
@@ -701,12 +739,12 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                     const int pp = min(rg * 16 + lr, ngf - 1) + ngf * a;
                     const bool isD = t < TD;
                     const int col = isD ? min(t * 16 + lr, npv - 1) : min((t - TD) * 16 + lr, ndf - 1);
-                    const double *bsrc = isD ? C + x * npv * npv + col * npv : E + x * npv * ndf + col * npv;
+                    const double *bsrc = isD ? C + x * CS * npv + col * CS : E + x * CS * ndf + col * CS;
                     f64x4 acc = (f64x4){0.0, 0.0, 0.0, 0.0};
 #pragma unroll
                     for (int k0 = 0; k0 < NPV; k0 += 4) {
                         const int k = min(k0 + lk, npv - 1);
-                        const double qv = Qm[k + npv * (ngv + pp)];
+                        const double qv = Qm[k + CS * (ngv + pp)];
                         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(k0 + lk < npv ? qv : 0.0, bsrc[k], acc, 0, 0, 0);
                     }
 #pragma unroll
