@@ -243,7 +243,7 @@ def cpu_baseline(prob, seconds=10.0, threads=None):
     main/digasoser.cpp:46) and all the host threads this GPU's share allows (at
     most 16), each thread with its own reference structs.  The all-threads row is
     the reported baseline; both rows are returned."""
-    os.environ.setdefault("MKL_NUM_THREADS", "1")
+    os.environ["MKL_NUM_THREADS"] = "1"          # (the harness also sets it per thread)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     model, ncpu = cpu_model()

@@ -173,6 +173,10 @@ extern "C" int ref_getq(const int* dims, const double* shapvt, const double* sha
     return 0;
 }
 
+// MKL's per-thread thread count (mkl_rt): the timed loops keep MKL sequential
+// inside each of their threads whatever MKL_NUM_THREADS the environment sets.
+extern "C" int MKL_Set_Num_Threads_Local(int);   // (mkl.h maps mkl_set_num_threads_local to it)
+
 // Timed element loop on nthreads host threads (the all-cores CPU baseline): each
 // thread has its own structs, as the reference's per-thread elems[] / temps[]
 // intend (main/digasoser.cpp:51-58), and takes every nthreads-th element of
@@ -196,6 +200,7 @@ extern "C" double ref_time_mt(const int* dims, const double* shapvt, const doubl
 #pragma omp parallel num_threads(nthreads)
     {
         const int th = omp_get_thread_num();
+        MKL_Set_Num_Threads_Local(1);
         Setup* s = S[th];
         double elemtimes[32] = {0}, schurtimes[8] = {0};
         long long flops = 0;
@@ -225,6 +230,7 @@ extern "C" double ref_time(const int* dims, const double* shapvt, const double* 
     double elemtimes[32] = {0}, schurtimes[8] = {0};
     long long flops = 0;
     int nb = dims[4] * dims[6] * dims[1];
+    const int prev = MKL_Set_Num_Threads_Local(1);
     struct timespec t0, t1;
     clock_gettime(CLOCK_MONOTONIC, &t0);
     for (int t = 0; t < n; t++) {
@@ -234,6 +240,7 @@ extern "C" double ref_time(const int* dims, const double* shapvt, const double* 
         schurElementMatrixVector(S->elem, S->mesh, S->app, S->temp, impl, ie, schurtimes, &flops);
     }
     clock_gettime(CLOCK_MONOTONIC, &t1);
+    MKL_Set_Num_Threads_Local(prev);
     delete S;
     return (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
 }
