@@ -130,6 +130,7 @@ dinv_mfma_kernel(int na, int e0, int n, const double *__restrict__ D, TO *__rest
     typedef typename G::v4 v4;
     typedef typename G::key_t key_t;
     constexpr int NN = 16 * NT, NTILE = NT * NT, TPW = (NTILE + W - 1) / W, RPL = (NN + 63) / 64;
+    constexpr int PMW = (4 * TPW + 31) / 32;   // words of the per-lane pivot-row mask
     constexpr int PS = 17;                // panel row stride: odd, so a column walk is bank-conflict free
     __shared__ T pan[NN * PS];            // panel / staging of 16 columns, row-major: pan[i*PS + k]
     __shared__ T Z[NN * PS];              // pivot rows of the panel, Z[k*NN + c]; output staging
@@ -260,18 +261,29 @@ dinv_mfma_kernel(int na, int e0, int n, const double *__restrict__ D, TO *__rest
                 if (lane == 0 && sing && !sing_s) sing_s = sing;
             }
             __syncthreads();
-            // 3. the old pivot rows of the other columns
+            // 3. the old pivot rows of the other columns.  A tile's four pivot
+            //    numbers are read together (one LDS round trip per tile, not per
+            //    row), and which of the lane's rows are pivot rows is kept as a
+            //    bitmask (bit 4s + v) for step 4
             wv = __builtin_amdgcn_readfirstlane(wave);
             asm volatile("" : "+s"(wv));
+            unsigned pm[PMW];
+#pragma unroll
+            for (int i = 0; i < PMW; i++) pm[i] = 0u;
 #pragma unroll
             for (int s = 0; s < TPW; s++) {
                 const int T_ = s * W + wv, r = T_ / NT, q = T_ % NT;
-                if (T_ < NTILE && q != p)
+                if (T_ < NTILE && q != p) {
+                    int k[4];
 #pragma unroll
-                    for (int v = 0; v < 4; v++) {
-                        const int k = rowk[r * 16 + G::row(lk, v)];
-                        if (k >= 0) Z[k * NN + q * 16 + lr] = a[s][v];
-                    }
+                    for (int v = 0; v < 4; v++) k[v] = rowk[r * 16 + G::row(lk, v)];
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        if (k[v] >= 0) {
+                            Z[k[v] * NN + q * 16 + lr] = a[s][v];
+                            pm[(4 * s + v) / 32] |= 1u << ((4 * s + v) % 32);
+                        }
+                }
             }
             __syncthreads();
             // 4. the rank-16 update (the panel columns take Gp)
@@ -287,7 +299,7 @@ dinv_mfma_kernel(int na, int e0, int n, const double *__restrict__ D, TO *__rest
                 } else {
 #pragma unroll
                     for (int v = 0; v < 4; v++)
-                        if (rowk[r * 16 + G::row(lk, v)] >= 0) a[s][v] = (T)0;
+                        if ((pm[(4 * s + v) / 32] >> ((4 * s + v) % 32)) & 1u) a[s][v] = (T)0;
 #pragma unroll
                     for (int ks = 0; ks < 4; ks++)
                         a[s] = G::mfma(pan[(r * 16 + lr) * PS + ks * 4 + lk], Z[(ks * 4 + lk) * NN + q * 16 + lr], a[s]);
@@ -296,8 +308,13 @@ dinv_mfma_kernel(int na, int e0, int n, const double *__restrict__ D, TO *__rest
             __syncthreads();
         }
         // inv(D)(a, pof[c]) = G(pof[a], c), two 16-column tiles of G at a time
-        // (staged in pan and Z)
+        // (staged in pan and Z).  A wave writes whole columns, lanes along the rows
+        // a (coalesced stores); the staging rows pof[a] of a lane's rows are read
+        // once per element
         TO *Oe = Dinv + e * (long)na * na;
+        int pa[RPL];
+#pragma unroll
+        for (int h = 0; h < RPL; h++) pa[h] = lane + 64 * h < na ? pof[lane + 64 * h] : 0;
         for (int q0 = 0; q0 < NT; q0 += 2) {
             int wv = __builtin_amdgcn_readfirstlane(wave);
             asm volatile("" : "+s"(wv));
@@ -312,10 +329,12 @@ dinv_mfma_kernel(int na, int e0, int n, const double *__restrict__ D, TO *__rest
             }
             __syncthreads();
             const int ncols = min(32, na - q0 * 16);
-            for (int idx = tid; idx < ncols * na; idx += 64 * W) {
-                const int cc = idx / na, ar = idx % na, c = q0 * 16 + cc;
+            for (int cc = wave; cc < ncols; cc += W) {
+                const int col = pof[q0 * 16 + cc];
                 const T *st = cc < 16 ? pan : Z;
-                Oe[ar + (long)na * pof[c]] = (TO)st[pof[ar] * PS + (cc & 15)];
+#pragma unroll
+                for (int h = 0; h < RPL; h++)
+                    if (lane + 64 * h < na) Oe[lane + 64 * h + (long)na * col] = (TO)st[pa[h] * PS + (cc & 15)];
             }
             __syncthreads();
         }
