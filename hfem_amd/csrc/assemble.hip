@@ -334,6 +334,7 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                     // M(jj, ii) at M[jj + npv*ii]
                     const int r = i - nG, jj = r % npv, ii = r / npv;
                     double s = 0;
+#pragma unroll 4
                     for (int g = 0; g < ngv; g++) s += (SVG(jj, g, 0) * SVT(g, ii, 0)) * jac[g];
                     M[r] = ph.fc_q * s;
                 } else if (i < nG + nM + nU) {
@@ -348,6 +349,7 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                     const int a = l / npf, jf = l % npf, k = INVP(nn, a);
                     double s = 0;
                     if (k >= 0)
+#pragma unroll 4
                         for (int g = 0; g < ngf; g++) {
                             const int pp = g + ngf * a;
                             s += (SFG(k, g, 0) * SFT(g, jf, 0)) * nl[pp + np * x] * jacf[pp];
@@ -370,6 +372,7 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
             for (int r = tid - 64; r < ND * npv * npv; r += nth - 64) {
                 const int x = r / (npv * npv), rr = r % (npv * npv), jj = rr % npv, ii = rr / npv;
                 double s = 0;
+#pragma unroll 4
                 for (int g = 0; g < ngv; g++) s += G[jj + npv * (g + ngv * x)] * SVT(g, ii, 0);
                 C[x * CS * npv + ii * CS + jj] = s;
             }
@@ -665,12 +668,14 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                 if (col < na) {
                     const int row = col, jj = row / npv, ii = row % npv;
                     double s = 0.0;
+#pragma unroll 4
                     for (int g = 0; g < ngv; g++)
 #pragma unroll
                         for (int r = 0; r < ND1; r++) s += SVG(ii, g, r) * T[g + ngv * (jj + ncu * r)];
                     for (int a = 0; a < nfe; a++) {
                         const int kk = INVP(ii, a);
                         if (kk < 0) continue;
+#pragma unroll 4
                         for (int g = 0; g < ngf; g++) s -= SFG(kk, g, 0) * Fi[g + ngf * a + np * jj];
                     }
                     Ru[e * na + row] = s;
@@ -679,6 +684,7 @@ assemble_kernel(Dims d, Master m, Phys ph, const double *__restrict__ dgnodes, c
                 } else {
                     const int row = col - na, jj = row % nch, tr = row / nch, a = tr / npf, kk = tr % npf;
                     double s = 0.0;
+#pragma unroll 4
                     for (int g = 0; g < ngf; g++) s += SFG(kk, g, 0) * Fo[g + ngf * a + np * jj];
                     Rh[e * nb + row] = -s;
                     Hx[e * (nH + nb) + nH + row] = -s;
