@@ -294,6 +294,8 @@ def main():
     ap.add_argument("--ncell", type=int, default=14)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
+    ap.add_argument("--mixed-tol", type=float, default=1e-10,
+                    help="refinement tolerance of the mixed path (the metric's bound)")
     ap.add_argument("--no-solver", action="store_true", help="skip the trace-system solve")
     ap.add_argument("--mode", choices=["bench", "copy", "newton", "dirk", "ragged"], default="bench",
                     help="copy: a plain 2 GiB device copy for the HBM counter calibration; newton: one Newton "
@@ -338,17 +340,21 @@ def main():
     res = {}
     precs = {"fp64": FP64, "mixed": MIXED}
     todo = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    # the mixed path refines to the metric's own bound (the library's default is 1e-11)
+    ctx.set_refinement(args.mixed_tol, 4)
     for name in todo:
         wall, a_ms, c_ms = run_steps(ctx, ne, precs[name], args.steps, args.warmup)
         res[name] = dict(wall=wall, asm_ms=a_ms, cond_ms=c_ms, eps=ne * args.steps / wall)
         if name == "mixed":
             sw, left, fb = ctx.condense_report()
-            res[name]["refinement"] = {"sweeps": sw, "unconverged_elements": left, "fp64_fallback": bool(fb)}
+            res[name]["refinement"] = {"tol": args.mixed_tol, "sweeps": sw, "unconverged_elements": left,
+                                       "fp64_fallback": bool(fb)}
         print(f"[bench] {name}: {res[name]['eps']:.1f} elements/s  assemble {a_ms:.3f} ms  condense {c_ms:.3f} ms",
               file=sys.stderr, flush=True)
 
-    # accuracy of the mixed path against the FP64 path on a sample of elements
-    sample = np.linspace(0, ne - 1, 64).astype(int)
+    # accuracy of the mixed path against the FP64 path on a sample of elements (four
+    # runs of 512 consecutive elements spread over the mesh)
+    chunks = [(int(e0), min(512, ne - int(e0))) for e0 in np.linspace(0, max(ne - 512, 0), 4)]
     errs = {}
     if "mixed" in res and "fp64" in res:
         outs = {}
@@ -356,11 +362,11 @@ def main():
             ctx.assemble_element_matrix_vector(0, ne)
             ctx.schur_element_matrix_vector(0, ne, precs[name])
             blocks = {}
-            for e in sample:
-                r = ctx.read_condensed(int(e), 1)
+            for e0, n in chunks:
+                r = ctx.read_condensed(e0, n)
                 for k in ("DinvRu", "DinvF", "AH", "FH"):
-                    blocks.setdefault(k, []).append(r[k][0])
-            outs[name] = {k: np.array(v) for k, v in blocks.items()}
+                    blocks.setdefault(k, []).append(r[k])
+            outs[name] = {k: np.concatenate(v) for k, v in blocks.items()}
         errs = {k: relerr(outs["mixed"][k], outs["fp64"][k]) for k in outs["fp64"]}
         print(f"[bench] mixed vs fp64 max rel err: {errs}", file=sys.stderr, flush=True)
 
@@ -397,6 +403,7 @@ def main():
         "mixed_elements_per_s": res.get("mixed", {}).get("eps"),
         "mixed_speedup_vs_fp64": (res["mixed"]["eps"] / res["fp64"]["eps"]) if len(res) == 2 else None,
         "mixed_max_rel_err_vs_fp64": errs or None,
+        "mixed_err_sample_elements": sum(n for _, n in chunks) if errs else None,
         "kernel_ms": {k: {"assemble": v["asm_ms"], "condense": v["cond_ms"]} for k, v in res.items()},
         "condense_tuning": tuning,
         "mixed_refinement": res.get("mixed", {}).get("refinement"),
