@@ -169,15 +169,17 @@ def dirk_step_hex(ncell=24, dt=0.1):
 
 def ragged_batch(steps, warmup, total=20000, ncell=10, seed=0):
     """Assembly + FP64 condensation of an element batch of p=1..4 tets, the order of
-    each of `total` elements drawn uniformly by the seed.  One context per order,
-    each on its own stream (elements of one order are contiguous in its context;
-    the meshes are the 10^3 x 6 jittered cube, of which the first count_p
-    elements are taken).  Timed two ways over the whole batch: every order's
-    launches queued before one synchronisation (kernels of different orders may
-    run side by side), and one order after another with a synchronisation in
-    between (the one-launch-per-order baseline)."""
+    each of `total` elements drawn uniformly by the seed.  One context per order
+    (elements of one order are contiguous in its context; the meshes are the
+    10^3 x 6 jittered cube, of which the first count_p elements are taken).
+    Timed three ways over the whole batch: one order after another with a
+    synchronisation in between (the one-launch-per-order baseline); every order's
+    launches queued on the orders' own streams before one synchronisation
+    (kernels of different orders may run side by side); and the ragged batch
+    (hfem_batch_*: one persistent launch per kernel over all orders, its shapes
+    picked per order by the batch tuner)."""
     from hfem_amd.problem import make_problem
-    from hfem_amd.lib import HDGContext, FP64
+    from hfem_amd.lib import HDGBatch, HDGContext, FP64
     rng = np.random.default_rng(seed)
     counts = np.bincount(rng.integers(1, 5, size=total), minlength=5)[1:]
     ctxs = []
@@ -215,11 +217,52 @@ def ragged_batch(steps, warmup, total=20000, ncell=10, seed=0):
         ctx.schur_element_matrix_vector(0, cnt, FP64)
         a, c = ctx.kernel_ms()
         per_order_ms[f"p{p}"] = {"elements": cnt, "assemble_ms": a, "condense_ms": c}
+    # the same elements as one ragged batch: one flux, one assembly and one
+    # inverse launch over all orders, the products per order on the orders' streams
+    batch = HDGBatch([c for _, _, c in ctxs], [cnt for _, cnt, _ in ctxs])
+    waves, tasm, tinv = batch.tune()
+    print(f"[bench] ragged batch tuned: inverse waves per element {waves}", file=sys.stderr, flush=True)
+
+    def run_batch():
+        batch.assemble_element_matrix_vector()
+        batch.schur_element_matrix_vector()
+        batch.sync()
+
+    tuned_modes, tuned_ms = batch.launch()
+
+    def time_batch(modes):
+        batch.set_launch(*modes)
+        for _ in range(warmup):
+            run_batch()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            run_batch()
+        wall = (time.perf_counter() - t0) / steps
+        ba, bc = batch.kernel_ms()
+        print(f"[bench] ragged batch, launch forms {modes}: {wall * 1e3:.2f} ms per batch, {total / wall:.0f} "
+              f"elements/s (assembly {ba:.2f} ms, condensation {bc:.2f} ms)", file=sys.stderr, flush=True)
+        return {"ms_per_batch": wall * 1e3, "elements_per_s": total / wall, "assemble_ms": ba, "condense_ms": bc}
+
+    res["batch_one_launch"] = time_batch((0, 0))
+    res["batch"] = time_batch(tuned_modes)
+    res["batch"]["launch_forms"] = {"assembly": ["one launch", "per order"][tuned_modes[0]],
+                                    "inverse": ["one launch", "per order"][tuned_modes[1]],
+                                    "tuning_ms": tuned_ms}
+    res["batch"]["tuning"] = {f"p{p}": {"inverse_waves_per_element": w, "assemble_ms_alone": a, "inverse_ms_alone": i}
+                              for (p, _, _), w, a, i in zip(ctxs, waves, tasm, tinv)}
+    batch.close()
     print(json.dumps({"mode": "ragged", "workload": f"3D NS HDG tets of orders 1-4 drawn uniformly by the seed, "
                       f"{total} elements, assemble+condense (FP64)", "elements": total, "steps": steps,
-                      "launch": "one context per order on its own stream; not a single persistent launch",
+                      "launch": "per_order / concurrent: one context per order on its own stream; "
+                      "batch_one_launch: one persistent launch each for the fluxes, the assembly and the inverse "
+                      "over all orders, the two condensation products per order; batch: the launch forms the "
+                      "batch tuner picked",
                       **res, "speedup_concurrent_vs_per_order": res["per_order"]["ms_per_batch"]
-                      / res["concurrent"]["ms_per_batch"], "per_order": per_order_ms}))
+                      / res["concurrent"]["ms_per_batch"],
+                      "speedup_batch_vs_per_order": res["per_order"]["ms_per_batch"] / res["batch"]["ms_per_batch"],
+                      "speedup_batch_one_launch_vs_per_order": res["per_order"]["ms_per_batch"]
+                      / res["batch_one_launch"]["ms_per_batch"],
+                      "per_order": per_order_ms}))
     for _, _, ctx in ctxs:
         ctx.close()
 

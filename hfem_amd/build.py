@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libhfem_amd.so")
 OBJDIR = os.path.join(HERE, "_build")
-SOURCES = ["api.hip", "assemble.hip", "condense.hip", "bgemm.hip", "xsolve.hip", "trace.hip", "solver.hip", "solve.hip", "noalloc.hip"]
+SOURCES = ["api.hip", "assemble.hip", "condense.hip", "bgemm.hip", "xsolve.hip", "trace.hip", "solver.hip", "solve.hip", "noalloc.hip", "batch.hip"]
 
 HIPCC_FLAGS = [
     "-O3", "--offload-arch=gfx950", "-fPIC", "-std=c++17",

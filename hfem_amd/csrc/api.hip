@@ -649,14 +649,21 @@ extern "C" int hfem_set_condensation_choice(hfem_ctx *c, const int *choice)
     return 0;
 }
 
+// The two products of the FP64 condensation of [e0, e0+n) from inv(D) in dinv:
+// X = inv(D)[F Ru], [AH FH] = [H Rh] - K X (the context's tuned shapes).
+int condense_products_fp64(hfem_ctx *c, int e0, int n)
+{
+    HIPCHK(run_prod(c, 0, c->prod_var[0], e0, n, c->dinv, c->Fx, c->D, c->K, c->X, c->Hx, nullptr, c->gemm_sol[0]));
+    HIPCHK(run_prod(c, 1, c->prod_var[1], e0, n, c->dinv, c->Fx, c->D, c->K, c->X, c->Hx, nullptr, c->gemm_sol[1]));
+    return 0;
+}
+
 // The FP64 condensation of [e0, e0+n): inv(D), X = inv(D)[F Ru], [AH FH] = [H Rh] - K X.
 static int condense_fp64(hfem_ctx *c, int e0, int n)
 {
     const Dims &d = c->d;
     HIPCHK(launch_dinv(d.na, e0, n, c->D, c->dinv, c->info, c->stream, c->inv_var[0]));
-    HIPCHK(run_prod(c, 0, c->prod_var[0], e0, n, c->dinv, c->Fx, c->D, c->K, c->X, c->Hx, nullptr, c->gemm_sol[0]));
-    HIPCHK(run_prod(c, 1, c->prod_var[1], e0, n, c->dinv, c->Fx, c->D, c->K, c->X, c->Hx, nullptr, c->gemm_sol[1]));
-    return 0;
+    return condense_products_fp64(c, e0, n);
 }
 
 extern "C" int hfem_condense(hfem_ctx *c, int e0, int n, int precision)

@@ -233,6 +233,8 @@ struct hfem_ctx {
 
 
 int hfem_reduce_sum(hfem_ctx *c, long n, const double *x, int square, double *out);
+// X = inv(D)[F Ru] and the Schur product of [e0, e0+n) from the inverse in c->dinv
+int condense_products_fp64(hfem_ctx *c, int e0, int n);
 
 template <class T>
 static inline hipError_t hfem_dalloc(T **p, size_t n)

@@ -25,6 +25,45 @@ struct Phys {
     int tdep;
 };
 
+// Element sources: which items (elements, or rounds of elements) of a range
+// [0, n) a workgroup takes.  A kernel body runs
+//   for (t = src.first(); t < n; t = src.next(t)) { ... }
+// with every thread of the workgroup calling next() together.
+//
+// Workgroup b takes items b, b + grid, b + 2 grid, ...
+struct StrideSource {
+    __device__ int first() const { return blockIdx.x; }
+    __device__ int next(int t) const { return t + gridDim.x; }
+};
+
+// Tickets from a device counter shared by the workgroups of a ragged launch: the
+// ranges of several buckets are laid end to end, [lo, lo + n) being the current
+// one.  next() draws the next ticket (thread 0 adds one to the counter and
+// passes the value on through LDS); a ticket outside the current range ends the
+// run and is kept in carry for the kernel to dispatch.  Tickets grow
+// monotonically for each workgroup, so a workgroup visits the buckets in order.
+struct TicketSource {
+    unsigned *ctr;
+    int *slot;      // one int of LDS
+    int lo, n, carry;
+    __device__ int draw()
+    {
+        if (threadIdx.x == 0) *slot = (int)atomicAdd(ctr, 1u);
+        __syncthreads();
+        const int v = __builtin_amdgcn_readfirstlane(*slot);
+        __syncthreads();
+        return v;
+    }
+    __device__ int first() const { return carry - lo; }
+    __device__ int next(int)
+    {
+        const int v = draw();
+        if (v >= lo && v < lo + n) return v - lo;
+        carry = v;
+        return n;
+    }
+};
+
 // Forward-mode dual number with one derivative direction.  The flux Jacobian at
 // a quadrature point is formed one column per thread: the thread seeds its
 // direction and evaluates the flux once.

@@ -82,6 +82,16 @@ EXPORTS = {
     "hfem_solver_stats": (ctypes.c_int, [_vp, _dp]),
     "hfem_last_kernel_ms": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
     "hfem_sync": (ctypes.c_int, [_vp]),
+    "hfem_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _ip, ctypes.POINTER(_vp)]),
+    "hfem_batch_destroy": (ctypes.c_int, [_vp]),
+    "hfem_batch_assemble": (ctypes.c_int, [_vp]),
+    "hfem_batch_condense": (ctypes.c_int, [_vp]),
+    "hfem_batch_tune": (ctypes.c_int, [_vp, _ip, ctypes.POINTER(ctypes.c_float)]),
+    "hfem_batch_set_inverse_waves": (ctypes.c_int, [_vp, _ip]),
+    "hfem_batch_set_launch": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "hfem_batch_get_launch": (ctypes.c_int, [_vp, _ip, ctypes.POINTER(ctypes.c_float)]),
+    "hfem_batch_kernel_ms": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
+    "hfem_batch_sync": (ctypes.c_int, [_vp]),
 }
 EXTRA_EXPORTS = {}
 
@@ -392,3 +402,69 @@ class HDGContext:
         _check(load().hfem_dirk_step(self.h, dt, nstage, order, ctypes.byref(p), ctypes.byref(conv),
                                      ctypes.byref(nit)))
         return bool(conv.value), nit.value
+
+
+class HDGBatch:
+    """Ragged batch over contexts of different 3D element families (tets of orders
+    1-4, hexes of orders 1-2, one context per family): the element path of every
+    context in single persistent launches (hfem_batch_* in include/hfem_amd.h).
+    Each context's buffers receive what its own assemble / schur calls write, and
+    the contexts stay usable on their own (read_elements, read_condensed ...)."""
+
+    def __init__(self, contexts, nelem=None):
+        """``nelem``: the batch takes elements [0, nelem[k]) of contexts[k] (default all)."""
+        L = load()
+        self.contexts = list(contexts)
+        arr = (_vp * len(self.contexts))(*[c.h for c in self.contexts])
+        ne = None if nelem is None else _i32(nelem)
+        h = _vp()
+        _check(L.hfem_batch_create(arr, len(self.contexts), None if ne is None else _i(ne), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().hfem_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def assemble_element_matrix_vector(self):
+        _check(load().hfem_batch_assemble(self.h))
+
+    def schur_element_matrix_vector(self):
+        _check(load().hfem_batch_condense(self.h))
+
+    def tune(self):
+        """Per bucket: the inverse's waves per element and the bucket order; returns
+        (waves per context, assembly ms per context, inverse ms per context)."""
+        n = len(self.contexts)
+        w = np.zeros(n, dtype=np.int32)
+        ms = (ctypes.c_float * (2 * n))()
+        _check(load().hfem_batch_tune(self.h, _i(w), ms))
+        return w.tolist(), list(ms[:n]), list(ms[n:])
+
+    def set_inverse_waves(self, waves):
+        _check(load().hfem_batch_set_inverse_waves(self.h, _i(_i32(waves))))
+
+    def set_launch(self, assemble_mode, inverse_mode):
+        """0: one persistent launch over all buckets, 1: one launch per bucket."""
+        _check(load().hfem_batch_set_launch(self.h, assemble_mode, inverse_mode))
+
+    def launch(self):
+        """(assembly mode, inverse mode) and the last tuning's times {assemble_one_launch,
+        assemble_per_bucket, inverse_one_launch, inverse_per_bucket} in ms."""
+        m = np.zeros(2, dtype=np.int32)
+        t = (ctypes.c_float * 4)()
+        _check(load().hfem_batch_get_launch(self.h, _i(m), t))
+        return (int(m[0]), int(m[1])), dict(zip(("assemble_one_launch", "assemble_per_bucket", "inverse_one_launch",
+                                                  "inverse_per_bucket"), list(t)))
+
+    def kernel_ms(self):
+        a = ctypes.c_float()
+        c = ctypes.c_float()
+        _check(load().hfem_batch_kernel_ms(self.h, ctypes.byref(a), ctypes.byref(c)))
+        return a.value, c.value
+
+    def sync(self):
+        _check(load().hfem_batch_sync(self.h))

@@ -114,6 +114,53 @@ int hfem_set_condensation_choice(hfem_ctx *ctx, const int *choice);
 /* Number of hand-written product variants (valid product choices 0 .. n-1). */
 int hfem_condense_variants(void);
 
+/* Ragged element batches.  No counterpart in the reference, which runs one
+ * element family per process (main/digasoser.cpp:62-63): the elements of up to
+ * six contexts of different 3D families (tets of orders 1-4, hexes of orders
+ * 1-2, one context per family, one device, at most one flux chunk of 32768
+ * elements each) go through single persistent launches with ragged per-bucket
+ * offsets.  Each element gets the per-element work of hfem_assemble /
+ * hfem_condense (FP64), and each context's buffers receive exactly what those
+ * entry points write.  The contexts stay usable on their own; batch calls are
+ * ordered after the work already queued on the contexts' streams and before
+ * work queued on them later. */
+typedef struct hfem_batch hfem_batch;
+/* nelem[nctx] (may be NULL: every element): the batch takes elements
+ * [0, nelem[k]) of ctxs[k].  Destroy the batch before its contexts. */
+int hfem_batch_create(hfem_ctx *const *ctxs, int nctx, const int *nelem, hfem_batch **batch);
+int hfem_batch_destroy(hfem_batch *batch);
+/* assembleElementMatrixVector (assembly/elementassembly.cpp:1481-1499) over every
+ * element of every context: one flux launch and one assembly launch. */
+int hfem_batch_assemble(hfem_batch *batch);
+/* schurElementMatrixVector (assembly/elementschur.cpp:331-343) in FP64 over every
+ * element: one inverse launch, then each context's two products with its own
+ * tuned shapes on its own stream. */
+int hfem_batch_condense(hfem_batch *batch);
+/* Per bucket (context): runs hfem_tune_condensation on the context, times its
+ * assembly and the inverse with 8, 4 or 2 waves per element (those its tile
+ * count allows) in the batch launch, keeps the fastest and orders the buckets by
+ * measured cost per element, most expensive first; then picks the launch forms
+ * (hfem_batch_set_launch).  waves[nctx] (may be NULL)
+ * receives the inverse's waves per element, ms[2*nctx] (may be NULL) the
+ * assembly and inverse milliseconds of each bucket alone.  Assembles (the
+ * element blocks are overwritten). */
+int hfem_batch_tune(hfem_batch *batch, int *waves, float *ms);
+/* Waves per element of the batch inverse for each context (8, 4 or 2). */
+int hfem_batch_set_inverse_waves(hfem_batch *batch, const int *waves);
+/* Launch form of the batch assembly (flux and assembly kernels) and of the batch
+ * inverse: 0 one persistent launch over all buckets (the default), 1 one launch
+ * per bucket with the per-family kernels, on the batch stream.  hfem_batch_tune
+ * times both over the whole batch and keeps the faster. */
+int hfem_batch_set_launch(hfem_batch *batch, int assemble_mode, int inverse_mode);
+/* modes[2] (may be NULL): the current launch forms; tuned_ms[4] (may be NULL): the
+ * last tuning's times of the assembly in one launch / per bucket and of the
+ * inverse in one launch / per bucket. */
+int hfem_batch_get_launch(hfem_batch *batch, int *modes, float *tuned_ms);
+/* Milliseconds of the last hfem_batch_assemble and hfem_batch_condense (HIP events). */
+int hfem_batch_kernel_ms(hfem_batch *batch, float *assemble_ms, float *condense_ms);
+/* Synchronise the batch stream and the contexts' streams. */
+int hfem_batch_sync(hfem_batch *batch);
+
 /* No counterpart in the reference: for callers that cannot allocate device memory
  * (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
  * The FP64 condensation of hfem_condense on caller-owned device blocks of n

@@ -1,7 +1,18 @@
-"""Element path on a batch large enough that every workgroup assembles and
-condenses several elements in turn (GPU): a sample of elements, including ones
-late in each workgroup's sequence and in later assembly chunks, against the
-CPU oracle.  Tolerances as in test_gpu_parity.py."""
+"""Ragged element batches (hfem_batch_*, GPU): the element path of contexts of
+different families -- tets of orders 1-4, hexes of orders 1-2 -- in single
+persistent launches.
+
+* Every context receives bit for bit what its own hfem_assemble / hfem_condense
+  write (the batch kernels run the same per-element bodies; the inverse's
+  waves per element change which wave holds a tile, not the arithmetic), with
+  the buckets' element counts chosen so that the inverse's groups of 2 and 4
+  elements per workgroup end on partial rounds.
+* Directly against the oracle on a batch of the per-context parity test's
+  small tets of orders 1-3 (its seeds and tolerances).
+* After the batch tuner has picked the waves per element, the bucket order and
+  the launch forms (one persistent launch, or one launch per bucket), and with
+  every launch form forced, the results are unchanged.
+"""
 import numpy as np
 import pytest
 
@@ -9,50 +20,133 @@ from hfem_amd.problem import make_problem
 
 pytestmark = pytest.mark.gpu
 
+FAMILIES = [("tet", 3, 1), ("tet", 2, 2), ("tet", 1, 3), ("tet", 1, 4), ("hex", 3, 1), ("hex", 3, 2)]
+ELEM = ("D", "F", "K", "H", "Ru", "Rh")
+COND = ("DinvRu", "DinvF", "AH", "FH", "info")
 
-def relerr(a, b):
-    return float((np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)).max())
+
+def say(*a):
+    print(*a, flush=True)
 
 
-@pytest.mark.parametrize("kind,nc,p", [("tet", 8, 3), ("tri", 40, 2)])
-def test_large_batch_sample_vs_oracle(oracle_mod, kind, nc, p):
-    from hfem_amd.lib import HDGContext, FP64, MIXED
+def per_context(ctxs):
+    from hfem_amd.lib import FP64
+    out = []
+    for c in ctxs:
+        c.assemble_element_matrix_vector()
+        e = c.read_elements()
+        c.schur_element_matrix_vector(precision=FP64)
+        out.append((e, c.read_condensed()))
+    return out
+
+
+def batched(batch, ctxs):
+    batch.assemble_element_matrix_vector()
+    el = [c.read_elements() for c in ctxs]
+    batch.schur_element_matrix_vector()
+    return [(e, c.read_condensed()) for e, c in zip(el, ctxs)]
+
+
+def assert_same(a, b, names):
+    for (ea, ca), (eb, cb), nm in zip(a, b, names):
+        for k in ELEM:
+            assert np.array_equal(ea[k], eb[k]), (nm, k)
+        for k in COND:
+            assert np.array_equal(ca[k], cb[k]), (nm, k)
+
+
+def test_batch_equals_per_context_calls():
+    from hfem_amd.lib import HDGBatch, HDGContext
+    probs = [make_problem(k, n, p, seed=5) for k, n, p in FAMILIES]
+    names = [f"{k}{n} p{p} ne={pr.ne}" for (k, n, p), pr in zip(FAMILIES, probs)]
+    say("batch:", ", ".join(names))
+    ctxs = [HDGContext(pr) for pr in probs]
+    want = per_context(ctxs)
+    batch = HDGBatch(ctxs)
+    got = batched(batch, ctxs)
+    assert_same(got, want, names)
+    say("  default shapes: bit-identical")
+    waves, tasm, tinv = batch.tune()
+    say(f"  tuned waves per element {waves}, assembly ms {np.round(tasm, 3)}, inverse ms {np.round(tinv, 3)}")
+    for (k, n, p), w, pr in zip(FAMILIES, waves, probs):
+        nt = (pr.dims["na"] + 15) // 16
+        assert w in ((8,) if nt > 8 else (4, 2) if nt <= 4 else (8, 4, 2))
+    # the tuner also tuned each context's products: per-context calls again
+    want = per_context(ctxs)
+    got = batched(batch, ctxs)
+    assert_same(got, want, names)
+    say("  tuned shapes: bit-identical")
+    nts = [(pr.dims["na"] + 15) // 16 for pr in probs]
+    for rule in (lambda nt: 8 if nt > 8 else 2, lambda nt: 8 if nt > 8 else 4, lambda nt: 8 if nt > 4 else 4):
+        w = [rule(nt) for nt in nts]
+        batch.set_inverse_waves(w)
+        got = batched(batch, ctxs)
+        assert_same(got, want, names)
+        say(f"  inverse waves {w}: bit-identical")
+    (ma, mi), tm = batch.launch()
+    say(f"  tuned launch forms: assembly {ma}, inverse {mi} ({tm})")
+    for modes in ((1, 1), (0, 1), (1, 0), (0, 0)):
+        batch.set_launch(*modes)
+        got = batched(batch, ctxs)
+        assert_same(got, want, names)
+        say(f"  launch forms {modes}: bit-identical")
+    batch.close()
+    for c in ctxs:
+        c.close()
+
+
+def test_batch_against_oracle(oracle_mod):
+    from hfem_amd.lib import HDGBatch, HDGContext
+    from test_gpu_parity import RTOL_MAT, RTOL_RES, relerr, relerr_res, res_floor
     O = oracle_mod
-    pr = make_problem(kind, nc, p, seed=2)
-    d = pr.dims
-    print(f"case {kind}{nc} p{p}: ne={pr.ne}", flush=True)
-    assert pr.ne > 2048
-    sample = np.unique(np.r_[0, 1, 511, 512, 513, 1023, 1024, 2047, 2048, pr.ne - 1,
-                             np.linspace(0, pr.ne - 1, 22).astype(int)])
-    orc = O.Oracle(pr)
-    ref = {k: [] for k in ("D", "F", "K", "H", "Rh")}
-    cond = {k: [] for k in ("DinvF", "AH")}
-    for e in sample:
-        b = orc.assemble(int(e), 1)
-        for k in ref:
-            ref[k].append(b[k][0].copy())
+    # the per-context parity cases (same seeds, same tolerances), run as one batch
+    fams = [("tet", 1, 1), ("tet", 1, 2), ("tet", 1, 3)]
+    probs = [make_problem(k, n, p, seed=3) for k, n, p in fams]
+    ctxs = [HDGContext(pr) for pr in probs]
+    batch = HDGBatch(ctxs)
+    got = batched(batch, ctxs)
+    for (k, n, p), pr, (g, c) in zip(fams, probs, got):
+        say(f"oracle {k}{n} p{p}: ne={pr.ne}")
+        ref = O.Oracle(pr).assemble()
+        for key in ("D", "F", "K", "H", "Rh"):
+            assert relerr(g[key], ref[key]) < RTOL_MAT, key
+        fl_ru = res_floor(ref["Ru"], ref["Rh"])
+        assert relerr_res(g["Ru"], ref["Ru"], fl_ru) < RTOL_RES
+        d = pr.dims
+        b = {kk: v.copy() for kk, v in ref.items()}
         O.schur(0, d["npv"], d["ncu"], b)
-        cond["DinvF"].append(b["F"][0])
-        cond["AH"].append(b["H"][0])
-    ref = {k: np.array(v) for k, v in ref.items()}
-    cond = {k: np.array(v) for k, v in cond.items()}
-    ctx = HDGContext(pr)
-    ctx.assemble_element_matrix_vector()
-    got = {k: [] for k in ref}
-    for e in sample:
-        g = ctx.read_elements(int(e), 1)
-        for k in ref:
-            got[k].append(g[k][0])
-    for k in ref:
-        err = relerr(np.array(got[k]), ref[k])
-        print(f"  assemble {k}: {err:.2e}", flush=True)
-        assert err < 1e-13, k
-    for prec, tol in ((FP64, 1e-13), (MIXED, 1e-10)):
-        ctx.assemble_element_matrix_vector()
-        ctx.schur_element_matrix_vector(precision=prec)
-        for k in cond:
-            vals = np.array([ctx.read_condensed(int(e), 1)[k][0] for e in sample])
-            err = relerr(vals, cond[k])
-            print(f"  precision {prec} {k}: {err:.2e}", flush=True)
-            assert err < tol, (prec, k)
-    ctx.close()
+        nr = np.linalg.norm(ref["Ru"].reshape(pr.ne, -1), axis=1)
+        fl_y = fl_ru * np.linalg.norm(b["Ru"].reshape(pr.ne, -1), axis=1) / np.maximum(nr, 1e-300)
+        assert np.all(c["info"] == 0)
+        assert relerr(c["DinvF"], b["F"]) < RTOL_MAT
+        assert relerr(c["AH"], b["H"]) < RTOL_MAT
+        assert relerr(c["FH"], b["Rh"]) < RTOL_MAT
+        assert relerr_res(c["DinvRu"], b["Ru"], fl_y) < RTOL_RES
+    batch.close()
+    for c in ctxs:
+        c.close()
+
+
+def test_batch_rejects_what_it_cannot_run():
+    from hfem_amd.lib import HDGBatch, HDGContext, HfemError
+    a = HDGContext(make_problem("tet", 1, 2, seed=1))
+    b = HDGContext(make_problem("tet", 1, 2, seed=2))
+    t = HDGContext(make_problem("tri", 2, 2, seed=1))
+    with pytest.raises(HfemError, match="same element family"):
+        HDGBatch([a, b])
+    with pytest.raises(HfemError, match="not a batch family"):
+        HDGBatch([a, t])
+    with pytest.raises(HfemError, match="given twice"):
+        HDGBatch([a, a])
+    with pytest.raises(HfemError, match="outside"):
+        HDGBatch([])
+    bt = HDGBatch([a])
+    with pytest.raises(HfemError, match="waves per element not available"):
+        bt.set_inverse_waves([8])
+    with pytest.raises(HfemError, match="modes must be 0 or 1"):
+        bt.set_launch(2, 0)
+    with pytest.raises(HfemError, match="outside"):
+        HDGBatch([a], [a.prob.ne + 1])
+    bt.close()
+    for c in (a, b, t):
+        c.close()
