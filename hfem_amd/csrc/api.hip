@@ -132,6 +132,7 @@ extern "C" int hfem_destroy(hfem_ctx *c)
     for (auto &ev : c->sv.ev)
         if (ev) hipEventDestroy(ev);
     if (c->sv.apply_graph) hipGraphExecDestroy(c->sv.apply_graph);
+    if (c->lin_exec) hipGraphExecDestroy(c->lin_exec);
     if (c->blas) rocblas_destroy_handle(c->blas);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -302,11 +303,11 @@ static int assemble_range(hfem_ctx *c, int e0, int n, int res, const char *who)
     HIPCHK(hipSetDevice(c->device));
     Master m{c->shapvt, c->shapvg, c->shapft, c->shapfg, c->perm, c->invperm};
     const int grid = std::min(n, c->grid_asm);
-    HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    if (!c->capturing) HIPCHK(hipEventRecord(c->ev[0], c->stream));
     HIPCHK(launch_assemble(c->d, m, c->ph, c->dgnodes, c->UDG, c->UH, c->SH, c->elcon, c->bf, e0, n, grid, c->ldx,
                            c->fluxbuf, c->fstride, c->chunk, c->Ru, c->D, c->Fx, c->K, c->Hx, c->Rh, c->ru2,
                            c->spill, c->stream, res));
-    HIPCHK(hipEventRecord(c->ev[1], c->stream));
+    if (!c->capturing) HIPCHK(hipEventRecord(c->ev[1], c->stream));
     return 0;
 }
 
@@ -674,7 +675,7 @@ extern "C" int hfem_condense(hfem_ctx *c, int e0, int n, int precision)
     if (n == 0) return 0;
     HIPCHK(hipSetDevice(c->device));
     const Dims &d = c->d;
-    HIPCHK(hipEventRecord(c->ev[2], c->stream));
+    if (!c->capturing) HIPCHK(hipEventRecord(c->ev[2], c->stream));
     c->last_sweeps = 0;
     c->last_unconverged = 0;
     c->last_fallback = 0;
@@ -718,7 +719,7 @@ extern "C" int hfem_condense(hfem_ctx *c, int e0, int n, int precision)
                             c->gemm_sol[1]));
         }
     }
-    HIPCHK(hipEventRecord(c->ev[3], c->stream));
+    if (!c->capturing) HIPCHK(hipEventRecord(c->ev[3], c->stream));
     return 0;
 }
 
@@ -920,9 +921,12 @@ extern "C" int hfem_assemble_rhs(hfem_ctx *c, double *rNorm)
     return 0;
 }
 
-extern "C" int hfem_assemble_linear_system(hfem_ctx *c, int precision, double *rNorm)
+// The device work of one linearisation: element assembly, the residual norm's
+// two sums of squares (left on the device in partial[512] and partial[513]), the
+// condensation and the BSR scatter.  The FP64 path has no host synchronisation
+// in it, so it can be captured as a graph.
+static int linearize(hfem_ctx *c, int precision)
 {
-    if (!c) return hfem_fail(-1, "hfem_assemble_linear_system: null context");
     const int ne = c->hd.ne, nf = c->hd.nf;
     const Dims &d = c->d;
     int rc = hfem_assemble(c, 0, ne);
@@ -931,10 +935,10 @@ extern "C" int hfem_assemble_linear_system(hfem_ctx *c, int precision, double *r
     // r = assembled uncondensed Rh (Rhonly)
     HIPCHK(launch_face_gather(nf, bsz, d.nch, d.npf, d.nb, d.nb, c->f2e, c->einv, c->Rh, c->r, nullptr, nullptr,
                               c->stream));
-    double s1 = 0, s2 = 0;
-    if ((rc = hfem_reduce_sum(c, ne, c->ru2, 0, &s1))) return rc;
-    if ((rc = hfem_reduce_sum(c, (long)nf * bsz, c->r, 1, &s2))) return rc;
-    if (rNorm) *rNorm = sqrt(s1 + s2);
+    HIPCHK(launch_sumsq(ne, c->ru2, 0, c->partial, 256, c->stream));
+    HIPCHK(launch_sumsq(256, c->partial, 0, c->partial + 512, 1, c->stream));
+    HIPCHK(launch_sumsq((long)nf * bsz, c->r, 1, c->partial + 256, 256, c->stream));
+    HIPCHK(launch_sumsq(256, c->partial + 256, 0, c->partial + 513, 1, c->stream));
     if ((rc = hfem_condense(c, 0, ne, precision))) return rc;
     if (c->qn_acc >= 0 && (rc = store_quasi_newton(c, precision))) return rc;
     const long es = (long)d.nb * (d.nb + 1);
@@ -942,7 +946,81 @@ extern "C" int hfem_assemble_linear_system(hfem_ctx *c, int precision, double *r
                              c->stream));
     HIPCHK(launch_face_gather(nf, bsz, d.nch, d.npf, d.nb, es, c->f2e, c->einv, c->Hx + (long)d.nb * d.nb, c->Rg,
                               nullptr, nullptr, c->stream));
+    return 0;
+}
+
+static hfem_ctx::LinKey lin_key(const hfem_ctx *c)
+{
+    hfem_ctx::LinKey k;
+    memset(&k, 0, sizeof k);
+    k.ph = c->ph;
+    k.inv_var0 = c->inv_var[0];
+    k.prod_var0 = c->prod_var[0];
+    k.prod_var1 = c->prod_var[1];
+    k.gemm_sol0 = c->gemm_sol[0];
+    k.gemm_sol1 = c->gemm_sol[1];
+    const void *ptr[] = {c->shapvt, c->shapvg, c->shapft, c->shapfg, c->perm, c->invperm, c->dgnodes, c->elcon, c->bf,
+                         c->UDG, c->UH, c->SH, c->f2e, c->einv, c->blkptr, c->blklist, c->Hg, c->Rg, c->r, c->rwork};
+    static_assert(sizeof ptr == sizeof k.ptr, "linearisation key size");
+    memcpy(k.ptr, ptr, sizeof ptr);
+    return k;
+}
+
+static void drop_lin_graph(hfem_ctx *c)
+{
+    if (c->lin_exec) (void)hipGraphExecDestroy(c->lin_exec);
+    c->lin_exec = nullptr;
+}
+
+// No counterpart in the reference: whether hfem_assemble_linear_system replays the
+// FP64 linearisation as a captured graph (1, the default) or launches it kernel by
+// kernel (0).  Quasi-Newton storage and the mixed condensation always launch
+// kernel by kernel.
+extern "C" int hfem_set_linearization_graph(hfem_ctx *c, int use)
+{
+    if (!c) return hfem_fail(-1, "hfem_set_linearization_graph: null context");
+    c->lin_graph = use != 0;
+    if (!c->lin_graph) drop_lin_graph(c);
+    return 0;
+}
+
+extern "C" int hfem_assemble_linear_system(hfem_ctx *c, int precision, double *rNorm)
+{
+    if (!c) return hfem_fail(-1, "hfem_assemble_linear_system: null context");
+    if (!c->has_master || !c->has_phys || !c->has_mesh || !c->has_sol)
+        return hfem_fail(-3, "hfem_assemble_linear_system: master, physics, mesh and solution must be set first");
+    if (precision != HFEM_FP64 && precision != HFEM_MIXED)
+        return hfem_fail(-1, "hfem_assemble_linear_system: unknown precision %d", precision);
+    HIPCHK(hipSetDevice(c->device));
+    int rc;
+    if (c->lin_graph && precision == HFEM_FP64 && c->qn_acc < 0) {
+        const hfem_ctx::LinKey k = lin_key(c);
+        if (c->lin_exec && memcmp(&k, &c->lin_key, sizeof k) != 0) drop_lin_graph(c);
+        if (!c->lin_exec) {
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+            c->capturing = true;
+            rc = linearize(c, precision);
+            c->capturing = false;
+            hipError_t e = hipStreamEndCapture(c->stream, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            HIPCHK(e);
+            e = hipGraphInstantiate(&c->lin_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIPCHK(e);
+            c->lin_key = k;
+        }
+        HIPCHK(hipGraphLaunch(c->lin_exec, c->stream));
+    } else if ((rc = linearize(c, precision))) {
+        return rc;
+    }
+    double s[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(s, c->partial + 512, sizeof s, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    if (rNorm) *rNorm = sqrt(s[0] + s[1]);
     return 0;
 }
 

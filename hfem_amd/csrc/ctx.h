@@ -229,6 +229,16 @@ struct hfem_ctx {
     void *qn_dinv = nullptr, *qn_K = nullptr;
     // trace-system solver (solve.hip)
     SolverState sv;
+    // linearisation graph (hfem_set_linearization_graph): the FP64 element path and
+    // BSR scatter of hfem_assemble_linear_system captured once and replayed, with
+    // the launch arguments it was captured with (recaptured when they change)
+    bool lin_graph = true, capturing = false;
+    hipGraphExec_t lin_exec = nullptr;
+    struct LinKey {
+        Phys ph;
+        int inv_var0, prod_var0, prod_var1, gemm_sol0, gemm_sol1;
+        const void *ptr[20];   // buffers the setters may (re)allocate
+    } lin_key;
 };
 
 

@@ -82,6 +82,7 @@ EXPORTS = {
     "hfem_solver_stats": (ctypes.c_int, [_vp, _dp]),
     "hfem_last_kernel_ms": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
     "hfem_sync": (ctypes.c_int, [_vp]),
+    "hfem_set_linearization_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
     "hfem_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _ip, ctypes.POINTER(_vp)]),
     "hfem_batch_destroy": (ctypes.c_int, [_vp]),
     "hfem_batch_assemble": (ctypes.c_int, [_vp]),
@@ -336,6 +337,10 @@ class HDGContext:
         precision (sys.precPrecision)."""
         _check(load().hfem_set_precond_precision(self.h, prec_precision))
         _check(load().hfem_precond_factor(self.h, kind))
+
+    def set_linearization_graph(self, use=True):
+        """Replay the FP64 linearisation of assemble_linear_system as a HIP graph."""
+        _check(load().hfem_set_linearization_graph(self.h, int(bool(use))))
 
     def set_apply_graph(self, use=True):
         """Replay the BILU0 application as a captured HIP graph, or launch its

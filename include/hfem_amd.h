@@ -312,6 +312,15 @@ int hfem_last_kernel_ms(hfem_ctx *ctx, float *assemble_ms, float *condense_ms);
 /* Synchronise the context stream. */
 int hfem_sync(hfem_ctx *ctx);
 
+/* No counterpart in the reference: use = 1 makes hfem_assemble_linear_system (and
+ * so every Newton iteration of hfem_solve_steady and hfem_dirk_step) replay its
+ * FP64 device work -- element fluxes and assembly, the residual norm's sums, the
+ * condensation, the BSR scatter -- as one HIP graph, captured on first use and
+ * again whenever its launch arguments change (physics, condensation choices,
+ * buffers), the default; 0 launches it kernel by kernel.  The mixed
+ * condensation and quasi-Newton storage always run kernel by kernel. */
+int hfem_set_linearization_graph(hfem_ctx *ctx, int use);
+
 #ifdef __cplusplus
 }
 #endif
