@@ -36,6 +36,7 @@ static hipError_t launch_dinv_t(int na, int e0, int n, const double *D, TO *Dinv
 {
     const int NT = (na + 15) / 16;
     if (NT < 1 || NT > 11 || variant < 0 || variant > 2) return hipErrorInvalidValue;
+    if (n <= 0) return hipSuccess;
     const int grid = std::min(n, 65536);
 #define HFEM_DINV(NTV, W, MW)                                                                                 \
     hipLaunchKernelGGL((dinv_mfma_kernel<T, TO, NTV, W, MW>), dim3(grid), dim3(64 * W),                         \

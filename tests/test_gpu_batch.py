@@ -127,6 +127,37 @@ def test_batch_against_oracle(oracle_mod):
         c.close()
 
 
+def test_batch_with_empty_and_partial_buckets():
+    """Buckets of 0 elements and buckets that take only the first elements of
+    their context, in every launch form: the taken elements get the per-context
+    results, the others keep their blocks."""
+    from hfem_amd.lib import HDGBatch, HDGContext, FP64
+    probs = [make_problem("tet", 2, 1, seed=3), make_problem("tet", 1, 2, seed=3), make_problem("tet", 1, 3, seed=3)]
+    ctxs = [HDGContext(pr) for pr in probs]
+    take = [0, 5, 3]
+    want = []
+    for c, k in zip(ctxs, take):
+        c.assemble_element_matrix_vector()
+        c.schur_element_matrix_vector(precision=FP64)
+        want.append(c.read_condensed())
+    batch = HDGBatch(ctxs, take)
+    for modes in ((0, 0), (1, 1)):
+        batch.set_launch(*modes)
+        # the untaken elements keep the per-context results; the taken ones are redone
+        batch.assemble_element_matrix_vector()
+        batch.schur_element_matrix_vector()
+        for c, k, w in zip(ctxs, take, want):
+            got = c.read_condensed()
+            for name in ("DinvRu", "DinvF", "AH", "FH", "info"):
+                assert np.array_equal(got[name], w[name]), (modes, k, name)
+        say(f"  launch forms {modes}, elements taken {take}: bit-identical")
+    waves, tasm, tinv = batch.tune()
+    say(f"  tuned waves {waves}")
+    batch.close()
+    for c in ctxs:
+        c.close()
+
+
 def test_batch_rejects_what_it_cannot_run():
     from hfem_amd.lib import HDGBatch, HDGContext, HfemError
     a = HDGContext(make_problem("tet", 1, 2, seed=1))
