@@ -140,31 +140,44 @@ def dirk_step_hex(ncell=24, dt=0.1):
     """Wall time of DIRK(3,3) steps (solveUnsteadyProblem's inner step: three
     Newton solves with GMRES(30) and BILU0 on the trace system) on 24^3 jittered
     periodic p=2 hexes from the Taylor-Green vortex (velocity amplitude 0.1) plus
-    seeded noise 1e-3.  The first step also computes the MDF ordering, which the
-    second reuses; both are reported."""
+    seeded noise 1e-3, every Newton iteration's linearisation replayed as a HIP
+    graph.  Two configurations from the same state: the plain FP64 path, and
+    FP32-MFMA condensation with FP64 refinement and FP32 BILU0 factors (FP64
+    residuals and Gram-Schmidt).  The first step also computes the MDF ordering,
+    which the second reuses; both are reported, with the relative difference of
+    the two configurations' final states."""
     from hfem_amd.problem import make_problem
-    from hfem_amd.lib import HDGContext
+    from hfem_amd.lib import HDGContext, FP64, MIXED
     prob = make_problem("hex", ncell, 2, seed=0, state="taylor_green", amp=1e-3)
     print(f"[bench] dirk: hex p=2 ne={prob.ne} nf={prob.mesh.nf}", file=sys.stderr, flush=True)
-    ctx = HDGContext(prob)
-    ctx.tune_condensation()
-    steps = []
-    for k in range(2):
-        ctx.sync()
-        t0 = time.perf_counter()
-        conv, nit = ctx.dirk_step(dt, 3, 3)
-        ctx.sync()
-        wall = time.perf_counter() - t0
-        st = ctx.solver_stats()
-        steps.append({"wall_s": wall, "converged": conv, "newton_iterations": nit,
-                      "last_solve": {k2: float(v) for k2, v in st.items()}})
-        print(f"[bench] dirk step {k}: {wall:.3f} s, {nit} Newton iterations, converged {conv}",
-              file=sys.stderr, flush=True)
-    _, nlev = ctx.get_ordering()
+    runs, finals, nlev = {}, {}, None
+    for name, prec, pp in (("fp64", FP64, 1), ("mixed", MIXED, 0)):
+        ctx = HDGContext(prob)
+        ctx.tune_condensation()
+        steps = []
+        for k in range(2):
+            ctx.sync()
+            t0 = time.perf_counter()
+            conv, nit = ctx.dirk_step(dt, 3, 3, precision=prec, prec_precision=pp)
+            ctx.sync()
+            wall = time.perf_counter() - t0
+            st = ctx.solver_stats()
+            steps.append({"wall_s": wall, "converged": conv, "newton_iterations": nit,
+                          "last_solve": {k2: float(v) for k2, v in st.items()}})
+            print(f"[bench] dirk {name} step {k}: {wall:.3f} s, {nit} Newton iterations, converged {conv}",
+                  file=sys.stderr, flush=True)
+        runs[name] = steps
+        finals[name] = ctx.get_solution()[0]
+        if nlev is None:
+            nlev = ctx.get_ordering()[1]
+        ctx.close()
+    rel = float(np.linalg.norm(finals["mixed"] - finals["fp64"]) / np.linalg.norm(finals["fp64"]))
     print(json.dumps({"mode": "dirk", "workload": f"3D NS HDG p=2 hexes, {ncell}^3={prob.ne} elements, periodic, "
                       "Taylor-Green 0.1 + noise 1e-3", "elements": prob.ne, "faces": prob.mesh.nf, "dt": dt,
-                      "stages": 3, "order": 3, "mdf_levels": [int(v) for v in nlev], "steps": steps}))
-    ctx.close()
+                      "stages": 3, "order": 3, "mdf_levels": [int(v) for v in nlev], "graphs": "linearisation "
+                      "and BILU0 application replayed as HIP graphs", "steps": runs["fp64"],
+                      "mixed_steps": runs["mixed"], "mixed": "FP32-MFMA condensation + FP64 refinement, FP32 BILU0 "
+                      "factors", "mixed_vs_fp64_final_state_rel": rel}))
 
 
 def ragged_batch(steps, warmup, total=20000, ncell=10, seed=0):

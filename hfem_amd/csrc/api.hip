@@ -97,7 +97,7 @@ extern "C" int hfem_create(hfem_ctx **out, const hfem_dims *hd, int device)
     ALLOC(rdmax, ne);
     ALLOC(rcmax, ne);
     ALLOC(rdprev, ne);
-    ALLOC(rcount, 1);
+    ALLOC(rcount, 17);   // [0]: the adaptive loop's count; [1 + s]: sweep s of a fixed-sweep graph
     if (assemble_spill_doubles(d) > 0) ALLOC(spill, (long)c->grid_asm * assemble_spill_doubles(d));
 #undef ALLOC
     if (e == hipSuccess) e = hipStreamCreate(&c->stream);
@@ -921,11 +921,46 @@ extern "C" int hfem_assemble_rhs(hfem_ctx *c, double *rNorm)
     return 0;
 }
 
+// The mixed condensation of all elements with a fixed number of refinement
+// sweeps and no host synchronisation: FP32 inverse, X0, `sweeps` sweeps (the
+// unconverged count after sweep s left in rcount[1 + s]), the FP64 Schur
+// product.  The caller reads the counts afterwards: it redoes the condensation
+// kernel by kernel if an element did not converge after the last sweep, and
+// runs fewer sweeps next time if an earlier one already sufficed.
+static int condense_mixed_fixed(hfem_ctx *c, int sweeps)
+{
+    const Dims &d = c->d;
+    const int n = c->hd.ne;
+    float *inv32 = (float *)c->dinv;
+    HIPCHK(launch_dinv32(d.na, 0, n, c->D, inv32, c->info, c->stream, c->inv_var[1]));
+    const bool fused = c->prod_var[2] == kFused;
+    if (!fused) HIPCHK(run_prod(c, 2, c->prod_var[2], 0, n, inv32, c->Fx, c->D, c->K, c->X, c->Hx, c->rwork));
+    for (int sweep = 1; sweep <= sweeps; sweep++) {
+        int *cnt = c->rcount + 1 + sweep - 1;
+        HIPCHK(launch_refine_reset(n, c->rdmax, c->rcmax, cnt, c->stream));
+        if (fused) {
+            HIPCHK(launch_xsolve(1, d.na, d.nb + 1, c->ldx, 0, n, c->D, inv32, c->Fx, c->X, 1, sweep > 1, c->rdmax,
+                                 c->rcmax, c->stream));
+        } else {
+            HIPCHK(run_prod(c, 3, c->prod_var[3], 0, n, inv32, c->Fx, c->D, c->K, c->X, c->Hx, c->rwork,
+                            c->gemm_sol[2]));
+            HIPCHK(run_prod(c, 4, c->prod_var[4], 0, n, inv32, c->Fx, c->D, c->K, c->X, c->Hx, c->rwork));
+        }
+        HIPCHK(launch_refine_count(n, c->rdmax, c->rcmax, c->rdprev, sweep == 1, c->mix_tol, cnt, c->stream));
+    }
+    HIPCHK(run_prod(c, 1, c->prod_var[1], 0, n, nullptr, c->Fx, c->D, c->K, c->X, c->Hx, nullptr, c->gemm_sol[1]));
+    c->last_sweeps = sweeps;
+    c->last_unconverged = 0;
+    c->last_fallback = 0;
+    return 0;
+}
+
 // The device work of one linearisation: element assembly, the residual norm's
 // two sums of squares (left on the device in partial[512] and partial[513]), the
-// condensation and the BSR scatter.  The FP64 path has no host synchronisation
-// in it, so it can be captured as a graph.
-static int linearize(hfem_ctx *c, int precision)
+// condensation and the BSR scatter.  The FP64 path, and the mixed path with a
+// fixed number of sweeps (mixed_sweeps > 0), have no host synchronisation in
+// them, so they can be captured as a graph.
+static int linearize(hfem_ctx *c, int precision, int mixed_sweeps = 0)
 {
     const int ne = c->hd.ne, nf = c->hd.nf;
     const Dims &d = c->d;
@@ -939,7 +974,11 @@ static int linearize(hfem_ctx *c, int precision)
     HIPCHK(launch_sumsq(256, c->partial, 0, c->partial + 512, 1, c->stream));
     HIPCHK(launch_sumsq((long)nf * bsz, c->r, 1, c->partial + 256, 256, c->stream));
     HIPCHK(launch_sumsq(256, c->partial + 256, 0, c->partial + 513, 1, c->stream));
-    if ((rc = hfem_condense(c, 0, ne, precision))) return rc;
+    if (precision == HFEM_MIXED && mixed_sweeps > 0) {
+        if ((rc = condense_mixed_fixed(c, mixed_sweeps))) return rc;
+    } else if ((rc = hfem_condense(c, 0, ne, precision))) {
+        return rc;
+    }
     if (c->qn_acc >= 0 && (rc = store_quasi_newton(c, precision))) return rc;
     const long es = (long)d.nb * (d.nb + 1);
     HIPCHK(launch_bsr_gather(c->nblocks, bsz, d.nch, d.npf, d.nb, es, c->blkptr, c->blklist, c->einv, c->Hx, c->Hg,
@@ -949,11 +988,21 @@ static int linearize(hfem_ctx *c, int precision)
     return 0;
 }
 
-static hfem_ctx::LinKey lin_key(const hfem_ctx *c)
+static hfem_ctx::LinKey lin_key(const hfem_ctx *c, int precision, int sweeps)
 {
     hfem_ctx::LinKey k;
     memset(&k, 0, sizeof k);
     k.ph = c->ph;
+    k.precision = precision;
+    k.sweeps = sweeps;
+    if (precision == HFEM_MIXED) {
+        k.inv_var1 = c->inv_var[1];
+        k.prod_var2 = c->prod_var[2];
+        k.prod_var3 = c->prod_var[3];
+        k.prod_var4 = c->prod_var[4];
+        k.gemm_sol2 = c->gemm_sol[2];
+        k.mix_tol = c->mix_tol;
+    }
     k.inv_var0 = c->inv_var[0];
     k.prod_var0 = c->prod_var[0];
     k.prod_var1 = c->prod_var[1];
@@ -973,9 +1022,11 @@ static void drop_lin_graph(hfem_ctx *c)
 }
 
 // No counterpart in the reference: whether hfem_assemble_linear_system replays the
-// FP64 linearisation as a captured graph (1, the default) or launches it kernel by
-// kernel (0).  Quasi-Newton storage and the mixed condensation always launch
-// kernel by kernel.
+// linearisation as a captured graph (1, the default) or launches it kernel by
+// kernel (0).  The mixed condensation is captured with the number of sweeps the
+// last kernel-by-kernel mixed condensation needed (the first one runs kernel by
+// kernel to learn it); a replay in which an element does not converge is redone
+// kernel by kernel.  Quasi-Newton storage always launches kernel by kernel.
 extern "C" int hfem_set_linearization_graph(hfem_ctx *c, int use)
 {
     if (!c) return hfem_fail(-1, "hfem_set_linearization_graph: null context");
@@ -993,14 +1044,18 @@ extern "C" int hfem_assemble_linear_system(hfem_ctx *c, int precision, double *r
         return hfem_fail(-1, "hfem_assemble_linear_system: unknown precision %d", precision);
     HIPCHK(hipSetDevice(c->device));
     int rc;
-    if (c->lin_graph && precision == HFEM_FP64 && c->qn_acc < 0) {
-        const hfem_ctx::LinKey k = lin_key(c);
+    const int sweeps = precision == HFEM_MIXED ? c->lin_mixed_sweeps : 0;
+    const bool graph = c->lin_graph && c->qn_acc < 0 && (precision == HFEM_FP64 || sweeps > 0);
+    if (graph) {
+        if (precision == HFEM_MIXED && !c->rwork && c->prod_var[2] != kFused)
+            HIPCHK(hfem_dalloc(&c->rwork, (size_t)c->hd.ne * c->d.na * c->ldx));
+        const hfem_ctx::LinKey k = lin_key(c, precision, sweeps);
         if (c->lin_exec && memcmp(&k, &c->lin_key, sizeof k) != 0) drop_lin_graph(c);
         if (!c->lin_exec) {
             hipGraph_t g = nullptr;
             HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
             c->capturing = true;
-            rc = linearize(c, precision);
+            rc = linearize(c, precision, sweeps);
             c->capturing = false;
             hipError_t e = hipStreamEndCapture(c->stream, &g);
             if (rc) {
@@ -1014,8 +1069,26 @@ extern "C" int hfem_assemble_linear_system(hfem_ctx *c, int precision, double *r
             c->lin_key = k;
         }
         HIPCHK(hipGraphLaunch(c->lin_exec, c->stream));
-    } else if ((rc = linearize(c, precision))) {
-        return rc;
+        if (precision == HFEM_MIXED) {
+            int cnt[16] = {0};
+            HIPCHK(hipMemcpyAsync(cnt, c->rcount + 1, sizeof(int) * sweeps, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            if (cnt[sweeps - 1] > 0) {
+                // an element needs more sweeps than the graph runs: redo kernel by kernel
+                // (the graph's Schur product has overwritten H), and learn the new count
+                if ((rc = linearize(c, precision))) return rc;
+                c->lin_mixed_sweeps = c->last_fallback ? 0 : c->last_sweeps;
+            } else {
+                // the first sweep after which every element had converged: the next
+                // capture runs that many
+                int s1 = sweeps;
+                while (s1 > 1 && cnt[s1 - 2] == 0) s1--;
+                c->lin_mixed_sweeps = s1;
+            }
+        }
+    } else {
+        if ((rc = linearize(c, precision))) return rc;
+        if (precision == HFEM_MIXED) c->lin_mixed_sweeps = c->last_fallback ? 0 : c->last_sweeps;
     }
     double s[2] = {0, 0};
     HIPCHK(hipMemcpyAsync(s, c->partial + 512, sizeof s, hipMemcpyDeviceToHost, c->stream));

@@ -234,8 +234,11 @@ struct hfem_ctx {
     // the launch arguments it was captured with (recaptured when they change)
     bool lin_graph = true, capturing = false;
     hipGraphExec_t lin_exec = nullptr;
+    int lin_mixed_sweeps = 0;   // sweeps the last kernel-by-kernel mixed condensation needed (0: none yet)
     struct LinKey {
         Phys ph;
+        int precision, sweeps, inv_var1, prod_var2, prod_var3, prod_var4, gemm_sol2;
+        double mix_tol;
         int inv_var0, prod_var0, prod_var1, gemm_sol0, gemm_sol1;
         const void *ptr[20];   // buffers the setters may (re)allocate
     } lin_key;
