@@ -192,7 +192,7 @@ def ragged_batch(steps, warmup, total=20000, ncell=10, seed=0):
     (hfem_batch_*: one persistent launch per kernel over all orders, its shapes
     picked per order by the batch tuner)."""
     from hfem_amd.problem import make_problem
-    from hfem_amd.lib import HDGBatch, HDGContext, FP64
+    from hfem_amd.lib import HDGBatch, HDGContext, FP64, MIXED
     rng = np.random.default_rng(seed)
     counts = np.bincount(rng.integers(1, 5, size=total), minlength=5)[1:]
     ctxs = []
@@ -204,26 +204,43 @@ def ragged_batch(steps, warmup, total=20000, ncell=10, seed=0):
         ctxs.append((p, int(cnt), ctx))
     print(f"[bench] ragged batch: counts by order {dict((p, c) for p, c, _ in ctxs)}", file=sys.stderr, flush=True)
 
-    def run(concurrent):
+    for _, _, ctx in ctxs:
+        ctx.set_refinement(1e-10, 4)   # the mixed path refines to the metric's bound
+
+    def run(concurrent, prec=FP64):
         for p, cnt, ctx in ctxs:
             ctx.assemble_element_matrix_vector(0, cnt)
-            ctx.schur_element_matrix_vector(0, cnt, FP64)
+            ctx.schur_element_matrix_vector(0, cnt, prec)
             if not concurrent:
                 ctx.sync()
         for _, _, ctx in ctxs:
             ctx.sync()
 
     res = {}
-    for name, conc in (("per_order", False), ("concurrent", True)):
+    for name, conc, prec in (("per_order", False, FP64), ("concurrent", True, FP64),
+                             ("per_order_mixed", False, MIXED)):
         for _ in range(warmup):
-            run(conc)
+            run(conc, prec)
         t0 = time.perf_counter()
         for _ in range(steps):
-            run(conc)
+            run(conc, prec)
         wall = (time.perf_counter() - t0) / steps
         res[name] = {"ms_per_batch": wall * 1e3, "elements_per_s": total / wall}
         print(f"[bench] ragged {name}: {wall * 1e3:.2f} ms per batch, {total / wall:.0f} elements/s",
               file=sys.stderr, flush=True)
+    # the mixed path's error against the FP64 path, the first 256 elements of each order
+    mixed_err = {}
+    for p, cnt, ctx in ctxs:
+        m = min(cnt, 256)
+        ctx.assemble_element_matrix_vector(0, cnt)
+        ctx.schur_element_matrix_vector(0, cnt, FP64)
+        ref = ctx.read_condensed(0, m)
+        ctx.assemble_element_matrix_vector(0, cnt)
+        ctx.schur_element_matrix_vector(0, cnt, MIXED)
+        got = ctx.read_condensed(0, m)
+        mixed_err[f"p{p}"] = max(float(np.max(np.linalg.norm((got[k] - ref[k]).reshape(m, -1), axis=1)
+                                              / np.linalg.norm(ref[k].reshape(m, -1), axis=1)))
+                                 for k in ("DinvF", "AH", "FH"))
     per_order_ms = {}
     for p, cnt, ctx in ctxs:
         ctx.assemble_element_matrix_vector(0, cnt)
@@ -236,27 +253,29 @@ def ragged_batch(steps, warmup, total=20000, ncell=10, seed=0):
     waves, tasm, tinv = batch.tune()
     print(f"[bench] ragged batch tuned: inverse waves per element {waves}", file=sys.stderr, flush=True)
 
-    def run_batch():
+    def run_batch(prec=FP64):
         batch.assemble_element_matrix_vector()
-        batch.schur_element_matrix_vector()
+        batch.schur_element_matrix_vector(prec)
         batch.sync()
 
     tuned_modes, tuned_ms = batch.launch()
 
-    def time_batch(modes):
+    def time_batch(modes, prec=FP64):
         batch.set_launch(*modes)
         for _ in range(warmup):
-            run_batch()
+            run_batch(prec)
         t0 = time.perf_counter()
         for _ in range(steps):
-            run_batch()
+            run_batch(prec)
         wall = (time.perf_counter() - t0) / steps
         ba, bc = batch.kernel_ms()
-        print(f"[bench] ragged batch, launch forms {modes}: {wall * 1e3:.2f} ms per batch, {total / wall:.0f} "
-              f"elements/s (assembly {ba:.2f} ms, condensation {bc:.2f} ms)", file=sys.stderr, flush=True)
+        print(f"[bench] ragged batch, launch forms {modes}, {['fp64', 'mixed'][prec]}: {wall * 1e3:.2f} ms per "
+              f"batch, {total / wall:.0f} elements/s (assembly {ba:.2f} ms, condensation {bc:.2f} ms)",
+              file=sys.stderr, flush=True)
         return {"ms_per_batch": wall * 1e3, "elements_per_s": total / wall, "assemble_ms": ba, "condense_ms": bc}
 
     res["batch_one_launch"] = time_batch((0, 0))
+    res["batch_mixed"] = time_batch(tuned_modes, MIXED)
     res["batch"] = time_batch(tuned_modes)
     res["batch"]["launch_forms"] = {"assembly": ["one launch", "per order"][tuned_modes[0]],
                                     "inverse": ["one launch", "per order"][tuned_modes[1]],
@@ -275,6 +294,8 @@ def ragged_batch(steps, warmup, total=20000, ncell=10, seed=0):
                       "speedup_batch_vs_per_order": res["per_order"]["ms_per_batch"] / res["batch"]["ms_per_batch"],
                       "speedup_batch_one_launch_vs_per_order": res["per_order"]["ms_per_batch"]
                       / res["batch_one_launch"]["ms_per_batch"],
+                      "mixed_speedup_vs_fp64_batch": res["batch"]["ms_per_batch"] / res["batch_mixed"]["ms_per_batch"],
+                      "mixed_max_rel_err_vs_fp64": mixed_err, "mixed_tol": 1e-10,
                       "per_order": per_order_ms}))
     for _, _, ctx in ctxs:
         ctx.close()

@@ -423,13 +423,30 @@ extern "C" int hfem_batch_assemble(hfem_batch *b)
     return batch_join_out(b);
 }
 
-extern "C" int hfem_batch_condense(hfem_batch *b)
+extern "C" int hfem_batch_condense(hfem_batch *b, int precision)
 {
     int rc = batch_check_ready(b, "hfem_batch_condense");
     if (rc) return rc;
+    if (precision != HFEM_FP64 && precision != HFEM_MIXED)
+        return hfem_fail(-1, "hfem_batch_condense: unknown precision %d", precision);
     HIPCHK(hipSetDevice(b->device));
     if ((rc = batch_join_in(b))) return rc;
     HIPCHK(hipEventRecord(b->ev[2], b->stream));
+    if (precision == HFEM_MIXED) {
+        // each bucket's mixed condensation (FP32 inverse, adaptive refinement with
+        // its convergence read-backs, FP64 Schur product) on its own stream
+        HIPCHK(hipEventRecord(b->join, b->stream));
+        for (int k : b->order) {
+            hfem_ctx *c = b->ctx[k];
+            if (b->nelem[k] == 0) continue;
+            HIPCHK(hipStreamWaitEvent(c->stream, b->join, 0));
+            if ((rc = hfem_condense(c, 0, b->nelem[k], HFEM_MIXED))) return rc;
+            HIPCHK(hipEventRecord(b->done[k], c->stream));
+            HIPCHK(hipStreamWaitEvent(b->stream, b->done[k], 0));
+        }
+        HIPCHK(hipEventRecord(b->ev[3], b->stream));
+        return batch_join_out(b);
+    }
     HIPCHK(batch_launch_inv(b, b->order, b->mode_inv));
     HIPCHK(hipEventRecord(b->join, b->stream));
     // the products of each context on its own stream (its tuned shapes), joined

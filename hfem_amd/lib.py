@@ -86,7 +86,7 @@ EXPORTS = {
     "hfem_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _ip, ctypes.POINTER(_vp)]),
     "hfem_batch_destroy": (ctypes.c_int, [_vp]),
     "hfem_batch_assemble": (ctypes.c_int, [_vp]),
-    "hfem_batch_condense": (ctypes.c_int, [_vp]),
+    "hfem_batch_condense": (ctypes.c_int, [_vp, ctypes.c_int]),
     "hfem_batch_tune": (ctypes.c_int, [_vp, _ip, ctypes.POINTER(ctypes.c_float)]),
     "hfem_batch_set_inverse_waves": (ctypes.c_int, [_vp, _ip]),
     "hfem_batch_set_launch": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
@@ -437,8 +437,8 @@ class HDGBatch:
     def assemble_element_matrix_vector(self):
         _check(load().hfem_batch_assemble(self.h))
 
-    def schur_element_matrix_vector(self):
-        _check(load().hfem_batch_condense(self.h))
+    def schur_element_matrix_vector(self, precision=FP64):
+        _check(load().hfem_batch_condense(self.h, precision))
 
     def tune(self):
         """Per bucket: the inverse's waves per element and the bucket order; returns

@@ -132,10 +132,12 @@ int hfem_batch_destroy(hfem_batch *batch);
 /* assembleElementMatrixVector (assembly/elementassembly.cpp:1481-1499) over every
  * element of every context: one flux launch and one assembly launch. */
 int hfem_batch_assemble(hfem_batch *batch);
-/* schurElementMatrixVector (assembly/elementschur.cpp:331-343) in FP64 over every
- * element: one inverse launch, then each context's two products with its own
- * tuned shapes on its own stream. */
-int hfem_batch_condense(hfem_batch *batch);
+/* schurElementMatrixVector (assembly/elementschur.cpp:331-343) over every element.
+ * HFEM_FP64: one inverse launch, then each context's two products with its own
+ * tuned shapes on its own stream.  HFEM_MIXED: each context's mixed condensation
+ * (hfem_condense(.., HFEM_MIXED): FP32-MFMA inverse, FP64 refinement to its
+ * tolerance, FP64 Schur product) on its own stream. */
+int hfem_batch_condense(hfem_batch *batch, int precision);
 /* Per bucket (context): runs hfem_tune_condensation on the context, times its
  * assembly and the inverse with 8, 4 or 2 waves per element (those its tile
  * count allows) in the batch launch, keeps the fastest and orders the buckets by

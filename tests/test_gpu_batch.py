@@ -7,6 +7,8 @@ persistent launches.
   waves per element change which wave holds a tile, not the arithmetic), with
   the buckets' element counts chosen so that the inverse's groups of 2 and 4
   elements per workgroup end on partial rounds.
+* The mixed condensation through the batch equals each context's own mixed
+  condensation bit for bit (and the FP64 one within the mixed bound).
 * Directly against the oracle on a batch of the per-context parity test's
   small tets of orders 1-3 (its seeds and tolerances).
 * After the batch tuner has picked the waves per element, the bucket order and
@@ -122,6 +124,38 @@ def test_batch_against_oracle(oracle_mod):
         assert relerr(c["AH"], b["H"]) < RTOL_MAT
         assert relerr(c["FH"], b["Rh"]) < RTOL_MAT
         assert relerr_res(c["DinvRu"], b["Ru"], fl_y) < RTOL_RES
+    batch.close()
+    for c in ctxs:
+        c.close()
+
+
+def test_batch_mixed_equals_per_context_mixed():
+    """The mixed condensation through the batch (each bucket's FP32-MFMA inverse
+    and FP64 refinement on its own stream) equals each context's own mixed
+    condensation bit for bit, and the FP64 condensation within the mixed bound."""
+    from hfem_amd.lib import HDGBatch, HDGContext, FP64, MIXED
+    from test_gpu_parity import RTOL_MIXED, relerr
+    probs = [make_problem("tet", 2, 1, seed=2), make_problem("tet", 2, 2, seed=2), make_problem("tet", 1, 3, seed=2),
+             make_problem("tet", 1, 4, seed=2)]
+    ctxs = [HDGContext(pr) for pr in probs]
+    want, fp64 = [], []
+    for c in ctxs:
+        c.assemble_element_matrix_vector()
+        c.schur_element_matrix_vector(precision=FP64)
+        fp64.append(c.read_condensed())
+        c.assemble_element_matrix_vector()
+        c.schur_element_matrix_vector(precision=MIXED)
+        want.append(c.read_condensed())
+    batch = HDGBatch(ctxs)
+    batch.assemble_element_matrix_vector()
+    batch.schur_element_matrix_vector(MIXED)
+    for c, w, f, pr in zip(ctxs, want, fp64, probs):
+        got = c.read_condensed()
+        for name in ("DinvRu", "DinvF", "AH", "FH", "info"):
+            assert np.array_equal(got[name], w[name]), name
+        e = max(relerr(got[k], f[k]) for k in ("DinvF", "AH", "FH"))
+        say(f"  mixed batch, ne={pr.ne} na={pr.dims['na']}: bit-identical to the context's own; vs FP64 {e:.1e}")
+        assert e < RTOL_MIXED
     batch.close()
     for c in ctxs:
         c.close()
