@@ -180,6 +180,45 @@ def dirk_step_hex(ncell=24, dt=0.1):
                       "factors", "mixed_vs_fp64_final_state_rel": rel}))
 
 
+def tri_p4(steps, warmup, ncell=128, mixed_tol=1e-10):
+    """2D NS HDG on p=4 triangles, ncell^2 x 2 jittered cells (32 768 elements at
+    128), assembly + condensation: the FP64 path against FP32-MFMA factorisation
+    with FP64 refinement to mixed_tol, with the mixed path's largest error per
+    element block against the FP64 path over every element."""
+    from hfem_amd.problem import make_problem
+    from hfem_amd.lib import HDGContext, FP64, MIXED
+    prob = make_problem("tri", ncell, 4, seed=0)
+    d = prob.dims
+    ne, na, nb = prob.ne, d["na"], d["nb"]
+    print(f"[bench] tri p=4 ne={ne} na={na} nb={nb}", file=sys.stderr, flush=True)
+    ctx = HDGContext(prob)
+    ctx.tune_condensation()
+    ctx.set_refinement(mixed_tol, 4)
+    res = {}
+    for name, prec in (("fp64", FP64), ("mixed", MIXED)):
+        wall, a_ms, c_ms = run_steps(ctx, ne, prec, steps, warmup)
+        res[name] = {"elements_per_s": ne * steps / wall, "ms_per_step": wall / steps * 1e3, "assemble_ms": a_ms,
+                     "condense_ms": c_ms}
+        if name == "mixed":
+            sw, left, fb = ctx.condense_report()
+            res[name]["refinement"] = {"tol": mixed_tol, "sweeps": sw, "unconverged_elements": left,
+                                       "fp64_fallback": bool(fb)}
+        print(f"[bench] tri p4 {name}: {res[name]['elements_per_s']:.0f} elements/s, assemble {a_ms:.3f} ms, "
+              f"condense {c_ms:.3f} ms", file=sys.stderr, flush=True)
+    outs = {}
+    for name, prec in (("fp64", FP64), ("mixed", MIXED)):
+        ctx.assemble_element_matrix_vector(0, ne)
+        ctx.schur_element_matrix_vector(0, ne, prec)
+        outs[name] = ctx.read_condensed()
+    errs = {k: relerr(outs["mixed"][k].reshape(ne, -1), outs["fp64"][k].reshape(ne, -1))
+            for k in ("DinvRu", "DinvF", "AH", "FH")}
+    ctx.close()
+    print(json.dumps({"mode": "tri_p4", "workload": f"2D NS HDG p=4 triangles, {ncell}^2x2={ne} elements, "
+                      "assemble+condense", "elements": ne, "na": na, "nb": nb, "steps": steps, **res,
+                      "mixed_speedup_vs_fp64": res["mixed"]["elements_per_s"] / res["fp64"]["elements_per_s"],
+                      "mixed_max_rel_err_vs_fp64_per_block": errs, "error_sample": "every element"}))
+
+
 def ragged_batch(steps, warmup, total=20000, ncell=10, seed=0):
     """Assembly + FP64 condensation of an element batch of p=1..4 tets, the order of
     each of `total` elements drawn uniformly by the seed.  One context per order
@@ -374,11 +413,11 @@ def main():
     ap.add_argument("--mixed-tol", type=float, default=1e-10,
                     help="refinement tolerance of the mixed path (the metric's bound)")
     ap.add_argument("--no-solver", action="store_true", help="skip the trace-system solve")
-    ap.add_argument("--mode", choices=["bench", "copy", "newton", "dirk", "ragged"], default="bench",
+    ap.add_argument("--mode", choices=["bench", "copy", "newton", "dirk", "ragged", "tri_p4"], default="bench",
                     help="copy: a plain 2 GiB device copy for the HBM counter calibration; newton: one Newton "
-                         "step's trace solve on 512 p=2 triangles; dirk: one DIRK(3,3) step on 24^3 periodic "
-                         "p=2 hexes; ragged: 20000 tets of orders 1-4 drawn by the seed (these three print "
-                         "their own line, not the element-path metric)")
+                         "step's trace solve on 512 p=2 triangles; dirk: DIRK(3,3) steps on 24^3 periodic "
+                         "p=2 hexes; ragged: 20000 tets of orders 1-4 drawn by the seed; tri_p4: 32768 p=4 "
+                         "triangles, FP64 vs mixed (these print their own line, not the element-path metric)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -396,6 +435,9 @@ def main():
         return
     if args.mode == "ragged":
         ragged_batch(args.steps, args.warmup)
+        return
+    if args.mode == "tri_p4":
+        tri_p4(args.steps, args.warmup)
         return
 
     from hfem_amd.problem import make_problem
