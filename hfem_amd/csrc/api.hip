@@ -193,6 +193,10 @@ extern "C" int hfem_set_mesh(hfem_ctx *c, const double *dgnodes, const int *elco
     const Dims &d = c->d;
     const int ne = c->hd.ne, nf = c->hd.nf;
     if (rowstart[0] != 0 || rowstart[nf] != nblocks) return hfem_fail(-1, "hfem_set_mesh: rowstart inconsistent with nblocks");
+    // the trace-system buffers are sized by the first mesh's block count
+    if (c->has_mesh && nblocks != c->nblocks)
+        return hfem_fail(-1, "hfem_set_mesh: the trace graph has %d blocks, this context's has %d (create a new "
+                             "context for a mesh of another size)", nblocks, c->nblocks);
     // einv, face-to-element lists and the gather lists of every BSR block
     std::vector<int> einv((size_t)ne * d.ndf, -1), f2e((size_t)nf * 4, -1);
     std::vector<int> cnt(nblocks + 1, 0);
@@ -243,6 +247,10 @@ extern "C" int hfem_set_mesh(hfem_ctx *c, const double *dgnodes, const int *elco
     c->nblocks = nblocks;
     c->sv.rowpts.assign(rowstart, rowstart + nf + 1);
     c->sv.colind.assign(cols, cols + nblocks);
+    // a new trace graph invalidates the ordering and the preconditioner built on the old one
+    c->sv.order_method = -1;
+    c->sv.prec = -2;
+    c->sv.graph_dirty = true;
     for (int i = 0; i < nf; i++)
         if (rowstart[i + 1] <= rowstart[i] || cols[rowstart[i]] != i)
             return hfem_fail(-1, "hfem_set_mesh: row %d of the trace graph must start with its diagonal block", i);
