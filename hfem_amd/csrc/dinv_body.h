@@ -56,6 +56,15 @@ template <> struct GJ<double> {
         const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
         return __hiloint2double(hi, lo);
     }
+    // 1/x for a normal nonzero pivot: the hardware reciprocal refined by two
+    // Newton-Raphson steps r <- r + r (1 - x r) (each squares the relative error),
+    // three dependent FMAs shorter than an IEEE division on the pivot chain
+    static __device__ __forceinline__ double recip(double x)
+    {
+        double r = __builtin_amdgcn_rcp(x);
+        r = fma(r, fma(-x, r, 1.0), r);
+        return fma(r, fma(-x, r, 1.0), r);
+    }
 };
 template <> struct GJ<float> {
     typedef float v4 __attribute__((ext_vector_type(4)));
@@ -85,6 +94,11 @@ template <> struct GJ<float> {
     static __device__ __forceinline__ float bcast(float v, int lane)
     {
         return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+    }
+    static __device__ __forceinline__ float recip(float x)
+    {
+        const float r = __builtin_amdgcn_rcpf(x);
+        return fmaf(r, fmaf(-x, r, 1.0f), r);
     }
 };
 
@@ -239,7 +253,7 @@ __device__ __forceinline__ void dinv_body(int na, int e0, int n, const double *_
                         }
                         const T pv = prow[jj];
                         if (pv == (T)0 && !sing) sing = j + 1;
-                        const T pinv = pv == (T)0 ? (T)0 : (T)1 / pv;
+                        const T pinv = pv == (T)0 ? (T)0 : G::recip(pv);
                         // scaled pivot row s_k = a(p, k) / a(p, j); every row takes one FMA per
                         // column, then the pivot row (one lane) is overwritten
                         T sk[16];
