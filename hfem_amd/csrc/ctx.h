@@ -14,6 +14,12 @@
 #include "hfem_device.h"
 
 namespace hfem {
+// after a BILU0 factorisation: upper blocks U_rl -> inv(U_rr) U_rl, the form the
+// backward sweeps read
+hipError_t launch_bilu0_scale_upper(int bsz, int n, const int *rowpts, const int *colind, const int *u2o,
+                                    double *Mg, hipStream_t s);
+hipError_t launch_bilu0_scale_upper(int bsz, int n, const int *rowpts, const int *colind, const int *u2o,
+                                    float *Mg, hipStream_t s);
 hipError_t launch_assemble(const Dims &, const Master &, const Phys &, const double *, const double *, const double *,
                            const double *, const int *, const int *, int, int, int, int, double *, long,
                            int, double *, double *, double *, double *, double *, double *, double *, double *,
@@ -248,6 +254,7 @@ struct hfem_ctx {
 int hfem_reduce_sum(hfem_ctx *c, long n, const double *x, int square, double *out);
 // X = inv(D)[F Ru] and the Schur product of [e0, e0+n) from the inverse in c->dinv
 int condense_products_fp64(hfem_ctx *c, int e0, int n);
+
 
 template <class T>
 static inline hipError_t hfem_dalloc(T **p, size_t n)

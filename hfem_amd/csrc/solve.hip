@@ -295,6 +295,11 @@ extern "C" int hfem_precond_factor(hfem_ctx *c, int kind)
                                                      s.lptr[l + 1] - s.lptr[l], s.Mg32, s.d_info, c->stream));
         }
     }
+    if (kind != 2) {
+        // the sweeps' form of the upper blocks
+        if (!fp32) HIPCHK(launch_bilu0_scale_upper(bsz, n, c->rowstart, c->cols, s.d_u2o, s.Mg, c->stream));
+        else HIPCHK(launch_bilu0_scale_upper(bsz, n, c->rowstart, c->cols, s.d_u2o, s.Mg32, c->stream));
+    }
     std::vector<int> info(n);
     HIPCHK(hipMemcpyAsync(info.data(), s.d_info, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));

@@ -395,6 +395,50 @@ __device__ __forceinline__ void wave_block_matvec(int bsz, const TM *__restrict_
 // 256-thread workgroup per row: the four waves take the neighbour blocks in turn
 // (lane = block row, the neighbour vector broadcast from registers), their partial
 // sums meet in LDS.
+// The backward sweep's inv(U_rr) b_r, this wave's share: wave w takes the columns
+// cc = w, w + NW, ... of inv(U_rr) (preloaded in dv for blocks up to 64 wide, else
+// read from the block in chunks of 8 columns), b_r staged in LDS (tb); rows lane
+// (a0) and lane + 64 (a1).  Shared by both sweep kernels so that they compute the
+// same bits.
+template <bool SMALL, int ND, int NW, typename TM>
+__device__ __forceinline__ void inv_diag_share(int bsz, int wv, int lane, int r0, int r1, const TM *Ui,
+                                               const double (&dv)[ND], const double *tb, double &a0, double &a1)
+{
+    a0 = 0.0;
+    a1 = 0.0;
+    if (SMALL) {
+#pragma unroll
+        for (int j = 0; j < ND; j++) {
+            const int cc = wv + NW * j;
+            a0 += dv[j] * (cc < bsz ? tb[min(cc, bsz - 1)] : 0.0);
+        }
+    } else {
+        // (loads of 8 columns issued together, addresses clamped into the block)
+        for (int c0 = wv; c0 < bsz; c0 += 8 * NW) {
+            double v0[8], v1[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const long col = (long)bsz * min(c0 + j * NW, bsz - 1);
+                v0[j] = Ui[r0 + col];
+                v1[j] = Ui[r1 + col];
+            }
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int cc = c0 + j * NW;
+                const double tc = cc < bsz ? tb[min(cc, bsz - 1)] : 0.0;
+                a0 += v0[j] * tc;
+                a1 += v1[j] * tc;
+            }
+        }
+    }
+    if (lane >= bsz) a0 = 0.0;
+    if (lane + 64 >= bsz) a1 = 0.0;
+}
+
+// Backward sweeps read upper blocks scaled by the factorisation's last step
+// (bilu0_scale_upper_kernel): U~_rl = inv(U_rr) U_rl, so that
+//   z_r = inv(U_rr) b_r - sum_l U~_rl z_l
+// and inv(U_rr) b_r is formed before the neighbours' z are needed.
 template <bool BACKWARD, bool SMALL, typename TM>
 __global__ void __launch_bounds__(SV_THREADS)
 bilu0_sweep_kernel(int bsz, const int *__restrict__ rowpts, const int *__restrict__ colind,
@@ -421,6 +465,7 @@ bilu0_sweep_kernel(int bsz, const int *__restrict__ rowpts, const int *__restric
             stake[tid] = BACKWARD ? (u2o[c] > o) : (u2o[c] < o);
         }
         const double bv = tid < bsz ? b[(long)r * bsz + tid] : 0.0;
+        if (BACKWARD && tid < bsz) tb[tid] = bv;
         constexpr int ND = (SMALL && BACKWARD) ? 64 / NW : 1;
         double dv[ND];
         if (SMALL && BACKWARD) {
@@ -428,6 +473,8 @@ bilu0_sweep_kernel(int bsz, const int *__restrict__ rowpts, const int *__restric
             for (int j = 0; j < ND; j++) dv[j] = Ui[r0 + (long)bsz * min(wv + NW * j, bsz - 1)];
         }
         __syncthreads();
+        double p0 = 0.0, p1 = 0.0;
+        if (BACKWARD) inv_diag_share<SMALL, ND, NW>(bsz, wv, lane, r0, r1, Ui, dv, tb, p0, p1);
         double acc0 = 0.0, acc1 = 0.0;
         int sel = 0;
         for (int q = 1; q < len; q++) {
@@ -439,57 +486,14 @@ bilu0_sweep_kernel(int bsz, const int *__restrict__ rowpts, const int *__restric
             if (SMALL) wave_block_matvec_ch<64, false>(bsz, Mg + (long)(k0 + q) * b2, z0, z1, acc0, acc1);
             else wave_block_matvec_ch<16, true>(bsz, Mg + (long)(k0 + q) * b2, z0, z1, acc0, acc1);
         }
-        red[wv][lane] = acc0;
-        red[wv][lane + 64] = acc1;
+        red[wv][lane] = BACKWARD ? p0 - acc0 : acc0;
+        red[wv][lane + 64] = BACKWARD ? p1 - acc1 : acc1;
         __syncthreads();
         if (tid < bsz) {
             double sum = 0.0;
 #pragma unroll
             for (int w = 0; w < NW; w++) sum += red[w][tid];
-            const double v = bv - sum;
-            if (BACKWARD) tb[tid] = v;
-            else z[(long)r * bsz + tid] = v;
-        }
-        if (BACKWARD) {
-            __syncthreads();
-            // z_r = inv(U_rr) t: wave w takes the columns cc = w, w + NW, ...
-            double a0 = 0.0, a1 = 0.0;
-            if (SMALL) {
-#pragma unroll
-                for (int j = 0; j < ND; j++) {
-                    const int cc = wv + NW * j;
-                    a0 += dv[j] * (cc < bsz ? tb[min(cc, bsz - 1)] : 0.0);
-                }
-            } else {
-                // (loads of 8 columns issued together, addresses clamped into the block)
-                for (int c0 = wv; c0 < bsz; c0 += 8 * NW) {
-                    double v0[8], v1[8];
-#pragma unroll
-                    for (int j = 0; j < 8; j++) {
-                        const long col = (long)bsz * min(c0 + j * NW, bsz - 1);
-                        v0[j] = Ui[r0 + col];
-                        v1[j] = Ui[r1 + col];
-                    }
-#pragma unroll
-                    for (int j = 0; j < 8; j++) {
-                        const int cc = c0 + j * NW;
-                        const double tc = cc < bsz ? tb[min(cc, bsz - 1)] : 0.0;
-                        a0 += v0[j] * tc;
-                        a1 += v1[j] * tc;
-                    }
-                }
-            }
-            if (lane >= bsz) a0 = 0.0;
-            if (lane + 64 >= bsz) a1 = 0.0;
-            red[wv][lane] = a0;
-            red[wv][lane + 64] = a1;
-            __syncthreads();
-            if (tid < bsz) {
-                double sum = 0.0;
-#pragma unroll
-                for (int w = 0; w < NW; w++) sum += red[w][tid];
-                z[(long)r * bsz + tid] = sum;
-            }
+            z[(long)r * bsz + tid] = BACKWARD ? sum : bv - sum;
         }
         __syncthreads();
     }
@@ -568,6 +572,7 @@ bilu0_sweep_syncfree_kernel(int bsz, const int *__restrict__ rowpts, const int *
         // stores the row (so that its own wait covers every store of the row)
         const double bv0 = wv == 0 && lane < bsz ? b[(long)r * bsz + lane] : 0.0;
         const double bv1 = wv == 0 && lane + 64 < bsz ? b[(long)r * bsz + lane + 64] : 0.0;
+        if (BACKWARD && tid < bsz) tb[tid] = b[(long)r * bsz + tid];
         constexpr int ND = (SMALL && BACKWARD) ? 64 / NW : 1;
         double dv[ND];
         if (SMALL && BACKWARD) {
@@ -575,6 +580,9 @@ bilu0_sweep_syncfree_kernel(int bsz, const int *__restrict__ rowpts, const int *
             for (int j = 0; j < ND; j++) dv[j] = Ui[r0 + (long)bsz * min(wv + NW * j, bsz - 1)];
         }
         __syncthreads();
+        // backward: inv(U_rr) b_r before any wait on a neighbour
+        double p0 = 0.0, p1 = 0.0;
+        if (BACKWARD) inv_diag_share<SMALL, ND, NW>(bsz, wv, lane, r0, r1, Ui, dv, tb, p0, p1);
         double acc0 = 0.0, acc1 = 0.0;
         int sel = 0;
         for (int q = 1; q < len; q++) {
@@ -608,8 +616,8 @@ bilu0_sweep_syncfree_kernel(int bsz, const int *__restrict__ rowpts, const int *
                 wave_block_matvec_ch<16, true>(bsz, B, z0, z1, acc0, acc1);
             }
         }
-        red[wv][lane] = acc0;
-        red[wv][lane + 64] = acc1;
+        red[wv][lane] = BACKWARD ? p0 - acc0 : acc0;
+        red[wv][lane + 64] = BACKWARD ? p1 - acc1 : acc1;
         __syncthreads();
         if (wv == 0) {
 #pragma unroll
@@ -619,54 +627,8 @@ bilu0_sweep_syncfree_kernel(int bsz, const int *__restrict__ rowpts, const int *
                     double sum = 0.0;
 #pragma unroll
                     for (int w = 0; w < NW; w++) sum += red[w][i];
-                    const double v = (h ? bv1 : bv0) - sum;
-                    if (BACKWARD) tb[i] = v;
-                    else __hip_atomic_store(z + (long)r * bsz + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                }
-            }
-        }
-        if (BACKWARD) {
-            __syncthreads();
-            double a0 = 0.0, a1 = 0.0;
-            if (SMALL) {
-#pragma unroll
-                for (int j = 0; j < ND; j++) {
-                    const int cc = wv + NW * j;
-                    a0 += dv[j] * (cc < bsz ? tb[min(cc, bsz - 1)] : 0.0);
-                }
-            } else {
-                for (int c0 = wv; c0 < bsz; c0 += 8 * NW) {
-                    double v0[8], v1[8];
-#pragma unroll
-                    for (int j = 0; j < 8; j++) {
-                        const long col = (long)bsz * min(c0 + j * NW, bsz - 1);
-                        v0[j] = Ui[r0 + col];
-                        v1[j] = Ui[r1 + col];
-                    }
-#pragma unroll
-                    for (int j = 0; j < 8; j++) {
-                        const int cc = c0 + j * NW;
-                        const double tc = cc < bsz ? tb[min(cc, bsz - 1)] : 0.0;
-                        a0 += v0[j] * tc;
-                        a1 += v1[j] * tc;
-                    }
-                }
-            }
-            if (lane >= bsz) a0 = 0.0;
-            if (lane + 64 >= bsz) a1 = 0.0;
-            red[wv][lane] = a0;
-            red[wv][lane + 64] = a1;
-            __syncthreads();
-            if (wv == 0) {
-#pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    const int i = lane + 64 * h;
-                    if (i < bsz) {
-                        double sum = 0.0;
-#pragma unroll
-                        for (int w = 0; w < NW; w++) sum += red[w][i];
-                        __hip_atomic_store(z + (long)r * bsz + i, sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    }
+                    const double v = BACKWARD ? sum : (h ? bv1 : bv0) - sum;
+                    __hip_atomic_store(z + (long)r * bsz + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
             }
         }
@@ -681,6 +643,48 @@ bilu0_sweep_syncfree_kernel(int bsz, const int *__restrict__ rowpts, const int *
 
 // A new epoch for the flags of the next application and fresh tickets for its two
 // sweeps (a kernel, not a memset, so that the application can be captured in a graph).
+// The factorisation's last step for the sweeps: every upper block of row r becomes
+// U~_rl = inv(U_rr) U_rl (inv(U_rr) is the row's diagonal block), in place, after
+// the whole factorisation (the other rows' eliminations read the unscaled U_rl).
+// One workgroup per row: inv(U_rr) in LDS (FP64), the upper blocks a few columns
+// at a time, each thread one entry of inv(U_rr) times a staged column.
+template <typename TM>
+__global__ void __launch_bounds__(SV_THREADS)
+bilu0_scale_upper_kernel(int bsz, int n, const int *__restrict__ rowpts, const int *__restrict__ colind,
+                         const int *__restrict__ u2o, TM *__restrict__ Mg)
+{
+    extern __shared__ double su[];
+    const long b2 = (long)bsz * bsz;
+    const int tid = threadIdx.x, cpp = max(1, SV_THREADS / bsz);
+    double *Dl = su, *Cl = su + b2;
+    for (int r = blockIdx.x; r < n; r += gridDim.x) {
+        // (row bounds and the upper test through readfirstlane: loop control with
+        // barriers inside must be uniform)
+        const int k0 = __builtin_amdgcn_readfirstlane(rowpts[r]);
+        const int len = __builtin_amdgcn_readfirstlane(rowpts[r + 1]) - k0;
+        const int o = __builtin_amdgcn_readfirstlane(u2o[r]);
+        const TM *Ui = Mg + (long)k0 * b2;
+        for (long i = tid; i < b2; i += SV_THREADS) Dl[i] = (double)Ui[i];
+        __syncthreads();
+        for (int q = 1; q < len; q++) {
+            if (__builtin_amdgcn_readfirstlane(u2o[colind[k0 + q]]) <= o) continue;
+            TM *B = Mg + (long)(k0 + q) * b2;
+            for (int c0 = 0; c0 < bsz; c0 += cpp) {
+                const int nc = min(cpp, bsz - c0);
+                for (int i = tid; i < nc * bsz; i += SV_THREADS) Cl[i] = (double)B[(long)c0 * bsz + i];
+                __syncthreads();
+                if (tid < nc * bsz) {
+                    const int c = tid / bsz, i = tid - c * bsz;
+                    double acc = 0.0;
+                    for (int k = 0; k < bsz; k++) acc += Dl[i + (long)bsz * k] * Cl[c * bsz + k];
+                    B[(long)(c0 + c) * bsz + i] = (TM)acc;
+                }
+                __syncthreads();
+            }
+        }
+    }
+}
+
 __global__ void sweep_reset_kernel(unsigned *sync)
 {
     if (threadIdx.x == 0) {
@@ -1051,6 +1055,29 @@ hipError_t launch_bilu0_factor_level(int bsz, const int *rowpts, const int *coli
                                      int nlev, float *Mg, int *info, hipStream_t s)
 {
     return factor_level(bsz, rowpts, colind, u2o, rows, nlev, Mg, info, s);
+}
+
+template <typename TM>
+static hipError_t scale_upper(int bsz, int n, const int *rowpts, const int *colind, const int *u2o, TM *Mg,
+                              hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const size_t lds = sizeof(double) * ((size_t)bsz * bsz + (size_t)std::max(1, SV_THREADS / bsz) * bsz);
+    hipError_t e = set_lds((const void *)bilu0_scale_upper_kernel<TM>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(bilu0_scale_upper_kernel<TM>, dim3(std::min(n, 16384)), dim3(SV_THREADS), lds, s, bsz, n,
+                       rowpts, colind, u2o, Mg);
+    return hipGetLastError();
+}
+hipError_t launch_bilu0_scale_upper(int bsz, int n, const int *rowpts, const int *colind, const int *u2o,
+                                    double *Mg, hipStream_t s)
+{
+    return scale_upper(bsz, n, rowpts, colind, u2o, Mg, s);
+}
+hipError_t launch_bilu0_scale_upper(int bsz, int n, const int *rowpts, const int *colind, const int *u2o,
+                                    float *Mg, hipStream_t s)
+{
+    return scale_upper(bsz, n, rowpts, colind, u2o, Mg, s);
 }
 
 template <bool BACKWARD, typename TM>
