@@ -194,23 +194,46 @@ __device__ __forceinline__ void lds_block_product(const T *A, const T *B, int n,
 {
     typedef BlkMfma<T> M;
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nw = blockDim.x >> 6;
-    const int lr = lane & 15, lk = lane >> 4, nt = (n + 15) / 16;
-    for (int t = wave; t < nt * nt; t += nw) {
-        const int i0 = (t % nt) * 16, j0 = (t / nt) * 16;
-        const int ia = min(i0 + lr, n - 1), jb = min(j0 + lr, n - 1);
-        const bool va = i0 + lr < n, vb = j0 + lr < n;
-        typename M::v4 acc = {(T)0, (T)0, (T)0, (T)0};
+    const int lr = lane & 15, lk = lane >> 4, nt = (n + 15) / 16, ntt = nt * nt;
+    // four of the wave's tiles at a time (t = tb + nw u), their MFMA chains
+    // interleaved; each tile's k order is that of one tile alone
+    constexpr int U = 4;
+    for (int tb = wave; tb < ntt; tb += U * nw) {
+        int ia[U], jb[U], i0[U], j0[U];
+        bool va[U], vb[U], tv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t0 = tb + nw * u;
+            tv[u] = t0 < ntt;
+            const int t = tv[u] ? t0 : tb;
+            i0[u] = (t % nt) * 16;
+            j0[u] = (t / nt) * 16;
+            ia[u] = min(i0[u] + lr, n - 1);
+            jb[u] = min(j0[u] + lr, n - 1);
+            va[u] = i0[u] + lr < n;
+            vb[u] = j0[u] + lr < n;
+        }
+        typename M::v4 acc[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) acc[u] = (typename M::v4){(T)0, (T)0, (T)0, (T)0};
         for (int k0 = 0; k0 < n; k0 += 4) {
             const int k = k0 + lk, kc = min(k, n - 1);
-            const T a = (va && k < n) ? A[ia + n * kc] : (T)0;
-            const T b = (vb && k < n) ? B[kc + n * jb] : (T)0;
-            acc = M::run(a, b, acc);
+            T a[U], b[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                a[u] = (va[u] && k < n) ? A[ia[u] + n * kc] : (T)0;
+                b[u] = (vb[u] && k < n) ? B[kc + n * jb[u]] : (T)0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) acc[u] = M::run(a[u], b[u], acc[u]);
         }
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int i = i0 + M::row(lk, v), j = j0 + lr;
-            if (i < n && j < n) out(i, j, acc[v]);
-        }
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int i = i0[u] + M::row(lk, v), j = j0[u] + lr;
+                if (tv[u] && i < n && j < n) out(i, j, acc[u][v]);
+            }
     }
 }
 
