@@ -12,6 +12,7 @@
 namespace hfem {
 
 constexpr int SV_THREADS = 256;
+constexpr int ELIM_BATCH = 8;   // block-inverse entries a thread updates per batch
 
 // In-place Gauss-Jordan inverse of an n x n column-major block in LDS with
 // partial pivoting: rows are interchanged as pivots are chosen and the columns of
@@ -50,13 +51,31 @@ __device__ int lds_inverse(T *A, int n, int *piv, T *rowk, T *colk, T *red)
         for (int i = tid; i < n; i += nth) colk[i] = A[i + n * k];
         __syncthreads();
         const T pinv = (T)1 / rowk[k];
-        for (int idx = tid; idx < n * n; idx += nth) {
-            const int i = idx % n, j = idx / n;
-            T v;
-            if (i == k) v = (j == k) ? pinv : rowk[j] * pinv;
-            else if (j == k) v = -colk[i] * pinv;
-            else v = A[idx] - colk[i] * (rowk[j] * pinv);
-            A[idx] = v;
+        // a thread's entries idx = tid + u * nth in batches of ELIM_BATCH: every read
+        // of a batch is issued before its first write, so the LDS latency is paid
+        // once per batch rather than once per entry
+        for (int base = tid; base < n * n; base += ELIM_BATCH * nth) {
+            T a[ELIM_BATCH], c[ELIM_BATCH], rw[ELIM_BATCH];
+#pragma unroll
+            for (int u = 0; u < ELIM_BATCH; u++) {
+                const int idx = base + u * nth;
+                const int j = idx / n, i = idx - j * n;
+                const bool in = idx < n * n;
+                a[u] = in ? A[idx] : (T)0;
+                c[u] = in ? colk[i] : (T)0;
+                rw[u] = in ? rowk[j] : (T)0;
+            }
+#pragma unroll
+            for (int u = 0; u < ELIM_BATCH; u++) {
+                const int idx = base + u * nth;
+                if (idx >= n * n) continue;
+                const int j = idx / n, i = idx - j * n;
+                T v;
+                if (i == k) v = (j == k) ? pinv : rw[u] * pinv;
+                else if (j == k) v = -c[u] * pinv;
+                else v = a[u] - c[u] * (rw[u] * pinv);
+                A[idx] = v;
+            }
         }
         __syncthreads();
     }
