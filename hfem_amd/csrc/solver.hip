@@ -56,25 +56,27 @@ __device__ int lds_inverse(T *A, int n, int *piv, T *rowk, T *colk, T *red)
         // once per batch rather than once per entry
         for (int base = tid; base < n * n; base += ELIM_BATCH * nth) {
             T a[ELIM_BATCH], c[ELIM_BATCH], rw[ELIM_BATCH];
+            bool rk[ELIM_BATCH], ck[ELIM_BATCH];
 #pragma unroll
             for (int u = 0; u < ELIM_BATCH; u++) {
-                const int idx = base + u * nth;
+                // (an entry past the block reads entry 0's operands and is not stored)
+                const int idx = base + u * nth < n * n ? base + u * nth : 0;
                 const int j = idx / n, i = idx - j * n;
-                const bool in = idx < n * n;
-                a[u] = in ? A[idx] : (T)0;
-                c[u] = in ? colk[i] : (T)0;
-                rw[u] = in ? rowk[j] : (T)0;
+                rk[u] = i == k;
+                ck[u] = j == k;
+                a[u] = A[idx];
+                c[u] = colk[i];
+                rw[u] = rowk[j];
             }
+            // the three cases computed alike and selected (no branches per entry)
 #pragma unroll
             for (int u = 0; u < ELIM_BATCH; u++) {
                 const int idx = base + u * nth;
-                if (idx >= n * n) continue;
-                const int j = idx / n, i = idx - j * n;
-                T v;
-                if (i == k) v = (j == k) ? pinv : rw[u] * pinv;
-                else if (j == k) v = -c[u] * pinv;
-                else v = a[u] - c[u] * (rw[u] * pinv);
-                A[idx] = v;
+                const T rp = rw[u] * pinv;
+                const T other = a[u] - c[u] * rp;
+                const T colv = -c[u] * pinv;
+                const T v = rk[u] ? (ck[u] ? pinv : rp) : (ck[u] ? colv : other);
+                if (idx < n * n) A[idx] = v;
             }
         }
         __syncthreads();
