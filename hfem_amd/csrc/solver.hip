@@ -14,6 +14,35 @@ namespace hfem {
 constexpr int SV_THREADS = 256;
 constexpr int ELIM_BATCH = 8;   // block-inverse entries a thread updates per batch
 
+// A lane's value from lane l - s (DPP control CTRL, rows in ROWS); lanes without
+// a source keep idv.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_take(double idv, double v)
+{
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(idv), __double2hiint(v), CTRL, ROWS, 0xf, false);
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(idv), __double2loint(v), CTRL, ROWS, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_take(float idv, float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(idv), __float_as_int(v), CTRL, ROWS, 0xf, false));
+}
+
+// One step of the pivot choice's scan: (b, i) becomes the better of itself and
+// the pair DPP brings in -- the larger value, the smaller row on ties -- with
+// (-1, k) where no pair comes in (the search's starting pair: no row of column k
+// has a magnitude below 0, and a NaN never replaces it).
+template <int CTRL, int ROWS, typename T>
+__device__ __forceinline__ void argmax_step(T &b, int &i, int k)
+{
+    const T yb = dpp_take<CTRL, ROWS>((T)-1, b);
+    const int yi = __builtin_amdgcn_update_dpp(k, i, CTRL, ROWS, 0xf, false);
+    const bool t = yb > b || (yb == b && yi < i);
+    b = t ? yb : b;
+    i = t ? yi : i;
+}
+
 // In-place Gauss-Jordan inverse of an n x n column-major block in LDS with
 // partial pivoting: rows are interchanged as pivots are chosen and the columns of
 // the result unscrambled in reverse order at the end.  All threads of the
@@ -32,12 +61,16 @@ __device__ int lds_inverse(T *A, int n, int *piv, T *rowk, T *colk, T *red)
                 const T v = fabs(A[i + n * k]);
                 if (v > best) { best = v; bi = i; }
             }
-            for (int off = 32; off > 0; off >>= 1) {
-                const T ob = __shfl_down(best, off);
-                const int oi = __shfl_down(bi, off);
-                if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
-            }
-            if (tid == 0) { piv[k] = bi; red[0] = best; }
+            // inclusive scan over the wave by DPP: shifts by 1, 2, 4, 8 within each
+            // row of 16 lanes, then row 0's total into row 1 and row 2's into row 3,
+            // then lanes 0-31's total into rows 2 and 3: lane 63 ends with the choice
+            argmax_step<0x111, 0xf>(best, bi, k);   // row_shr:1
+            argmax_step<0x112, 0xf>(best, bi, k);   // row_shr:2
+            argmax_step<0x114, 0xf>(best, bi, k);   // row_shr:4
+            argmax_step<0x118, 0xf>(best, bi, k);   // row_shr:8
+            argmax_step<0x142, 0xa>(best, bi, k);   // row_bcast:15
+            argmax_step<0x143, 0xc>(best, bi, k);   // row_bcast:31
+            if (tid == 63) { piv[k] = bi; red[0] = best; }
         }
         __syncthreads();
         if (red[0] == (T)0) return k + 1;
