@@ -48,8 +48,12 @@ __device__ __forceinline__ void argmax_step(T &b, int &i, int k)
 // the result unscrambled in reverse order at the end.  All threads of the
 // workgroup call it; piv (n ints), rowk/colk (n values each) and red (1 value)
 // are LDS; T is the arithmetic (double, or float for FP32 preconditioner factors).
+// LATENCY picks the elimination for an inverse on a dependency chain (batched
+// reads, no branches per entry: the BILU0 factorisations) over the one for many
+// independent blocks per CU (one entry at a time, fewer registers: the block-Jacobi
+// and MDF inverses); both compute the same values.
 // Returns 0, or 1 + the step of the first zero pivot (workgroup-uniform).
-template <typename T>
+template <typename T, bool LATENCY = true>
 __device__ int lds_inverse(T *A, int n, int *piv, T *rowk, T *colk, T *red)
 {
     const int tid = threadIdx.x, nth = blockDim.x;
@@ -84,10 +88,21 @@ __device__ int lds_inverse(T *A, int n, int *piv, T *rowk, T *colk, T *red)
         for (int i = tid; i < n; i += nth) colk[i] = A[i + n * k];
         __syncthreads();
         const T pinv = (T)1 / rowk[k];
+        if (!LATENCY) {
+            // many blocks per CU (throughput): one entry at a time, fewer registers
+            for (int idx = tid; idx < n * n; idx += nth) {
+                const int j = idx / n, i = idx - j * n;
+                T v;
+                if (i == k) v = (j == k) ? pinv : rowk[j] * pinv;
+                else if (j == k) v = -colk[i] * pinv;
+                else v = A[idx] - colk[i] * (rowk[j] * pinv);
+                A[idx] = v;
+            }
+        }
         // a thread's entries idx = tid + u * nth in batches of ELIM_BATCH: every read
         // of a batch is issued before its first write, so the LDS latency is paid
         // once per batch rather than once per entry
-        for (int base = tid; base < n * n; base += ELIM_BATCH * nth) {
+        for (int base = tid; LATENCY && base < n * n; base += ELIM_BATCH * nth) {
             T a[ELIM_BATCH], c[ELIM_BATCH], rw[ELIM_BATCH];
             bool rk[ELIM_BATCH], ck[ELIM_BATCH];
 #pragma unroll
@@ -160,7 +175,7 @@ block_inverse_kernel(int nrows, int bsz, const int *__restrict__ rowpts, const d
         const double *src = Hg + (long)rowpts[row] * b2;
         for (long i = tid; i < b2; i += nth) A[i] = src[i];
         __syncthreads();
-        const int r = lds_inverse(A, bsz, w.piv, w.rowk, w.colk, w.red);
+        const int r = lds_inverse<double, false>(A, bsz, w.piv, w.rowk, w.colk, w.red);
         for (long i = tid; i < b2; i += nth) Dinv[row * b2 + i] = A[i];
         if (tid == 0 && info) info[row] = r;
         __syncthreads();
