@@ -57,6 +57,9 @@ template <typename T, bool LATENCY = true>
 __device__ int lds_inverse(T *A, int n, int *piv, T *rowk, T *colk, T *red)
 {
     const int tid = threadIdx.x, nth = blockDim.x;
+    // idx / n as a multiply-high by ceil(2^32 / n): exact for every idx < n * n
+    // with n <= 128 (checked exhaustively); n = 1 gives 0, and idx is 0 there
+    const unsigned ndiv = (unsigned)((0x100000000ull + (unsigned)n - 1) / (unsigned)n);
     for (int k = 0; k < n; k++) {
         if (tid < 64) {
             T best = (T)-1;
@@ -109,7 +112,7 @@ __device__ int lds_inverse(T *A, int n, int *piv, T *rowk, T *colk, T *red)
             for (int u = 0; u < ELIM_BATCH; u++) {
                 // (an entry past the block reads entry 0's operands and is not stored)
                 const int idx = base + u * nth < n * n ? base + u * nth : 0;
-                const int j = idx / n, i = idx - j * n;
+                const int j = (int)__umulhi((unsigned)idx, ndiv), i = idx - j * n;
                 rk[u] = i == k;
                 ck[u] = j == k;
                 a[u] = A[idx];
